@@ -1,0 +1,87 @@
+// hnt_internal.h -- context, device-buffer cache and error plumbing of libhuntian_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/huntian_abi.h"
+#include "hnt_math.h"
+
+namespace hnt {
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// Device copy of the shell tables, with the host copy used to detect a change.
+struct TablesDev {
+    int32_t rmax = -1, nshells = 0;
+    int64_t ncells = 0;
+    std::vector<int32_t> host;  // di, dj, dk, start, count, r2 concatenated
+    DevBuf cells;               // int4 {di, dj, dk, 0} per cell
+    DevBuf start, count, r2;    // int32 [nshells]
+    DevBuf rad;                 // double [nshells + 1], rad[s] = sqrt(r2 of shell s), 1-based
+};
+
+struct CTabDev {
+    int32_t nx = 0, ny = 0, nz = 0;
+    std::vector<float> host;
+    DevBuf table;
+};
+
+}  // namespace hnt
+
+struct hnt_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    // buffers reused across calls (grow only)
+    hnt::DevBuf recs, claim, counts, offsets, cub_tmp, keys_a, keys_b, vals_a, vals_b, prof,
+        queue, fieldptrs, fcrit, akk, small;
+    hnt::TablesDev tables;
+    hnt::CTabDev ctab;
+};
+
+namespace hnt {
+
+hnt_status reserve(hnt_ctx* ctx, DevBuf& b, size_t bytes);
+hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t);
+hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
+hnt_status ensure_akk(hnt_ctx* ctx);
+hnt_status begin_call(hnt_ctx* ctx);
+hnt_status finish_call(hnt_ctx* ctx, const char* what);
+
+}  // namespace hnt
+
+#define HNT_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            hnt::set_error("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,       \
+                           __LINE__);                                                      \
+            return HNT_ERR_DEVICE;                                                         \
+        }                                                                                  \
+    } while (0)
+
+#define HNT_TRY(expr)                       \
+    do {                                    \
+        hnt_status s_ = (expr);             \
+        if (s_ != HNT_OK) return s_;        \
+    } while (0)
+
+#define HNT_REQUIRE(cond, ...)              \
+    do {                                    \
+        if (!(cond)) {                      \
+            hnt::set_error(__VA_ARGS__);    \
+            return HNT_ERR_INVALID;         \
+        }                                   \
+    } while (0)

@@ -1,0 +1,969 @@
+// shell_analyse.hip -- radial-shell homogeneous-ellipsoid collapse of every peak candidate.
+//
+// Replaces _shell_gather_full_kernel! + _post_process_kernel! and their launchers
+// analyse_peaks_gpu_cuda_multirf / analyse_peak_gpu_cuda (ext/CUDAExt.jl:752-924,
+// 1213-1912, 1914-2344).  Semantics follow the FP64 CPU path: RadialShell.analyse_peak
+// (src/HaloFinder/RadialShell.jl:281-865, variant A) and ShellAnalysisGPU.analyse_peak_gpu
+// (src/ShellAnalysisGPU.jl:81-535, variants B and C), with the same operation order, so
+// the results equal the FP64 CPU restatement bit for bit except where log10 / pow differ
+// by an ulp between the device and host math libraries.
+//
+// Design (MI355X):
+//  * repack_kernel turns the caller's eight column-major fields into one 32-byte record
+//    per cell {delta, delta^T, psi1 xyz, psi2 xyz}: a cell visit is two 16-byte loads from
+//    one cache line instead of eight scattered 4-byte loads.  delta^T[i,j,k] = delta[k,i,j]
+//    (the transposed read of the reference, RadialShell.jl:441) goes through an LDS-tiled
+//    transpose so both the reads and the writes stay coalesced.
+//  * Peaks are sorted by (ir2min, Morton code of the cell): a wavefront holds 64 peaks of
+//    one filter scale that sit close together, so their shell balls overlap in L2.
+//  * One lane per peak.  All 64 lanes of a wavefront walk the same static shell table in
+//    lock step (the offset table is read with scalar loads), each gathering its own peak's
+//    cells: no cross-lane reductions, and each lane sums its shell in the reference's cell
+//    order, in FP64 with the Float32 products the reference forms.  Per-shell profiles go
+//    to a per-lane slot of scratch memory (L2-resident while the lane post-processes).
+//  * A lane stops gathering once its profile can no longer change any result (first
+//    shell past ir2min with Fbar < fcrit, plus 3 cells, and past the filter-scale
+//    gradient window); the wave stops when its last lane stops.
+//  * Persistent grid: one wavefront per workgroup, as many as are resident; eight work
+//    queues, one per XCD (workgroup i runs on XCD i % 8), each owning a contiguous eighth
+//    of the sorted peak list, so neighbouring peaks share an XCD's L2.  A wave that
+//    drains its queue moves on to the next queue; every wave exits once all eight are
+//    empty.
+#include <hipcub/hipcub.hpp>
+
+#include "hnt_internal.h"
+
+namespace {
+
+using namespace hnt;
+
+constexpr int PW = 24;  // doubles per shell in a profile slot
+// [0..2] G/rad  [3..5] Gf/rad  [6..11] SR+SR^T over rad: 11 12 13 22 23 33
+// [12] Fbar  [13] cumulative n  [14..16] cumulative S  [17..19] cumulative S2
+// [20] running lapd sum  [21] running lapd count
+constexpr int K_F = 12, K_N = 13, K_S = 14, K_S2 = 17, K_LAP = 20, K_LAPN = 21;
+
+struct SAArgs {
+    const float4* recs;
+    const float* lapd;
+    int n1, n2, n3;
+    int64_t s1, s2;
+    const int32_t* pi;
+    const int32_t* pj;
+    const int32_t* pk;
+    const int32_t* ir2min;
+    const double* Rf;
+    const double* ZZon;
+    const double* fcrit;
+    double ZZon_s, fcrit_s;
+    const int32_t* order;
+    int64_t npeaks;
+    const int4* cells;
+    const int32_t* shell_start;
+    const int32_t* shell_count;
+    const int32_t* shell_r2;
+    const double* rad;  // 1-based
+    int NS, rmax;
+    CTabView ct;
+    const double* akk;
+    double alatt, wRnor, aRnor;
+    int nbuff, variant, early, has_eta2, has_lapd;
+    double* prof;
+    int64_t slot_doubles;
+    unsigned long long* qctr;
+    int64_t qbeg[8], qend[8];
+    hnt_results res;
+    int8_t* mask;
+};
+
+// ------------------------------------------------------------------ repack
+
+__global__ __launch_bounds__(256) void repack_kernel(const float* __restrict__ d,
+                                                     const float* __restrict__ ex,
+                                                     const float* __restrict__ ey,
+                                                     const float* __restrict__ ez,
+                                                     const float* __restrict__ e2x,
+                                                     const float* __restrict__ e2y,
+                                                     const float* __restrict__ e2z, int n,
+                                                     float4* __restrict__ recs) {
+    __shared__ float tile[64][65];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64, j = blockIdx.z;
+    const int64_t nn = n, n2 = (int64_t)n * n;
+    // tile[ii][kk] = delta[k0+kk, i0+ii, j]  (delta^T at (i0+ii, j, k0+kk))
+    for (int r = 0; r < 16; ++r) {
+        int ii = (t >> 6) + 4 * r, kk = t & 63;
+        if (i0 + ii < n && k0 + kk < n) tile[ii][kk] = d[(k0 + kk) + (i0 + ii) * nn + j * n2];
+    }
+    __syncthreads();
+    for (int r = 0; r < 16; ++r) {
+        int ii = t & 63, kk = (t >> 6) + 4 * r;
+        int i = i0 + ii, k = k0 + kk;
+        if (i < n && k < n) {
+            int64_t c = i + j * nn + k * n2;
+            float4 a = make_float4(d[c], tile[ii][kk], ex[c], ey[c]);
+            float4 b = make_float4(ez[c], e2x ? e2x[c] : 0.0f, e2y ? e2y[c] : 0.0f,
+                                   e2z ? e2z[c] : 0.0f);
+            recs[2 * c] = a;
+            recs[2 * c + 1] = b;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ sort keys
+
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+
+__global__ void key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+                           const int32_t* __restrict__ pk, const int32_t* __restrict__ ir2min,
+                           int64_t np, uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    uint64_t mort = spread3((uint32_t)(pi[q] - 1) & 0x1fff) |
+                    (spread3((uint32_t)(pj[q] - 1) & 0x1fff) << 1) |
+                    (spread3((uint32_t)(pk[q] - 1) & 0x1fff) << 2);
+    uint64_t g = (uint64_t)min(max(ir2min[q], 0), (1 << 24) - 1);
+    keys[q] = (g << 39) | mort;
+    vals[q] = (int32_t)q;
+}
+
+// Flags peaks outside the grid or with ZZon <= -1 (bit 0 / bit 1 of *bad).
+__global__ void validate_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+                                const int32_t* __restrict__ pk, const double* __restrict__ ZZon,
+                                int64_t np, int n, unsigned int* bad) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    unsigned int b = 0;
+    if (pi[q] < 1 || pi[q] > n || pj[q] < 1 || pj[q] > n || pk[q] < 1 || pk[q] > n) b |= 1u;
+    if (ZZon && !(ZZon[q] > -1.0)) b |= 2u;
+    if (b) atomicOr(bad, b);
+}
+
+__global__ void fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+// ------------------------------------------------------------------ device helpers
+
+struct KS {
+    double E11, E12, E13, E22, E23, E33, g0, g1, g2, f0, f1, f2;
+};
+
+// kernel_strain (RadialShell.jl:225-274) over the lane's profile slot P.
+__device__ KS kstrain(const double* __restrict__ rad, const double* __restrict__ P,
+                      const double* __restrict__ akk, int mp, int mlow, int mupp, double wRnor,
+                      double aRnor) {
+    KS r = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (mp > 1) {
+        const double rmp = rad[mp];
+        for (int m1 = mlow; m1 <= mupp; ++m1) {
+            const double rm1 = rad[m1];
+            const double u0 = rmp - rm1;
+            const double u02 = u0 * u0;
+            if (u02 < 4.0) {
+                const double u12 = rmp * rmp + rm1 * rm1;
+                const double dd = u12 - u02;
+                const double wt = hRinteg(u0, u12) / (dd * dd);
+                const double* k = P + (int64_t)m1 * PW;
+                const double hw = 0.5 * wt;
+                r.g0 += wt * k[0];
+                r.g1 += wt * k[1];
+                r.g2 += wt * k[2];
+                r.f0 += wt * k[3];
+                r.f1 += wt * k[4];
+                r.f2 += wt * k[5];
+                r.E11 -= hw * k[6];
+                r.E12 -= hw * k[7];
+                r.E13 -= hw * k[8];
+                r.E22 -= hw * k[9];
+                r.E23 -= hw * k[10];
+                r.E33 -= hw * k[11];
+            }
+        }
+        const double sc = wRnor / rmp;
+        r.E11 *= sc; r.E12 *= sc; r.E13 *= sc; r.E22 *= sc; r.E23 *= sc; r.E33 *= sc;
+        r.g0 *= sc; r.g1 *= sc; r.g2 *= sc; r.f0 *= sc; r.f1 *= sc; r.f2 *= sc;
+    } else {
+        for (int m1 = mlow; m1 <= mupp; ++m1) {
+            const double rm1 = rad[m1];
+            const double con = 100.0 * rm1 * rm1;
+            const double fl = floor(con);
+            if (!(fl >= 0.0 && fl < 1100.0)) continue;
+            const int icon = (int)fl;
+            const double diff = con - (double)icon;
+            const double aww = akk[icon] + diff * (akk[icon + 1] - akk[icon]);
+            const double wt = 0.5 * aww / rm1;
+            const double* k = P + (int64_t)m1 * PW;
+            const double hw = 0.5 * wt;
+            r.g0 += wt * k[0];
+            r.g1 += wt * k[1];
+            r.g2 += wt * k[2];
+            r.f0 += wt * k[3];
+            r.f1 += wt * k[4];
+            r.f2 += wt * k[5];
+            r.E11 -= hw * k[6];
+            r.E12 -= hw * k[7];
+            r.E13 -= hw * k[8];
+            r.E22 -= hw * k[9];
+            r.E23 -= hw * k[10];
+            r.E33 -= hw * k[11];
+        }
+        r.E11 *= aRnor; r.E12 *= aRnor; r.E13 *= aRnor; r.E22 *= aRnor; r.E23 *= aRnor;
+        r.E33 *= aRnor;
+        r.g0 *= aRnor; r.g1 *= aRnor; r.g2 *= aRnor; r.f0 *= aRnor; r.f1 *= aRnor;
+        r.f2 *= aRnor;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void eig_ks(const KS& k, double lam[3]) {
+    eig3_sym(k.E11, k.E12, k.E13, k.E22, k.E23, k.E33, lam);
+}
+
+// first index s in [lo, hi] with r2[s-1] > x (shells are 1-based; r2 is 0-based)
+__device__ __forceinline__ int first_r2_gt(const int32_t* r2, int lo, int hi, int x) {
+    int a = lo, b = hi + 1;
+    while (a < b) {
+        int mid = (a + b) >> 1;
+        if (r2[mid - 1] > x) b = mid;
+        else a = mid + 1;
+    }
+    return a;
+}
+
+// shell whose r^2 is closest to rf2, first one on ties, within s in [lo, hi]
+__device__ __forceinline__ int closest_shell(const int32_t* r2, int lo, int hi, double rf2,
+                                             double* dist) {
+    int a = lo, b = hi + 1;  // first s with (double)r2 >= rf2
+    while (a < b) {
+        int mid = (a + b) >> 1;
+        if ((double)r2[mid - 1] >= rf2) b = mid;
+        else a = mid + 1;
+    }
+    int best = -1;
+    double bd = 0.0;
+    if (a - 1 >= lo) {
+        best = a - 1;
+        bd = fabs((double)r2[a - 2] - rf2);
+    }
+    if (a <= hi) {
+        double d = fabs((double)r2[a - 1] - rf2);
+        if (best < 0 || d < bd) {
+            best = a;
+            bd = d;
+        }
+    }
+    *dist = bd;
+    return best;
+}
+
+struct Acc {
+    double F, S0, S1, S2, T0, T1, T2, G0, G1, G2, H0, H1, H2;
+    double R00, R01, R02, R10, R11, R12, R20, R21, R22;
+    double lap;
+    int64_t lapn, nin;
+};
+
+__device__ __forceinline__ void acc_cell(Acc& A, const float4 v0, const float4 v1, float fx,
+                                         float fy, float fz, bool eta2) {
+    A.F += (double)v0.x;
+    A.S0 += (double)v0.z;
+    A.S1 += (double)v0.w;
+    A.S2 += (double)v1.x;
+    if (eta2) {
+        A.T0 += (double)v1.y;
+        A.T1 += (double)v1.z;
+        A.T2 += (double)v1.w;
+    }
+    // Float32 products (Float32 * Int32 is Float32 in the reference), summed in FP64
+    A.G0 += (double)(v0.x * fx);
+    A.G1 += (double)(v0.x * fy);
+    A.G2 += (double)(v0.x * fz);
+    A.H0 += (double)(v0.y * fx);
+    A.H1 += (double)(v0.y * fy);
+    A.H2 += (double)(v0.y * fz);
+    A.R00 += (double)(v0.z * fx);
+    A.R01 += (double)(v0.z * fy);
+    A.R02 += (double)(v0.z * fz);
+    A.R10 += (double)(v0.w * fx);
+    A.R11 += (double)(v0.w * fy);
+    A.R12 += (double)(v0.w * fz);
+    A.R20 += (double)(v1.x * fx);
+    A.R21 += (double)(v1.x * fy);
+    A.R22 += (double)(v1.x * fz);
+}
+
+// Gather one shell for one lane.  CHECK = the wave holds a peak whose ball leaves the grid.
+template <bool CHECK>
+__device__ __forceinline__ void gather_shell(const SAArgs& a, int st, int nc, int ci, int cj,
+                                             int ck, int64_t c0, Acc& A, bool& oobA) {
+    const bool eta2 = a.has_eta2;
+    const bool lapd = a.has_lapd;
+    const bool varA = a.variant == HNT_VARIANT_A;
+    if (!CHECK) {
+        int c = 0;
+        for (; c + 4 <= nc; c += 4) {
+            int4 o[4];
+            float4 v0[4], v1[4];
+            float lv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                o[u] = a.cells[st + c + u];
+                const int64_t off = c0 + o[u].x + o[u].y * a.s1 + o[u].z * a.s2;
+                v0[u] = a.recs[2 * off];
+                v1[u] = a.recs[2 * off + 1];
+                lv[u] = lapd ? a.lapd[off] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc_cell(A, v0[u], v1[u], (float)o[u].x, (float)o[u].y, (float)o[u].z, eta2);
+                if (lapd) {
+                    A.lap += (double)lv[u];
+                    A.lapn += 1;
+                }
+            }
+        }
+        for (; c < nc; ++c) {
+            const int4 o = a.cells[st + c];
+            const int64_t off = c0 + o.x + o.y * a.s1 + o.z * a.s2;
+            const float4 v0 = a.recs[2 * off], v1 = a.recs[2 * off + 1];
+            acc_cell(A, v0, v1, (float)o.x, (float)o.y, (float)o.z, eta2);
+            if (lapd) {
+                A.lap += (double)a.lapd[off];
+                A.lapn += 1;
+            }
+        }
+        A.nin += nc;
+    } else {
+        for (int c = 0; c < nc; ++c) {
+            const int4 o = a.cells[st + c];
+            const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
+            if (i1 < 1 || i1 > a.n1 || i2 < 1 || i2 > a.n2 || i3 < 1 || i3 > a.n3) {
+                oobA = true;
+                continue;
+            }
+            const int64_t off = c0 + o.x + o.y * a.s1 + o.z * a.s2;
+            const float4 v0 = a.recs[2 * off], v1 = a.recs[2 * off + 1];
+            acc_cell(A, v0, v1, (float)o.x, (float)o.y, (float)o.z, eta2);
+            A.nin += 1;
+            if (lapd && (!varA || !oobA)) {
+                A.lap += (double)a.lapd[off];
+                A.lapn += 1;
+            }
+        }
+    }
+}
+
+__device__ void write_no_collapse(const SAArgs& a, int p) {
+    const hnt_results& r = a.res;
+    r.RTHL[p] = -1.0;
+    r.Fbarx[p] = 0.0;
+    r.e_v[p] = 0.0;
+    r.p_v[p] = 0.0;
+    r.Srb[p] = 0.0;
+    r.d2F[p] = 0.0;
+    r.zvir_half[p] = -1.0;
+    for (int q = 0; q < 9; ++q) r.strain[(int64_t)p * 9 + q] = 0.0;
+    for (int L = 0; L < 3; ++L) {
+        const int64_t o = (int64_t)p * 3 + L;
+        r.eigs[o] = 0.0;
+        r.Sbar[o] = 0.0;
+        r.Sbar2[o] = 0.0;
+        r.gradpk[o] = 0.0;
+        r.gradpkf[o] = 0.0;
+        r.gradpkrf[o] = 0.0;
+    }
+}
+
+// The radial logic of one peak from its profile slot P (RadialShell.jl:449-865 for A,
+// ShellAnalysisGPU.jl:231-535 for B; C = B + zvir_half of A).
+__device__ void post_process(const SAArgs& a, int p, const double* __restrict__ P, int ci, int cj,
+                             int ck, double ZZon, double fcrit, int ir2min, int mrf,
+                             int mlow_rf, int mupp_rf) {
+    const double* __restrict__ rad = a.rad;
+    const int NS = a.NS;
+    const int m = NS - 1;
+    const int variant = a.variant;
+    auto Fb = [&](int s) -> double { return P[(int64_t)s * PW + K_F]; };
+
+    const KS krf = kstrain(rad, P, a.akk, mrf, mlow_rf, mupp_rf, a.wRnor, a.aRnor);
+
+    // m0: first shell with r^2 > ir2min, else the last finalised shell
+    int m0 = first_r2_gt(a.shell_r2, 2, m, ir2min);
+    if (m0 > m) m0 = m;
+
+    int mupp;
+    if (Fb(m0) >= fcrit) {
+        int found = 0;
+        for (int mm = m0; mm <= m; ++mm)
+            if (Fb(mm) < fcrit) {
+                m0 = mm;
+                found = 1;
+                break;
+            }
+        if (!found) {
+            mupp = m;
+        } else {
+            mupp = first_ge(rad, m0 + 1, m, rad[m0] + 2.0) - 1;
+        }
+    } else {
+        int mstart = m0 - 1 > 1 ? m0 - 1 : 1;
+        int found = 0;
+        for (int mp = mstart; mp >= 1; --mp) {
+            if (Fb(mp) >= fcrit) {
+                found = 1;
+                break;
+            }
+            m0 = mp;
+        }
+        if (!found) {
+            write_no_collapse(a, p);
+            return;
+        }
+        mupp = first_ge(rad, m0 + 1, m, rad[m0] + 2.0) - 1;
+    }
+
+    double rlow = rad[m0] - 2.0;
+    if (rlow < 0.0) rlow = 0.0;
+    int mlow = first_gt(rad, 1, m0 - 1, rlow);
+    if (mlow > m0 - 1) mlow = m0;
+
+    const KS k0 = kstrain(rad, P, a.akk, m0, mlow, mupp, a.wRnor, a.aRnor);
+    double lam[3], Fr, ev, pv;
+    eig_ks(k0, lam);
+    ev_from_lam(lam, &Fr, &ev, &pv);
+    double zvir1 = -1.0, zvir1p = -1.0;
+    double Frhoh = Fr;
+    KS last = k0, prev = k0;  // Ebar_last/gradpk_last and strain_mat/gradpk_prev
+    if (m0 == 1) {
+        write_no_collapse(a, p);
+        return;
+    }
+    bool collapsed = false;
+    for (int mp = m0 - 1; mp >= 1; --mp) {
+        const double rupp_mp = rad[mp] + 2.0;
+        double rlow_mp = rad[mp] - 2.0;
+        if (rlow_mp < 0.0) rlow_mp = 0.0;
+        if (rad[mupp] > rupp_mp) {
+            int mupp_new = mp;
+            for (int m1 = mp + 1; m1 <= mupp; ++m1)
+                if (rad[m1] < rupp_mp) mupp_new = m1;
+            mupp = mupp_new;
+        }
+        if (mlow > 1 && rad[mlow - 1] > rlow_mp) {
+            int mlownew = mlow;
+            for (int m1 = mlow - 1; m1 >= 1; --m1)
+                if (rad[m1] > rlow_mp) mlownew = m1;
+            mlow = mlownew;
+        }
+        const KS kk = kstrain(rad, P, a.akk, mp, mlow, mupp, a.wRnor, a.aRnor);
+        double e_mp, p_mp, Fr_mp;
+        eig_ks(kk, lam);
+        ev_from_lam(lam, &Fr_mp, &e_mp, &p_mp);
+        const double Frho_mp = Fb(mp);
+        double zv = -1.0;
+        if (Frho_mp > 0.0) {
+            const double poe = e_mp < 1e-5 ? 0.0 : p_mp / e_mp;
+            zv = interp3(a.ct, log10(Frho_mp), e_mp, poe);
+        }
+        if (zv >= ZZon) {
+            m0 = mp + 1;
+            zvir1p = zv;
+            last = kk;
+            Frhoh = Fr_mp;
+            collapsed = true;
+            break;
+        }
+        zvir1 = zv;
+        prev = kk;
+        Frhoh = Fr_mp;
+        last = kk;
+    }
+    if (!collapsed) {
+        write_no_collapse(a, p);
+        return;
+    }
+
+    const double dZvir = zvir1p - zvir1;
+    double RTHL;
+    if (zvir1 > 0.0 && dZvir != 0.0) {
+        const double RTHL3 =
+            cube(rad[m0 - 1]) + (cube(rad[m0]) - cube(rad[m0 - 1])) * (zvir1p - ZZon) / dZvir;
+        RTHL = pow(RTHL3, 1.0 / 3.0);
+    } else {
+        RTHL = rad[m0 - 1];
+    }
+    if (RTHL <= 0.0) {
+        write_no_collapse(a, p);
+        return;
+    }
+    const double RTHL3 = cube(RTHL);
+    const double RTHL5 = RTHL3 * RTHL * RTHL;
+    const double rad3p = cube(rad[m0 - 1]);
+    const double rad3 = cube(rad[m0]);
+    const double drad3 = rad3 - rad3p;
+    const double dFb = Fb(m0) - Fb(m0 - 1);
+    double Fbarx;
+    double S11, S12, S13, S22, S23, S33, gk0, gk1, gk2, gf0, gf1, gf2;
+    if (zvir1 > 0.0 && drad3 != 0.0) {
+        const double frac = (RTHL3 - rad3p) / drad3;
+        Fbarx = Fb(m0 - 1) + frac * dFb;
+        S11 = last.E11 + frac * (prev.E11 - last.E11);
+        S12 = last.E12 + frac * (prev.E12 - last.E12);
+        S13 = last.E13 + frac * (prev.E13 - last.E13);
+        S22 = last.E22 + frac * (prev.E22 - last.E22);
+        S23 = last.E23 + frac * (prev.E23 - last.E23);
+        S33 = last.E33 + frac * (prev.E33 - last.E33);
+        gk0 = last.g0 + frac * (prev.g0 - last.g0);
+        gk1 = last.g1 + frac * (prev.g1 - last.g1);
+        gk2 = last.g2 + frac * (prev.g2 - last.g2);
+        gf0 = last.f0 + frac * (prev.f0 - last.f0);
+        gf1 = last.f1 + frac * (prev.f1 - last.f1);
+        gf2 = last.f2 + frac * (prev.f2 - last.f2);
+    } else {
+        Fbarx = Fb(m0 - 1);
+        S11 = last.E11; S12 = last.E12; S13 = last.E13;
+        S22 = last.E22; S23 = last.E23; S33 = last.E33;
+        gk0 = prev.g0; gk1 = prev.g1; gk2 = prev.g2;
+        gf0 = prev.f0; gf1 = prev.f1; gf2 = prev.f2;
+    }
+    // normalize_strain (RadialShell.jl:207-210) and the final eigenvalues
+    const double tr = S11 + S22 + S33;
+    if (tr != 0.0) {
+        const double sc = Fbarx / tr;
+        S11 *= sc; S12 *= sc; S13 *= sc; S22 *= sc; S23 *= sc; S33 *= sc;
+    }
+    double lamf[3], Frhoc, e_f, p_f;
+    eig3_sym(S11, S12, S13, S22, S23, S33, lamf);
+    ev_from_lam(lamf, &Frhoc, &e_f, &p_f);
+    double F11 = last.E11, F12 = last.E12, F13 = last.E13, F22 = last.E22, F23 = last.E23,
+           F33 = last.E33;
+    if (variant == HNT_VARIANT_A || Frhoh != 0.0) {
+        const double sc2 = Fbarx / Frhoh;
+        F11 *= sc2; F12 *= sc2; F13 *= sc2; F22 *= sc2; F23 *= sc2; F33 *= sc2;
+    }
+
+    // Sbar, Sbar2: running sums over shells 1..m0-1 kept in the slot
+    const double* X = P + (int64_t)(m0 - 1) * PW;
+    double Sb0 = X[K_S], Sb1 = X[K_S + 1], Sb2v = X[K_S + 2];
+    double Tb0 = X[K_S2], Tb1 = X[K_S2 + 1], Tb2 = X[K_S2 + 2];
+    const double nS = X[K_N];
+    if (nS > 0.0) {
+        Sb0 /= nS; Sb1 /= nS; Sb2v /= nS;
+        Tb0 /= nS; Tb1 /= nS; Tb2 /= nS;
+    }
+
+    double Srb = 0.0, rad5p = 0.0;
+    if (m0 > 2)
+        for (int mp = 2; mp <= m0 - 1; ++mp) {
+            const double rad5 = pow5(rad[mp]);
+            Srb += 0.5 * (Fb(mp - 1) + Fb(mp)) * (rad5 - rad5p);
+            rad5p = rad5;
+        }
+    if (zvir1 > 0.0 && dZvir != 0.0) Srb += 0.5 * (Fb(m0 - 1) + Fbarx) * (RTHL5 - rad5p);
+    if (Fbarx * RTHL5 > 0.0) Srb /= (Fbarx * RTHL5);
+
+    // last shell within RTHL: d2F and the mask
+    const int sR = first_gt(rad, 1, NS, RTHL) - 1;
+    double d2F = 0.0;
+    if (a.has_lapd) {
+        const double* Y = P + (int64_t)sR * PW;
+        if (Y[K_LAPN] > 0.0) d2F = Y[K_LAP] / Y[K_LAPN];
+    }
+    if (a.mask) {
+        const int nb = a.nbuff;
+        for (int s = 1; s <= sR; ++s) {
+            const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
+            for (int c = 0; c < nc; ++c) {
+                const int4 o = a.cells[st + c];
+                const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
+                if (i1 > nb && i1 <= a.n1 - nb && i2 > nb && i2 <= a.n2 - nb && i3 > nb &&
+                    i3 <= a.n3 - nb && i1 >= 1 && i2 >= 1 && i3 >= 1)
+                    a.mask[(int64_t)(i1 - 1) + (i2 - 1) * a.s1 + (i3 - 1) * a.s2] = 1;
+            }
+        }
+    }
+
+    // formation redshift (RadialShell.jl:801-859); variant B returns 0
+    double zvir_half = -1.0;
+    if (variant == HNT_VARIANT_B) {
+        zvir_half = 0.0;
+    } else if (RTHL / 1.0 >= 3.0) {
+        for (int jj = 1; jj <= 10; ++jj) {
+            const double rcur = RTHL * pow((double)(jj - 1) / 9.0 * 0.5 + 0.5, 1.0 / 3.0);
+            int mupp_rc = 1;
+            if (m - 1 >= 2) {
+                int f = first_gt(rad, 2, m - 1, rcur + 2.0);
+                mupp_rc = f <= m - 1 ? f : m - 1;
+            }
+            double rlow_rc = rcur - 2.0;
+            if (rlow_rc < 0.0) rlow_rc = 0.0;
+            int mlow_rc = 1;
+            {
+                int f = first_gt(rad, 1, m - 1, rlow_rc);
+                if (f <= m - 1) mlow_rc = f;
+            }
+            int m0_rc = 1;
+            if (m - 1 >= 1) {
+                int f = first_gt(rad, 1, m - 1, rcur);
+                m0_rc = f <= m - 1 ? f : m - 1;
+            }
+            const KS kr = kstrain(rad, P, a.akk, m0_rc, mlow_rc, mupp_rc, a.wRnor, a.aRnor);
+            double Frr, e_rc, p_rc;
+            eig_ks(kr, lam);
+            ev_from_lam(lam, &Frr, &e_rc, &p_rc);
+            const double Frhoc_rc = Fb(m0_rc);
+            const double poe = e_rc < 1e-5 ? 0.0 : p_rc / e_rc;
+            double zr = -1.0;
+            if (Frhoc_rc > 0.0) zr = interp3(a.ct, log10(Frhoc_rc), e_rc, poe);
+            const double zs = zr - 1.0;
+            if (zs > zvir_half) zvir_half = zs;
+        }
+    }
+
+    const hnt_results& R = a.res;
+    R.RTHL[p] = RTHL;
+    R.Fbarx[p] = Fbarx;
+    R.e_v[p] = e_f;
+    R.p_v[p] = p_f;
+    R.Srb[p] = Srb;
+    R.d2F[p] = d2F;
+    R.zvir_half[p] = zvir_half;
+    double* sm = R.strain + (int64_t)p * 9;
+    sm[0] = F11; sm[1] = F12; sm[2] = F13;
+    sm[3] = F12; sm[4] = F22; sm[5] = F23;
+    sm[6] = F13; sm[7] = F23; sm[8] = F33;
+    const int64_t o3 = (int64_t)p * 3;
+    R.eigs[o3] = lamf[0]; R.eigs[o3 + 1] = lamf[1]; R.eigs[o3 + 2] = lamf[2];
+    R.Sbar[o3] = Sb0; R.Sbar[o3 + 1] = Sb1; R.Sbar[o3 + 2] = Sb2v;
+    R.Sbar2[o3] = Tb0; R.Sbar2[o3 + 1] = Tb1; R.Sbar2[o3 + 2] = Tb2;
+    R.gradpk[o3] = gk0; R.gradpk[o3 + 1] = gk1; R.gradpk[o3 + 2] = gk2;
+    R.gradpkf[o3] = gf0; R.gradpkf[o3 + 1] = gf1; R.gradpkf[o3 + 2] = gf2;
+    R.gradpkrf[o3] = krf.g0; R.gradpkrf[o3 + 1] = krf.g1; R.gradpkrf[o3 + 2] = krf.g2;
+}
+
+__device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__ P) {
+    const int lane = threadIdx.x;
+    const int64_t idx = grp * 64 + lane;
+    const bool valid = idx < a.npeaks;
+    const int p = valid ? a.order[idx] : 0;
+    int ci = 1, cj = 1, ck = 1, ir2min = 0;
+    double ZZon = 0.0, fcrit = 0.0, Rf = 0.0;
+    if (valid) {
+        ci = a.pi[p];
+        cj = a.pj[p];
+        ck = a.pk[p];
+        ir2min = a.ir2min[p];
+        Rf = a.Rf[p];
+        ZZon = a.ZZon ? a.ZZon[p] : a.ZZon_s;
+        fcrit = a.fcrit ? a.fcrit[p] : a.fcrit_s;
+    }
+    const double* __restrict__ rad = a.rad;
+    const int NS = a.NS;
+    const int m = NS - 1;
+
+    // filter-scale gradient shell mrf and its window (RadialShell.jl:450-482 for A,
+    // ShellAnalysisGPU.jl:233-266 for B/C): the first shell whose r^2 is nearest to
+    // (Rf/alatt)^2, if within 10, recorded two shells below (A; one below for r^2 = 1)
+    // or one below (B, C)
+    int mrf = 1, mlow_rf = 1, mupp_rf = 1;
+    if (valid) {
+        const double rf = Rf / a.alatt;
+        const double rf2 = rf * rf;
+        double dist;
+        if (a.variant == HNT_VARIANT_A) {
+            int s = closest_shell(a.shell_r2, 2, NS, rf2, &dist);
+            if (s >= 2 && dist < 10.0) mrf = (s == 2) ? 1 : s - 2;
+        } else {
+            int s = closest_shell(a.shell_r2, 2, m, rf2, &dist);
+            if (s >= 2 && dist < 10.0) mrf = s - 1;
+        }
+        double rlow_rf = rad[mrf] - 2.0;
+        if (rlow_rf < 0.0) rlow_rf = 0.0;
+        mlow_rf = first_gt(rad, 1, mrf - 1, rlow_rf);
+        if (mlow_rf > mrf - 1) mlow_rf = mrf;
+        mupp_rf = first_ge(rad, mrf + 1, m, rad[mrf] + 2.0) - 1;
+    }
+
+    // ---- gather ----
+    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
+    const int R = a.rmax;
+    const bool inside = ci - R >= 1 && ci + R <= a.n1 && cj - R >= 1 && cj + R <= a.n2 &&
+                        ck - R >= 1 && ck + R <= a.n3;
+    const bool wave_inside = __ballot(!valid || inside) == ~0ull;
+
+    double Fprev = 0.0, dFbarp = 0.0, cumN = 0.0, cS0 = 0.0, cS1 = 0.0, cS2 = 0.0, cT0 = 0.0,
+           cT1 = 0.0, cT2 = 0.0, lap = 0.0;
+    int64_t lapn = 0;
+    if (valid) {
+        const float4 v0 = a.recs[2 * c0], v1 = a.recs[2 * c0 + 1];
+        Fprev = (double)v0.x;
+        dFbarp = Fprev;
+        cumN = 1.0;
+        cS0 = 0.0 + (double)v0.z;
+        cS1 = 0.0 + (double)v0.w;
+        cS2 = 0.0 + (double)v1.x;
+        if (a.has_eta2) {
+            cT0 = 0.0 + (double)v1.y;
+            cT1 = 0.0 + (double)v1.z;
+            cT2 = 0.0 + (double)v1.w;
+        }
+        if (a.has_lapd && a.variant != HNT_VARIANT_A) {
+            lap = 0.0 + (double)a.lapd[c0];
+            lapn = 1;
+        }
+        double* X = P + PW;
+        for (int q = 0; q < 12; ++q) X[q] = 0.0;
+        X[K_F] = Fprev;
+        X[K_N] = cumN;
+        X[K_S] = cS0; X[K_S + 1] = cS1; X[K_S + 2] = cS2;
+        X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
+        X[K_LAP] = lap;
+        X[K_LAPN] = (double)lapn;
+    }
+
+    bool active = valid;
+    bool past = false, crossed = false, stop_set = false, oobA = false;
+    double rstop = 0.0;
+    int stop = m;
+    for (int s = 2; s <= m; ++s) {
+        if (__ballot(active) == 0ull) break;
+        const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
+        const double radS = rad[s], radP = rad[s - 1];
+        if (active) {
+            Acc A;
+            A.F = 0.0; A.S0 = 0.0; A.S1 = 0.0; A.S2 = 0.0; A.T0 = 0.0; A.T1 = 0.0; A.T2 = 0.0;
+            A.G0 = 0.0; A.G1 = 0.0; A.G2 = 0.0; A.H0 = 0.0; A.H1 = 0.0; A.H2 = 0.0;
+            A.R00 = 0.0; A.R01 = 0.0; A.R02 = 0.0; A.R10 = 0.0; A.R11 = 0.0; A.R12 = 0.0;
+            A.R20 = 0.0; A.R21 = 0.0; A.R22 = 0.0;
+            A.lap = lap; A.lapn = lapn; A.nin = 0;
+            if (wave_inside) gather_shell<false>(a, st, nc, ci, cj, ck, c0, A, oobA);
+            else gather_shell<true>(a, st, nc, ci, cj, ck, c0, A, oobA);
+            lap = A.lap;
+            lapn = A.lapn;
+            const int64_t nin = A.nin;
+            const double dFbar = nin > 0 ? A.F / (double)nin : 0.0;
+            const double rad3p = cube(radP), rad3 = cube(radS);
+            const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
+            Fprev = Fb;
+            dFbarp = dFbar;
+            if (nin > 0) {
+                cumN += (double)nin;
+                cS0 += A.S0; cS1 += A.S1; cS2 += A.S2;
+                cT0 += A.T0; cT1 += A.T1; cT2 += A.T2;
+            }
+            const double r00 = A.R00 / radS, r01 = A.R01 / radS, r02 = A.R02 / radS;
+            const double r10 = A.R10 / radS, r11 = A.R11 / radS, r12 = A.R12 / radS;
+            const double r20 = A.R20 / radS, r21 = A.R21 / radS, r22 = A.R22 / radS;
+            double* X = P + (int64_t)s * PW;
+            X[0] = A.G0 / radS; X[1] = A.G1 / radS; X[2] = A.G2 / radS;
+            X[3] = A.H0 / radS; X[4] = A.H1 / radS; X[5] = A.H2 / radS;
+            X[6] = r00 + r00; X[7] = r01 + r10; X[8] = r02 + r20;
+            X[9] = r11 + r11; X[10] = r12 + r21; X[11] = r22 + r22;
+            X[K_F] = Fb;
+            X[K_N] = cumN;
+            X[K_S] = cS0; X[K_S + 1] = cS1; X[K_S + 2] = cS2;
+            X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
+            X[K_LAP] = lap;
+            X[K_LAPN] = (double)lapn;
+            // early exit: nothing beyond r_cross + 3 (and the gradient window) is read later
+            if (!past && a.shell_r2[s - 1] > ir2min) past = true;
+            if (past && !crossed && Fb < fcrit) {
+                crossed = true;
+                rstop = radS + 3.0;
+            }
+            if (!stop_set && crossed && radS >= rstop && s >= mupp_rf) {
+                stop = s;
+                stop_set = true;
+                if (a.early) active = false;
+            }
+        }
+    }
+    if (!valid) return;
+    if (a.res.nshells_used) a.res.nshells_used[p] = stop;
+    post_process(a, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
+}
+
+__global__ __launch_bounds__(64) void shell_analyse_kernel(SAArgs a) {
+    double* P = a.prof + ((int64_t)blockIdx.x * 64 + threadIdx.x) * a.slot_doubles;
+    int q = blockIdx.x & 7;  // workgroups are dispatched round-robin over the 8 XCDs
+    for (int tried = 0; tried < 8;) {
+        unsigned long long g = 0;
+        if (threadIdx.x == 0) g = atomicAdd(&a.qctr[q], 1ull);
+        g = __shfl(g, 0);
+        const int64_t grp = a.qbeg[q] + (int64_t)g;
+        if (grp >= a.qend[q]) {
+            ++tried;
+            q = (q + 1) & 7;
+            continue;
+        }
+        analyse_group(a, grp, P);
+    }
+}
+
+}  // namespace
+
+using namespace hnt;
+
+extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
+                                        const hnt_batch* batch, const hnt_shell_tables* tables,
+                                        const hnt_ctab* ctab, const hnt_params* params,
+                                        hnt_results* results, int8_t* mask) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(fields && batch && tables && ctab && params && results,
+                "hnt_shell_analyse: NULL argument");
+    const int n = fields->n1;
+    HNT_REQUIRE(fields->n1 == fields->n2 && fields->n2 == fields->n3 && n >= 1 && n <= 8192,
+                "hnt_shell_analyse: the transposed read needs a cubic grid, got %d x %d x %d",
+                fields->n1, fields->n2, fields->n3);
+    HNT_REQUIRE(fields->delta && fields->etax && fields->etay && fields->etaz,
+                "hnt_shell_analyse: delta and eta fields are required");
+    HNT_REQUIRE((fields->eta2x == nullptr) == (fields->eta2y == nullptr) &&
+                    (fields->eta2y == nullptr) == (fields->eta2z == nullptr),
+                "hnt_shell_analyse: eta2x/y/z must be all set or all NULL");
+    HNT_REQUIRE(params->variant >= 0 && params->variant <= 2, "hnt_shell_analyse: bad variant");
+    if (params->rmax2rs != 0.0) {
+        set_error("hnt_shell_analyse: rmax2rs > 0 is not supported by this build");
+        return HNT_ERR_UNSUPPORTED;
+    }
+    HNT_REQUIRE(params->alatt > 0.0, "hnt_shell_analyse: alatt must be > 0");
+    HNT_REQUIRE(batch->npeaks >= 0 && batch->npeaks < (1LL << 31), "hnt_shell_analyse: npeaks");
+    const int64_t np = batch->npeaks;
+    if (np > 0) {
+        HNT_REQUIRE(batch->pi && batch->pj && batch->pk && batch->Rf && batch->ir2min,
+                    "hnt_shell_analyse: NULL peak array");
+        HNT_REQUIRE(results->RTHL && results->Fbarx && results->e_v && results->p_v &&
+                        results->strain && results->eigs && results->Srb && results->Sbar &&
+                        results->Sbar2 && results->gradpk && results->gradpkf &&
+                        results->gradpkrf && results->d2F && results->zvir_half,
+                    "hnt_shell_analyse: NULL result array");
+        HNT_REQUIRE(batch->ZZon != nullptr || batch->ZZon_scalar > -1.0,
+                    "hnt_shell_analyse: ZZon must be > -1");
+    }
+    HNT_TRY(upload_tables(ctx, tables));
+    HNT_TRY(upload_ctab(ctx, ctab));
+    HNT_TRY(ensure_akk(ctx));
+    if (np == 0) return HNT_OK;
+    HNT_REQUIRE(tables->rmax < n, "hnt_shell_analyse: shell radius %d not below the grid size %d",
+                tables->rmax, n);
+    hipStream_t st = ctx->stream;
+
+    // ---- validate the peak cells and ZZon on the device before any gather ----
+    {
+        HNT_TRY(reserve(ctx, ctx->small, 64));
+        hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ctx->small.p, 1,
+                           0ull);
+        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
+                           batch->pi, batch->pj, batch->pk, batch->ZZon, np, n,
+                           (unsigned int*)ctx->small.p);
+        unsigned long long bad = 0;
+        HNT_HIP(hipMemcpyAsync(&bad, ctx->small.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+        HNT_HIP(hipStreamSynchronize(st));
+        HNT_REQUIRE(!(bad & 1ull), "hnt_shell_analyse: a peak lies outside the %d^3 grid", n);
+        HNT_REQUIRE(!(bad & 2ull), "hnt_shell_analyse: per-peak ZZon must be > -1");
+    }
+
+    // ---- repack the fields into 32-byte cell records ----
+    const int64_t ncell = (int64_t)n * n * n;
+    HNT_TRY(reserve(ctx, ctx->recs, (size_t)ncell * 32));
+    {
+        dim3 g((n + 63) / 64, (n + 63) / 64, n);
+        hipLaunchKernelGGL(repack_kernel, g, dim3(256), 0, st, fields->delta, fields->etax,
+                           fields->etay, fields->etaz, fields->eta2x, fields->eta2y,
+                           fields->eta2z, n, (float4*)ctx->recs.p);
+    }
+
+    // ---- peak order: (ir2min, Morton) ----
+    HNT_TRY(reserve(ctx, ctx->keys_a, (size_t)np * 8));
+    HNT_TRY(reserve(ctx, ctx->keys_b, (size_t)np * 8));
+    HNT_TRY(reserve(ctx, ctx->vals_a, (size_t)np * 4));
+    HNT_TRY(reserve(ctx, ctx->vals_b, (size_t)np * 4));
+    hipLaunchKernelGGL(key_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
+                       batch->pi, batch->pj, batch->pk, batch->ir2min, np,
+                       (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);
+    size_t tmp_bytes = 0;
+    HNT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint64_t*)ctx->keys_a.p,
+                                               (uint64_t*)ctx->keys_b.p, (int32_t*)ctx->vals_a.p,
+                                               (int32_t*)ctx->vals_b.p, (int)np, 0, 64, st));
+    HNT_TRY(reserve(ctx, ctx->cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
+    HNT_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp.p, tmp_bytes,
+                                               (uint64_t*)ctx->keys_a.p, (uint64_t*)ctx->keys_b.p,
+                                               (int32_t*)ctx->vals_a.p, (int32_t*)ctx->vals_b.p,
+                                               (int)np, 0, 64, st));
+
+    // ---- persistent grid and profile scratch ----
+    SAArgs A;
+    memset(&A, 0, sizeof(A));
+    A.recs = (const float4*)ctx->recs.p;
+    A.lapd = fields->lapd;
+    A.n1 = A.n2 = A.n3 = n;
+    A.s1 = n;
+    A.s2 = (int64_t)n * n;
+    A.pi = batch->pi;
+    A.pj = batch->pj;
+    A.pk = batch->pk;
+    A.ir2min = batch->ir2min;
+    A.Rf = batch->Rf;
+    A.ZZon = batch->ZZon;
+    A.fcrit = batch->fcrit;
+    A.ZZon_s = batch->ZZon_scalar;
+    A.fcrit_s = batch->fcrit_scalar;
+    A.order = (const int32_t*)ctx->vals_b.p;
+    A.npeaks = np;
+    A.cells = (const int4*)ctx->tables.cells.p;
+    A.shell_start = (const int32_t*)ctx->tables.start.p;
+    A.shell_count = (const int32_t*)ctx->tables.count.p;
+    A.shell_r2 = (const int32_t*)ctx->tables.r2.p;
+    A.rad = (const double*)ctx->tables.rad.p;
+    A.NS = tables->nshells;
+    A.rmax = tables->rmax;
+    A.ct = CTabView{(const float*)ctx->ctab.table.p, ctab->nx, ctab->ny, ctab->nz, ctab->x1,
+                    ctab->x2, ctab->y1, ctab->y2, ctab->z1, ctab->z2, ctab->out_val};
+    A.akk = (const double*)ctx->akk.p;
+    A.alatt = params->alatt;
+    // hlatt = 1 (RadialShell.jl:296-300)
+    A.aRnor = 3.0 * 1.0 / params->alatt;
+    A.wRnor = 3.0 * (1.0 / (4.0 * kPi)) / params->alatt;
+    A.nbuff = params->nbuff;
+    A.variant = params->variant;
+    A.early = params->early_exit ? 1 : 0;
+    A.has_eta2 = fields->eta2x != nullptr;
+    A.has_lapd = fields->lapd != nullptr;
+    A.res = *results;
+    A.mask = mask;
+
+    const int64_t ngroups = (np + 63) / 64;
+    int per_cu = 0;
+    HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shell_analyse_kernel, 64, 0));
+    if (per_cu < 1) per_cu = 1;
+    A.slot_doubles = (int64_t)(A.NS + 1) * PW;
+    const size_t slot_bytes = (size_t)A.slot_doubles * 8;
+    const size_t budget = (size_t)24 << 30;  // profile scratch cap
+    int64_t nblocks = (int64_t)per_cu * ctx->num_cus;
+    int64_t cap_blocks = (int64_t)(budget / (slot_bytes * 64));
+    if (nblocks > cap_blocks) nblocks = cap_blocks;
+    int64_t want = ((ngroups + 7) / 8) * 8;
+    if (nblocks > want) nblocks = want;
+    nblocks = (nblocks / 8) * 8;
+    if (nblocks < 8) nblocks = 8;
+    HNT_TRY(reserve(ctx, ctx->prof, (size_t)nblocks * 64 * slot_bytes));
+    A.prof = (double*)ctx->prof.p;
+    HNT_TRY(reserve(ctx, ctx->queue, 8 * sizeof(unsigned long long)));
+    A.qctr = (unsigned long long*)ctx->queue.p;
+    for (int x = 0; x < 8; ++x) {
+        A.qbeg[x] = ngroups * x / 8;
+        A.qend[x] = ngroups * (x + 1) / 8;
+    }
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, A.qctr, 8, 0ull);
+    hipLaunchKernelGGL(shell_analyse_kernel, dim3((unsigned)nblocks), dim3(64), 0, st, A);
+    return finish_call(ctx, "hnt_shell_analyse");
+}

@@ -1,0 +1,230 @@
+"""ctypes mirror of include/huntian_abi.h and the loader of libhuntian_hip.so.
+
+The struct layouts here must match the C header field for field; tests check the sizes
+against the library (hnt_abi_version plus a struct-size probe is not exported, so the test
+builds a tiny C program instead when a compiler is present).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhuntian_hip.so")
+
+HNT_OK = 0
+HNT_ERR_INVALID = 1
+HNT_ERR_DEVICE = 2
+HNT_ERR_CAPACITY = 3
+HNT_ERR_UNSUPPORTED = 4
+HNT_ERR_ALLOC = 5
+
+HNT_VARIANT_A = 0
+HNT_VARIANT_B = 1
+HNT_VARIANT_C = 2
+VARIANTS = {"A": HNT_VARIANT_A, "B": HNT_VARIANT_B, "C": HNT_VARIANT_C}
+
+_i32p = C.POINTER(C.c_int32)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_i8p = C.POINTER(C.c_int8)
+
+
+class ShellTables(C.Structure):
+    _fields_ = [
+        ("rmax", C.c_int32),
+        ("nshells", C.c_int32),
+        ("ncells", C.c_int64),
+        ("di", _i32p),
+        ("dj", _i32p),
+        ("dk", _i32p),
+        ("shell_start", _i32p),
+        ("shell_count", _i32p),
+        ("shell_r2", _i32p),
+    ]
+
+
+class CTab(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("x1", C.c_double),
+        ("x2", C.c_double),
+        ("y1", C.c_double),
+        ("y2", C.c_double),
+        ("z1", C.c_double),
+        ("z2", C.c_double),
+        ("out_val", C.c_double),
+        ("table", _f32p),
+    ]
+
+
+class Peaks(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_int64),
+        ("i", _i32p),
+        ("j", _i32p),
+        ("k", _i32p),
+        ("filt", _i32p),
+        ("delta", _f32p),
+        ("count", C.c_int64),
+        ("total", C.c_int64),
+    ]
+
+
+class Fields(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int32),
+        ("n2", C.c_int32),
+        ("n3", C.c_int32),
+        ("delta", _f32p),
+        ("etax", _f32p),
+        ("etay", _f32p),
+        ("etaz", _f32p),
+        ("eta2x", _f32p),
+        ("eta2y", _f32p),
+        ("eta2z", _f32p),
+        ("lapd", _f32p),
+    ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("npeaks", C.c_int64),
+        ("pi", _i32p),
+        ("pj", _i32p),
+        ("pk", _i32p),
+        ("Rf", _f64p),
+        ("ir2min", _i32p),
+        ("ZZon", _f64p),
+        ("fcrit", _f64p),
+        ("ZZon_scalar", C.c_double),
+        ("fcrit_scalar", C.c_double),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("alatt", C.c_double),
+        ("nbuff", C.c_int32),
+        ("rmax2rs", C.c_double),
+        ("variant", C.c_int32),
+        ("early_exit", C.c_int32),
+    ]
+
+
+RESULT_SCALARS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half")
+RESULT_VEC3 = ("eigs", "Sbar", "Sbar2", "gradpk", "gradpkf", "gradpkrf")
+
+
+class Results(C.Structure):
+    _fields_ = [
+        ("RTHL", _f64p),
+        ("Fbarx", _f64p),
+        ("e_v", _f64p),
+        ("p_v", _f64p),
+        ("strain", _f64p),
+        ("eigs", _f64p),
+        ("Srb", _f64p),
+        ("Sbar", _f64p),
+        ("Sbar2", _f64p),
+        ("gradpk", _f64p),
+        ("gradpkf", _f64p),
+        ("gradpkrf", _f64p),
+        ("d2F", _f64p),
+        ("zvir_half", _f64p),
+        ("nshells_used", _i32p),
+    ]
+
+
+def as_ptr(addr: int, ctype):
+    """Raw address (device or host) -> typed ctypes pointer (NULL for 0/None)."""
+    if not addr:
+        return C.cast(None, C.POINTER(ctype))
+    return C.cast(C.c_void_p(int(addr)), C.POINTER(ctype))
+
+
+# Symbols the header declares, in header order (the loader test checks every one).
+EXPORTED = (
+    "hnt_ctx_create",
+    "hnt_ctx_destroy",
+    "hnt_ctx_set_stream",
+    "hnt_last_error",
+    "hnt_abi_version",
+    "hnt_shell_table_size",
+    "hnt_build_shell_tables",
+    "hnt_fsc_of_z",
+    "hnt_peak_find",
+    "hnt_shell_analyse",
+    "hnt_shell_gather_full",
+    "hnt_eig3_symmetric_batch",
+    "hnt_interp3_trilinear",
+    "hnt_kernel_strain",
+)
+
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load libhuntian_hip.so (built in-tree); raise loudly if it is not there."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise NativeLibraryMissing(
+            f"{p} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    lib.hnt_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.hnt_ctx_destroy.argtypes = [vp]
+    lib.hnt_ctx_set_stream.argtypes = [vp, vp]
+    lib.hnt_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    lib.hnt_last_error.restype = C.c_size_t
+    lib.hnt_abi_version.restype = C.c_int
+    lib.hnt_shell_table_size.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    lib.hnt_build_shell_tables.argtypes = [C.c_int32] + [_i32p] * 6
+    lib.hnt_fsc_of_z.argtypes = [C.POINTER(CTab), C.c_double]
+    lib.hnt_fsc_of_z.restype = C.c_double
+    lib.hnt_peak_find.argtypes = [vp, C.POINTER(vp), _f32p, C.c_int32, C.c_int32, C.c_int32,
+                                  _i8p, C.POINTER(Peaks)]
+    lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
+                                      C.POINTER(ShellTables), C.POINTER(CTab),
+                                      C.POINTER(Params), C.POINTER(Results), _i8p]
+    lib.hnt_shell_gather_full.argtypes = [vp, C.POINTER(Fields), C.c_int64, _i32p, _i32p, _i32p,
+                                          C.POINTER(ShellTables), _f64p, _i32p, _f64p, _f64p,
+                                          _f64p, _f64p, _f64p, _f64p]
+    lib.hnt_eig3_symmetric_batch.argtypes = [vp, _f64p, C.c_int64, _f64p]
+    lib.hnt_interp3_trilinear.argtypes = [vp, C.POINTER(CTab), _f64p, _f64p, _f64p, C.c_int64,
+                                          _f64p]
+    lib.hnt_kernel_strain.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _f64p]
+    for name in EXPORTED:
+        if name not in ("hnt_last_error", "hnt_abi_version", "hnt_fsc_of_z"):
+            getattr(lib, name).restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error(lib) -> str:
+    buf = C.create_string_buffer(1024)
+    lib.hnt_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+class HuntianError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"huntian status {status}: {msg}")
+        self.status = status
+
+
+def check(lib, status: int):
+    if status != HNT_OK:
+        raise HuntianError(status, last_error(lib))

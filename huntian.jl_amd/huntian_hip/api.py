@@ -1,0 +1,557 @@
+"""Host-side mirror of Huntian.jl's GPU plugin API for the halo-finding path.
+
+Each function keeps the name and argument meaning of the stub the reference declares
+(src/Huntian.jl:98-412) and that ext/CUDAExt.jl implements, and calls libhuntian_hip.so
+through its C ABI.  Device memory comes from PyTorch (ROCm); the library never sees a
+torch type.  Arrays follow the reference layout: shape (n1, n2, n3) indexed [i, j, k],
+column-major, peak indices 1-based.
+
+There is no CPU fallback: every call fails loudly if the HIP library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .homeltab import make_ctab
+
+_ctx = None
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _require_gpu():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("huntian_hip needs a HIP GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+class Context:
+    """One hnt_ctx on one device (the reference binds one task per device,
+    set_cuda_device!, src/Huntian.jl:414-424)."""
+
+    def __init__(self, device: int = 0):
+        torch = _require_gpu()
+        self.lib = abi.load()
+        self.device = device
+        torch.cuda.set_device(device)
+        h = C.c_void_p()
+        abi.check(self.lib, self.lib.hnt_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self._stream_ptr = None
+
+    def bind_stream(self, stream=None):
+        """Run library work on `stream` (a torch.cuda.Stream) or torch's current stream."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        ptr = s.cuda_stream
+        if ptr != self._stream_ptr:
+            abi.check(self.lib, self.lib.hnt_ctx_set_stream(self.handle, C.c_void_p(ptr)))
+            self._stream_ptr = ptr
+
+    def close(self):
+        if self.handle:
+            self.lib.hnt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def context(device: int = 0) -> Context:
+    global _ctx
+    if _ctx is None or _ctx.device != device:
+        _ctx = Context(device)
+    _ctx.bind_stream()
+    return _ctx
+
+
+# ---------------------------------------------------------------- layout helpers
+
+def fortran_empty(shape, dtype, device="cuda"):
+    """torch tensor of `shape` with column-major strides (the reference layout)."""
+    torch = _torch()
+    return torch.empty(tuple(shape)[::-1], dtype=dtype, device=device).permute(
+        *range(len(shape) - 1, -1, -1))
+
+
+def to_device_field(a, dtype=None, device="cuda"):
+    """numpy (any order) or torch array -> column-major device tensor."""
+    torch = _torch()
+    dtype = dtype or torch.float32
+    if isinstance(a, torch.Tensor):
+        t = a.to(device=device, dtype=dtype)
+    else:
+        t = torch.from_numpy(np.asarray(a))
+        t = t.to(device=device, dtype=dtype)
+    if t.dim() == 3 and not t.permute(2, 1, 0).is_contiguous():
+        out = fortran_empty(t.shape, dtype, device)
+        out.copy_(t)
+        t = out
+    return t
+
+
+def _ptr(t):
+    return 0 if t is None else t.data_ptr()
+
+
+def _dev1(a, dtype, device="cuda"):
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), device=device).to(dtype).contiguous()
+
+
+# ---------------------------------------------------------------- tables
+
+@dataclass
+class ShellTables:
+    """build_shell_tables(rmax) (src/ShellAnalysisGPU.jl:28-71); shell_start 0-based."""
+    rmax: int
+    di: np.ndarray
+    dj: np.ndarray
+    dk: np.ndarray
+    shell_start: np.ndarray
+    shell_count: np.ndarray
+    shell_r2: np.ndarray
+
+    @property
+    def nshells(self):
+        return len(self.shell_r2)
+
+    @property
+    def ncells(self):
+        return len(self.di)
+
+    def struct(self) -> abi.ShellTables:
+        p = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+        return abi.ShellTables(self.rmax, self.nshells, self.ncells, p(self.di), p(self.dj),
+                               p(self.dk), p(self.shell_start), p(self.shell_count),
+                               p(self.shell_r2))
+
+
+def build_shell_tables(rmax: int) -> ShellTables:
+    lib = abi.load()
+    nc = C.c_int64()
+    ns = C.c_int32()
+    abi.check(lib, lib.hnt_shell_table_size(rmax, C.byref(nc), C.byref(ns)))
+    arr = {k: np.zeros(nc.value, np.int32) for k in ("di", "dj", "dk")}
+    arr.update({k: np.zeros(ns.value, np.int32)
+                for k in ("shell_start", "shell_count", "shell_r2")})
+    abi.check(lib, lib.hnt_build_shell_tables(
+        rmax, *[arr[k].ctypes.data_as(C.POINTER(C.c_int32))
+                for k in ("di", "dj", "dk", "shell_start", "shell_count", "shell_r2")]))
+    return ShellTables(rmax=rmax, **arr)
+
+
+def _ctab_struct(ct: dict):
+    tab = np.ascontiguousarray(np.asarray(ct["table"], np.float32).ravel(order="F"))
+    s = abi.CTab(ct["nx"], ct["ny"], ct["nz"], ct["x1"], ct["x2"], ct["y1"], ct["y2"], ct["z1"],
+                 ct["z2"], ct.get("out_val", -1.0), tab.ctypes.data_as(C.POINTER(C.c_float)))
+    return s, tab
+
+
+def fsc_of_z(ct: dict, z: float) -> float:
+    """RadialShell.fsc_of_z (src/HaloFinder/RadialShell.jl:181-195)."""
+    lib = abi.load()
+    s, keep = _ctab_struct(ct)
+    v = lib.hnt_fsc_of_z(C.byref(s), float(z))
+    del keep
+    return v
+
+
+def ir2min_of(Rf: float, alatt: float) -> int:
+    """src/MultiResolution.jl:575-576."""
+    import math
+
+    return min(int(math.floor((1.75 * Rf / alatt) ** 2)),
+               int(math.floor((40.0 / alatt - 1) ** 2)))
+
+
+# ---------------------------------------------------------------- peak finding
+
+def peak_find(smoothed, fcrits, nbuff: int, mask, capacity: int | None = None):
+    """Multi-filter peak finding on already-smoothed device fields (hnt_peak_find).
+
+    smoothed: list of column-major (n, n, n) float32 device tensors, in processing order.
+    mask: column-major (n, n, n) int8 device tensor, updated in place.
+    Returns dict of device tensors i, j, k (1-based), filt, delta and the total count."""
+    torch = _require_gpu()
+    ctx = context(smoothed[0].device.index or 0)
+    n = smoothed[0].shape[0]
+    for f in smoothed:
+        assert f.shape == (n, n, n) and f.dtype == torch.float32 and f.permute(2, 1, 0).is_contiguous()
+    assert mask.shape == (n, n, n) and mask.dtype == torch.int8 and mask.permute(2, 1, 0).is_contiguous()
+    m = max(n - 2 * nbuff, 0)
+    if capacity is None:
+        capacity = max(1024, m ** 3 // 64)
+    dev = smoothed[0].device
+    out = {k: torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
+           for k in ("i", "j", "k", "filt")}
+    out["delta"] = torch.empty(max(capacity, 1), dtype=torch.float32, device=dev)
+    pk = abi.Peaks(capacity, *[abi.as_ptr(out[k].data_ptr(), C.c_int32)
+                               for k in ("i", "j", "k", "filt")],
+                   abi.as_ptr(out["delta"].data_ptr(), C.c_float), 0, 0)
+    ptrs = (C.c_void_p * len(smoothed))(*[f.data_ptr() for f in smoothed])
+    fc = np.ascontiguousarray(np.asarray(fcrits, np.float32))
+    st = ctx.lib.hnt_peak_find(ctx.handle, ptrs, fc.ctypes.data_as(C.POINTER(C.c_float)),
+                               len(smoothed), n, nbuff,
+                               abi.as_ptr(mask.data_ptr(), C.c_int8), C.byref(pk))
+    if st != abi.HNT_ERR_CAPACITY:
+        abi.check(ctx.lib, st)
+    # On HNT_ERR_CAPACITY the first `capacity` peaks (in filter, k, j, i order) are returned
+    # and masked, like the reference GPU method that drops slots beyond max_peaks
+    # (ext/CUDAExt.jl:2921-2926, 3020-3021); "total" tells how many were found.
+    cnt = int(pk.count)
+    res = {k: v[:cnt] for k, v in out.items()}
+    res["total"] = int(pk.total)
+    return res
+
+
+def tophat_smooth_torch(delta_k, n, boxsize, R, wsmooth=1):
+    """Filter-bank window in k-space then irfft (src/HaloFinder/Filters.jl:72-99, GPU
+    _smooth_window_kernel!, ext/CUDAExt.jl:2831-2871).  delta_k is torch.fft.rfftn of the
+    C-order view [k, j, i].  Returns a column-major (n,n,n) float32 tensor."""
+    torch = _torch()
+    dev = delta_k.device
+    dk = 2.0 * np.pi / boxsize
+    kx = torch.arange(n // 2 + 1, device=dev, dtype=torch.float64) * dk
+    fy = torch.fft.fftfreq(n, d=1.0 / n, device=dev, dtype=torch.float64) * dk
+    k2 = kx[None, None, :] ** 2 + fy[None, :, None] ** 2 + fy[:, None, None] ** 2
+    k = torch.sqrt(k2)
+    x = k * R
+    if wsmooth == 0:
+        w = torch.exp(-x * x * 0.125)
+    elif wsmooth == 3:
+        w = torch.exp(-x * x * 0.125) * k2
+    else:
+        xs = torch.where(x > 0, x, torch.ones_like(x))
+        w = 3.0 * (torch.sin(xs) - xs * torch.cos(xs)) / xs ** 3
+    w = torch.where(k2 > 0, w.to(torch.float32).to(torch.float64), torch.ones_like(w))
+    out = torch.fft.irfftn(delta_k * w.to(delta_k.dtype), s=(n, n, n))
+    return out.to(torch.float32).permute(2, 1, 0)
+
+
+def peak_find_tile_gpu(delta_tile, filters, fcrits, tile_masks_in, xbx, ybx, zbx, alatt,
+                       nbuff, wsmooth, ioutshear, threads=256, max_peaks=2_000_000):
+    """Mirror of peak_find_tile_gpu (src/Huntian.jl:387-412, ext/CUDAExt.jl:2938-3060).
+
+    filters: list of (id, fcrit, Rf) in processing order (largest Rf first in the drivers).
+    Smoothing (one rfft, then window + irfft per filter) runs through torch.fft; the peak
+    test runs in libhuntian_hip.  Returns (peaks, Rf, FcRf, d2Rf, mask) with peaks a numpy
+    structured array of PeakCandidate fields (i, j, k, ipp, x, y, z, delta, Rsmooth).
+    Unlike the CUDA method the peak order is deterministic (filter, then k, j, i)."""
+    torch = _require_gpu()
+    n = int(delta_tile.shape[0])
+    d = to_device_field(delta_tile)
+    dk = torch.fft.rfftn(d.permute(2, 1, 0).to(torch.float64))
+    boxsize = n * float(alatt)
+    sm = [tophat_smooth_torch(dk, n, boxsize, float(f[2]), wsmooth) for f in filters]
+    mask = to_device_field(tile_masks_in, dtype=torch.int8)
+    res = peak_find(sm, fcrits, int(nbuff), mask, capacity=int(max_peaks))
+    cnt = len(res["i"])
+    i = res["i"].cpu().numpy().astype(np.int64)
+    j = res["j"].cpu().numpy().astype(np.int64)
+    k = res["k"].cpu().numpy().astype(np.int64)
+    fidx = res["filt"].cpu().numpy()
+    dl = res["delta"].cpu().numpy()
+    cen = 0.5 * (n + 1)
+    peaks = np.zeros(cnt, dtype=[("i", "i8"), ("j", "i8"), ("k", "i8"), ("ipp", "i8"),
+                                 ("x", "f8"), ("y", "f8"), ("z", "f8"), ("delta", "f4"),
+                                 ("Rsmooth", "f8")])
+    Rf = np.array([float(filters[f][2]) for f in fidx], np.float64)
+    peaks["i"], peaks["j"], peaks["k"] = i, j, k
+    peaks["ipp"] = i + (j - 1) * n + (k - 1) * n * n
+    peaks["x"] = float(xbx) + float(alatt) * (i - cen)
+    peaks["y"] = float(ybx) + float(alatt) * (j - cen)
+    peaks["z"] = float(zbx) + float(alatt) * (k - cen)
+    peaks["delta"] = dl
+    peaks["Rsmooth"] = Rf
+    d2Rf = np.zeros(cnt, np.float32)
+    if ioutshear >= 1:
+        for fi, f in enumerate(filters):
+            sel = np.nonzero(fidx == fi)[0]
+            if len(sel):
+                lap = tophat_smooth_torch(dk, n, boxsize, float(f[2]), 3)
+                st = torch.as_tensor(sel, device=lap.device)
+                d2Rf[sel] = lap[res["i"][st].long() - 1, res["j"][st].long() - 1,
+                                res["k"][st].long() - 1].cpu().numpy()
+    return peaks, Rf, dl.copy(), d2Rf, np.asfortranarray(mask.cpu().numpy())
+
+
+# ---------------------------------------------------------------- shell analysis
+
+def _fields_struct(fields: dict):
+    torch = _torch()
+    d = fields["delta"]
+    n1, n2, n3 = d.shape
+    keep = {}
+    ptrs = []
+    for key in ("delta", "etax", "etay", "etaz", "eta2x", "eta2y", "eta2z", "lapd"):
+        a = fields.get(key)
+        if a is None:
+            ptrs.append(abi.as_ptr(0, C.c_float))
+            continue
+        t = to_device_field(a, torch.float32, d.device if isinstance(d, torch.Tensor) else "cuda")
+        keep[key] = t
+        ptrs.append(abi.as_ptr(t.data_ptr(), C.c_float))
+    return abi.Fields(n1, n2, n3, *ptrs), keep
+
+
+def alloc_results(npk: int, device="cuda"):
+    torch = _torch()
+    r = {k: torch.empty(max(npk, 1), dtype=torch.float64, device=device)
+         for k in abi.RESULT_SCALARS}
+    r.update({k: torch.empty((max(npk, 1), 3), dtype=torch.float64, device=device)
+              for k in abi.RESULT_VEC3})
+    r["strain"] = torch.empty((max(npk, 1), 3, 3), dtype=torch.float64, device=device)
+    r["nshells_used"] = torch.empty(max(npk, 1), dtype=torch.int32, device=device)
+    return r
+
+
+def _results_struct(r):
+    vals = []
+    for name, _ in abi.Results._fields_:
+        t = r.get(name)
+        ct = C.c_int32 if name == "nshells_used" else C.c_double
+        vals.append(abi.as_ptr(_ptr(t), ct))
+    return abi.Results(*vals)
+
+
+def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct: dict, *,
+                  alatt: float, ZZon: float, fcrit: float, nbuff: int = 0, variant="C",
+                  early_exit: bool = True, mask=None, ZZon_pp=None, fcrit_pp=None,
+                  results=None):
+    """hnt_shell_analyse over one batch of peaks (device arrays or numpy).  Returns a dict of
+    device tensors: scalars [np], vectors [np, 3], strain [np, 3, 3] ([p, L, K])."""
+    torch = _require_gpu()
+    ctx = context()
+    fs, keep_f = _fields_struct(fields)
+    npk = int(len(pi))
+    dev = torch.device("cuda", ctx.device)
+    pi_ = _dev1(pi, torch.int32, dev)
+    pj_ = _dev1(pj, torch.int32, dev)
+    pk_ = _dev1(pk, torch.int32, dev)
+    Rf_ = _dev1(np.broadcast_to(np.asarray(Rf, np.float64), (npk,)) if not isinstance(
+        Rf, torch.Tensor) else Rf, torch.float64, dev)
+    ir_ = _dev1(np.broadcast_to(np.asarray(ir2min, np.int32), (npk,)) if not isinstance(
+        ir2min, torch.Tensor) else ir2min, torch.int32, dev)
+    zz = _dev1(ZZon_pp, torch.float64, dev) if ZZon_pp is not None else None
+    fc = _dev1(fcrit_pp, torch.float64, dev) if fcrit_pp is not None else None
+    b = abi.Batch(npk, abi.as_ptr(_ptr(pi_), C.c_int32), abi.as_ptr(_ptr(pj_), C.c_int32),
+                  abi.as_ptr(_ptr(pk_), C.c_int32), abi.as_ptr(_ptr(Rf_), C.c_double),
+                  abi.as_ptr(_ptr(ir_), C.c_int32), abi.as_ptr(_ptr(zz), C.c_double),
+                  abi.as_ptr(_ptr(fc), C.c_double), float(ZZon), float(fcrit))
+    prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)))
+    r = results if results is not None else alloc_results(npk, dev)
+    rs = _results_struct(r)
+    cs, keep_t = _ctab_struct(ct)
+    ts = tables.struct()
+    m = abi.as_ptr(_ptr(mask), C.c_int8)
+    st = ctx.lib.hnt_shell_analyse(ctx.handle, C.byref(fs), C.byref(b), C.byref(ts),
+                                   C.byref(cs), C.byref(prm), C.byref(rs), m)
+    abi.check(ctx.lib, st)
+    del keep_f, keep_t
+    if npk == 0:
+        return {k: v[:0] for k, v in r.items()}
+    return r
+
+
+def _host_results(r, npk):
+    out = {}
+    for k, v in r.items():
+        if v is None:
+            continue
+        a = v[:npk].cpu().numpy()
+        out[k] = a
+    return out
+
+
+def analyse_peaks_gpu_cuda_multirf(delta, etax, etay, etaz, eta2x, eta2y, eta2z, batches,
+                                   stab: ShellTables, ct_table, X1, X2, Y1, Y2, Z1, Z2, alatt,
+                                   ZZon, *, fcrit_override=None, ct=None, rmax2rs=0.0,
+                                   threads=128, ct_out_val=-1.0, lapd=None, mask=None,
+                                   nbuff=0, variant="C"):
+    """Mirror of analyse_peaks_gpu_cuda_multirf (src/Huntian.jl:259-281,
+    ext/CUDAExt.jl:2085-2344).
+
+    batches: list of (peaks_i, peaks_j, peaks_k, Rf, ir2min[, ZZon_pp, fcrit_pp]).
+    The fields are uploaded once and every batch runs in ONE library call (all peaks carry
+    their own Rf and ir2min).  Returns (results, mask): results[b] is a dict of host
+    Float32 arrays with the fields of the reference's NamedTuple; mask is the updated host
+    Int8 array or None.  `threads` is accepted for signature compatibility and ignored."""
+    torch = _require_gpu()
+    if rmax2rs != 0.0:
+        raise NotImplementedError("rmax2rs > 0 is not supported")
+    ctd = make_ctab(ct_table, X1, X2, Y1, Y2, Z1, Z2, ct_out_val)
+    if fcrit_override is not None:
+        fcrit = float(fcrit_override)
+    elif ct is not None:
+        fcrit = fsc_of_z(ct, float(ZZon) - 1.0)
+    else:
+        fcrit = 1.686
+    fields = dict(delta=delta, etax=etax, etay=etay, etaz=etaz, eta2x=eta2x, eta2y=eta2y,
+                  eta2z=eta2z, lapd=lapd)
+    n = int(delta.shape[0])
+    keys = ("pi", "pj", "pk", "Rf", "ir2", "zz", "fc")
+    cat = {k: [] for k in keys}
+    sizes = []
+    has_pp = any(len(b) >= 7 for b in batches)
+    for b in batches:
+        npb = len(b[0])
+        sizes.append(npb)
+        cat["pi"].append(np.asarray(b[0], np.int32))
+        cat["pj"].append(np.asarray(b[1], np.int32))
+        cat["pk"].append(np.asarray(b[2], np.int32))
+        cat["Rf"].append(np.full(npb, float(b[3])))
+        cat["ir2"].append(np.full(npb, int(b[4]), np.int32))
+        if has_pp:
+            cat["zz"].append(np.asarray(b[5], np.float64) if len(b) >= 7 else np.full(npb, float(ZZon)))
+            cat["fc"].append(np.asarray(b[6], np.float64) if len(b) >= 7 else np.full(npb, fcrit))
+    cc = {k: (np.concatenate(v) if v else None) for k, v in cat.items()}
+    mask_d = to_device_field(mask, dtype=torch.int8) if mask is not None else None
+    tot = int(sum(sizes))
+    r = shell_analyse(fields, cc["pi"], cc["pj"], cc["pk"], cc["Rf"], cc["ir2"], stab, ctd,
+                      alatt=alatt, ZZon=ZZon, fcrit=fcrit, nbuff=nbuff, variant=variant,
+                      early_exit=True, mask=mask_d,
+                      ZZon_pp=cc["zz"] if has_pp else None,
+                      fcrit_pp=cc["fc"] if has_pp else None)
+    host = _host_results(r, tot)
+    results = []
+    off = 0
+    for npb in sizes:
+        sl = slice(off, off + npb)
+        d = {k: host[k][sl].astype(np.float32) for k in abi.RESULT_SCALARS}
+        for k in abi.RESULT_VEC3:
+            d[k] = np.ascontiguousarray(host[k][sl].T.astype(np.float32))  # (3, npeaks)
+        d["strain_final"] = np.ascontiguousarray(
+            np.transpose(host["strain"][sl], (1, 2, 0)).astype(np.float32))  # (3,3,npeaks)
+        results.append(d)
+        off += npb
+    mask_out = None
+    if mask_d is not None:
+        mask_out = np.asfortranarray(mask_d.cpu().numpy())
+    return results, mask_out
+
+
+def analyse_peak_gpu_cuda(delta, etax, etay, etaz, eta2x, eta2y, eta2z, peaks_i, peaks_j,
+                          peaks_k, stab, ct_table, X1, X2, Y1, Y2, Z1, Z2, alatt, ir2min,
+                          ZZon, Rfclvi, *, fcrit_override=None, ct=None, rmax2rs=0.0,
+                          threads=128, ct_out_val=-1.0, lapd=None, mask=None, nbuff=0,
+                          variant="C"):
+    """Mirror of analyse_peak_gpu_cuda (src/Huntian.jl:219-257, ext/CUDAExt.jl:1914-2078):
+    one batch, scalar ZZon / fcrit; returns the reference's NamedTuple fields plus mask."""
+    res, m = analyse_peaks_gpu_cuda_multirf(
+        delta, etax, etay, etaz, eta2x, eta2y, eta2z,
+        [(peaks_i, peaks_j, peaks_k, Rfclvi, ir2min)], stab, ct_table, X1, X2, Y1, Y2, Z1, Z2,
+        alatt, ZZon, fcrit_override=fcrit_override, ct=ct, rmax2rs=rmax2rs, threads=threads,
+        ct_out_val=ct_out_val, lapd=lapd, mask=mask, nbuff=nbuff, variant=variant)
+    out = dict(res[0])
+    out["mask"] = m
+    return out
+
+
+# ---------------------------------------------------------------- primitive harnesses
+
+def shell_gather_full_gpu(delta, etax, etay, etaz, eta2x, eta2y, eta2z, peaks_i, peaks_j,
+                          peaks_k, stab: ShellTables, threads=128, lapd=None, return_lapd=False):
+    """Mirror of shell_gather_full_gpu (src/Huntian.jl:146-166): per-shell sums, Float64,
+    shaped like the reference's (nshells, npeaks) / (3, nshells, npeaks) /
+    (3, 3, nshells, npeaks) host arrays."""
+    torch = _require_gpu()
+    ctx = context()
+    fs, keep = _fields_struct(dict(delta=delta, etax=etax, etay=etay, etaz=etaz, eta2x=eta2x,
+                                   eta2y=eta2y, eta2z=eta2z, lapd=lapd))
+    npk = len(peaks_i)
+    ns = stab.nshells
+    dev = torch.device("cuda", ctx.device)
+    pi_ = _dev1(peaks_i, torch.int32, dev)
+    pj_ = _dev1(peaks_j, torch.int32, dev)
+    pk_ = _dev1(peaks_k, torch.int32, dev)
+    F = torch.empty((max(npk, 1), ns), dtype=torch.float64, device=dev)
+    N = torch.empty((max(npk, 1), ns), dtype=torch.int32, device=dev)
+    L = torch.empty_like(F)
+    S = torch.empty((max(npk, 1), ns, 3), dtype=torch.float64, device=dev)
+    S2 = torch.empty_like(S)
+    G = torch.empty_like(S)
+    Gf = torch.empty_like(S)
+    SR = torch.empty((max(npk, 1), ns, 3, 3), dtype=torch.float64, device=dev)  # [.., K, L]
+    ts = stab.struct()
+    d = lambda t: abi.as_ptr(t.data_ptr(), C.c_double)  # noqa: E731
+    abi.check(ctx.lib, ctx.lib.hnt_shell_gather_full(
+        ctx.handle, C.byref(fs), npk, abi.as_ptr(pi_.data_ptr(), C.c_int32),
+        abi.as_ptr(pj_.data_ptr(), C.c_int32), abi.as_ptr(pk_.data_ptr(), C.c_int32),
+        C.byref(ts), d(F), abi.as_ptr(N.data_ptr(), C.c_int32), d(S), d(S2), d(G), d(Gf),
+        d(SR), d(L)))
+    del keep
+    h = lambda t: t[:npk].cpu().numpy()  # noqa: E731
+    Fh, Nh, Sh, S2h, Gh, Gfh, SRh, Lh = map(h, (F, N, S, S2, G, Gf, SR, L))
+    out = (Fh.T, Nh.T, np.transpose(Sh, (2, 1, 0)), np.transpose(S2h, (2, 1, 0)),
+           np.transpose(Gh, (2, 1, 0)), np.transpose(Gfh, (2, 1, 0)),
+           np.transpose(SRh, (3, 2, 1, 0)))  # SR -> [L, K, s, p]
+    return out + (Lh.T,) if return_lapd else out
+
+
+def eig3_symmetric_batch_gpu(mats, threads=128):
+    """Mirror of eig3_symmetric_batch_gpu (src/Huntian.jl:168-180): mats (3, 3, N) ->
+    ascending eigenvalues (3, N), Float64."""
+    torch = _require_gpu()
+    ctx = context()
+    m = np.asarray(mats, np.float64)
+    N = m.shape[2]
+    flat = np.ascontiguousarray(np.transpose(m, (2, 1, 0)).reshape(N, 9))  # per matrix col-major
+    dm = torch.as_tensor(flat, device="cuda")
+    out = torch.empty((max(N, 1), 3), dtype=torch.float64, device="cuda")
+    abi.check(ctx.lib, ctx.lib.hnt_eig3_symmetric_batch(
+        ctx.handle, abi.as_ptr(dm.data_ptr(), C.c_double), N,
+        abi.as_ptr(out.data_ptr(), C.c_double)))
+    return out[:N].cpu().numpy().T.copy()
+
+
+def interp3_trilinear_gpu(table, X1, X2, Y1, Y2, Z1, Z2, xs, ys, zs, out_val=-1.0):
+    """Mirror of interp3_trilinear_gpu (src/Huntian.jl:182-195), Float64."""
+    torch = _require_gpu()
+    ctx = context()
+    ctd = make_ctab(table, X1, X2, Y1, Y2, Z1, Z2, out_val)
+    cs, keep = _ctab_struct(ctd)
+    x = torch.as_tensor(np.asarray(xs, np.float64), device="cuda")
+    y = torch.as_tensor(np.asarray(ys, np.float64), device="cuda")
+    z = torch.as_tensor(np.asarray(zs, np.float64), device="cuda")
+    n = len(x)
+    out = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+    abi.check(ctx.lib, ctx.lib.hnt_interp3_trilinear(
+        ctx.handle, C.byref(cs), abi.as_ptr(x.data_ptr(), C.c_double),
+        abi.as_ptr(y.data_ptr(), C.c_double), abi.as_ptr(z.data_ptr(), C.c_double), n,
+        abi.as_ptr(out.data_ptr(), C.c_double)))
+    del keep
+    return out[:n].cpu().numpy()
+
+
+def kernel_strain_gpu(rad, Gshell, Gshellf, SRshell, mp, mlow, mupp, wRnor, aRnor, hlatt_1,
+                      hlatt_2):
+    """Mirror of kernel_strain_gpu (src/Huntian.jl:197-217).  Gshell, Gshellf: (3, ns);
+    SRshell: (3, 3, ns); 1-based mp/mlow/mupp.  Returns (Ebar 3x3, grad 3, gradf 3)."""
+    ctx = context()
+    rad = np.ascontiguousarray(rad, np.float64)
+    ns = len(rad)
+    G = np.ascontiguousarray(np.asarray(Gshell, np.float64).T)       # [s, L]
+    Gf = np.ascontiguousarray(np.asarray(Gshellf, np.float64).T)
+    SR = np.ascontiguousarray(np.transpose(np.asarray(SRshell, np.float64), (2, 0, 1)))  # [s, L, K]
+    out = np.zeros(12)
+    d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    abi.check(ctx.lib, ctx.lib.hnt_kernel_strain(ctx.handle, d(rad), d(G), d(Gf), d(SR), ns,
+                                                 mp, mlow, mupp, wRnor, aRnor, hlatt_1,
+                                                 hlatt_2, d(out)))
+    E = np.array([[out[0], out[1], out[2]], [out[1], out[3], out[4]], [out[2], out[4], out[5]]])
+    return E, out[6:9].copy(), out[9:12].copy()

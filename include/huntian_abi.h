@@ -1,0 +1,240 @@
+/*
+ * huntian_abi.h -- C ABI of libhuntian_hip.so, the gfx950 (MI355X) implementation of
+ * Huntian.jl's halo-finding hot path: multi-filter peak finding over smoothed density
+ * fields, then the radial-shell homogeneous-ellipsoid collapse of every candidate.
+ *
+ * The ABI replaces the CUDA.jl package-extension methods that Huntian's drivers look up
+ * with getglobal (src/MultiResolution.jl:492,559,1037).  Each entry point below names the
+ * reference interface it replaces.  A Julia package extension binds these with ccall.
+ *
+ * Conventions
+ *  - Plain C types only.  No exceptions cross the boundary: every call returns an
+ *    hnt_status and, on failure, stores a message readable with hnt_last_error().
+ *  - The caller owns every buffer.  Arrays documented as "device" are HIP device
+ *    pointers (hipMalloc'd or equivalent); arrays documented as "host" are host memory.
+ *  - 3-D fields keep the reference (Julia) layout: column-major Float32, i fastest,
+ *    flat index i + (j-1)*n1 + (k-1)*n1*n2 with 1-based i, j, k.
+ *  - Peak coordinates are 1-based, as in PeakCandidate (src/HaloFinder/PeakFind.jl:3-13).
+ *  - One context per device; a context is not safe for concurrent calls from several
+ *    threads (the reference binds one Julia task per device, src/Huntian.jl:414-424).
+ *  - All work of a call is enqueued on the context's stream; the call returns after that
+ *    stream has completed it (the reference's host copies are synchronous as well).
+ */
+#ifndef HUNTIAN_ABI_H
+#define HUNTIAN_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HNT_ABI_VERSION 1
+
+typedef enum {
+    HNT_OK = 0,
+    HNT_ERR_INVALID = 1,     /* bad argument (shape, NULL pointer, out-of-range index) */
+    HNT_ERR_DEVICE = 2,      /* a HIP runtime call failed */
+    HNT_ERR_CAPACITY = 3,    /* output capacity too small; hnt_peaks.total says how much is needed */
+    HNT_ERR_UNSUPPORTED = 4, /* option not implemented by this build */
+    HNT_ERR_ALLOC = 5        /* device or host allocation failed */
+} hnt_status;
+
+/* Result variants of the collapse step.  The reference holds three forms of it:
+ *  A: RadialShell.analyse_peak, the FP64 CPU production path (src/HaloFinder/RadialShell.jl:281-865)
+ *  B: ShellAnalysisGPU.analyse_peak_gpu, FP64 CPU form of the GPU algorithm
+ *     (src/ShellAnalysisGPU.jl:81-535); zvir_half is returned as 0
+ *  C: the CUDA extension's semantics (ext/CUDAExt.jl:1213-1912): B plus the formation
+ *     redshift zvir_half of variant A.
+ * All three are computed here in FP64 with the reference CPU path's operation order. */
+enum { HNT_VARIANT_A = 0, HNT_VARIANT_B = 1, HNT_VARIANT_C = 2 };
+
+typedef struct hnt_ctx hnt_ctx;
+
+/* Context on HIP device `device` (0-based).  Replaces set_cuda_device! (src/Huntian.jl:414-424). */
+hnt_status hnt_ctx_create(int device, hnt_ctx** out);
+hnt_status hnt_ctx_destroy(hnt_ctx* ctx);
+/* Run subsequent calls on the caller's hipStream_t (NULL = the context's own stream). */
+hnt_status hnt_ctx_set_stream(hnt_ctx* ctx, void* hip_stream);
+/* Copy the calling thread's last error message into buf (NUL-terminated); returns its length. */
+size_t hnt_last_error(char* buf, size_t len);
+int hnt_abi_version(void);
+
+/* ---------------------------------------------------------------- shells and tables */
+
+/* Lattice offsets within radius rmax, sorted by r^2 with a stable sort, grouped into shells
+ * of equal r^2 (RadialShell.precompute_shells, src/HaloFinder/RadialShell.jl:151-166;
+ * ShellAnalysisGPU.build_shell_tables, src/ShellAnalysisGPU.jl:44-71).  All arrays host. */
+typedef struct {
+    int32_t rmax;
+    int32_t nshells;           /* distinct r^2 values, shell 1 (index 0 here) is r^2 = 0 */
+    int64_t ncells;
+    const int32_t* di;         /* [ncells] */
+    const int32_t* dj;
+    const int32_t* dk;
+    const int32_t* shell_start;  /* [nshells], 0-based offset into di/dj/dk */
+    const int32_t* shell_count;  /* [nshells] */
+    const int32_t* shell_r2;     /* [nshells] */
+} hnt_shell_tables;
+
+/* Number of cells and shells for rmax (to size the arrays of hnt_build_shell_tables). */
+hnt_status hnt_shell_table_size(int32_t rmax, int64_t* ncells, int32_t* nshells);
+/* Fill caller arrays (host) with the shell tables of radius rmax. */
+hnt_status hnt_build_shell_tables(int32_t rmax, int32_t* di, int32_t* dj, int32_t* dk,
+                                  int32_t* shell_start, int32_t* shell_count, int32_t* shell_r2);
+
+/* HomelTab collapse table: value 1 + z_vir on a regular grid of (log10 F, e, p/e).
+ * Layout of read_homeltab (src/EllipsoidalCollapse/CollapseTable.jl:96-115): column-major
+ * Float32 table, axis bounds widened to Float64.  Queries outside the box return out_val. */
+typedef struct {
+    int32_t nx, ny, nz;
+    double x1, x2, y1, y2, z1, z2;
+    double out_val;            /* -1 in the reference (CollapseTable.jl:132) */
+    const float* table;        /* host, [nx*ny*nz] */
+} hnt_ctab;
+
+/* fsc_of_z: critical linear overdensity for spherical collapse at redshift z, by 50-step
+ * bisection of the table (src/HaloFinder/RadialShell.jl:181-195).  Host computation. */
+double hnt_fsc_of_z(const hnt_ctab* ct, double z);
+
+/* ---------------------------------------------------------------- peak finding */
+
+typedef struct {
+    int64_t capacity;   /* length of the arrays below */
+    int32_t* i;         /* device [capacity], 1-based grid indices */
+    int32_t* j;
+    int32_t* k;
+    int32_t* filt;      /* device [capacity], 0-based index into the filter list passed */
+    float* delta;       /* device [capacity], smoothed delta at the peak (FcRf) */
+    int64_t count;      /* out: peaks written = min(total, capacity) */
+    int64_t total;      /* out: peaks found */
+} hnt_peaks;
+
+/*
+ * hnt_peak_find -- replaces the peak-finding loop of peak_find_tile_gpu
+ * (ext/CUDAExt.jl:2938-3060, stub src/Huntian.jl:387-412) and the CPU loop
+ * `for filter: find_peaks(...)` (src/MultiResolution.jl:1046-1066,
+ * src/HaloFinder/PeakFind.jl:15-78) on already-smoothed fields.
+ *
+ * smoothed: host array of nfilt device pointers, each an n^3 Float32 field, in the
+ *           order the filters are processed (the reference sorts largest radius first).
+ * fcrit:    host [nfilt] Float32 peak thresholds.
+ * mask:     device n^3 Int8, read and updated: every peak sets its own cell to 1.
+ * A cell of the inner cube (nbuff+1 : n-nbuff)^3 is a peak of filter f when the mask is
+ * not 1 there before the call, delta_f >= fcrit[f], no 26-neighbour of delta_f is larger
+ * (ties accepted), and no earlier filter took it.  Output order is the reference CPU
+ * order: filter, then k, j, i.  If total > capacity, the first `capacity` peaks in that
+ * order are written (and masked) and HNT_ERR_CAPACITY is returned.
+ */
+hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed, const float* fcrit,
+                         int32_t nfilt, int32_t n, int32_t nbuff, int8_t* mask, hnt_peaks* out);
+
+/* ---------------------------------------------------------------- shell analysis */
+
+typedef struct {
+    int32_t n1, n2, n3;        /* the transposed read delta[k,i,j] needs n1 == n2 == n3 */
+    const float* delta;        /* device fields, n1*n2*n3 Float32 each */
+    const float* etax;         /* 1LPT displacement psi_1 */
+    const float* etay;
+    const float* etaz;
+    const float* eta2x;        /* 2LPT displacement psi_2, or NULL (read as zeros) */
+    const float* eta2y;
+    const float* eta2z;
+    const float* lapd;         /* Laplacian of delta for d2F, or NULL (d2F = 0) */
+} hnt_fields;
+
+typedef struct {
+    int64_t npeaks;
+    const int32_t* pi;         /* device [npeaks], 1-based peak cell */
+    const int32_t* pj;
+    const int32_t* pk;
+    const double* Rf;          /* device [npeaks], smoothing radius that found the peak (Mpc/h) */
+    const int32_t* ir2min;     /* device [npeaks], first-crossing search floor in r^2 */
+    const double* ZZon;        /* device [npeaks] 1 + z_collapse, or NULL: use ZZon_scalar */
+    const double* fcrit;       /* device [npeaks] crossing threshold, or NULL: fcrit_scalar */
+    double ZZon_scalar;
+    double fcrit_scalar;
+} hnt_batch;
+
+typedef struct {
+    double alatt;              /* lattice spacing (Mpc/h) */
+    int32_t nbuff;             /* mask is only written on (nbuff+1 : n-nbuff)^3 */
+    double rmax2rs;            /* only 0 is supported (the BASELINE setting) */
+    int32_t variant;           /* HNT_VARIANT_A / _B / _C */
+    int32_t early_exit;        /* 1: stop gathering once no later shell can change a result */
+} hnt_params;
+
+/* Per-peak results, device arrays of Float64 (caller-allocated, npeaks entries each;
+ * vector fields hold 3 consecutive values per peak, strain 9 in column-major order).
+ * No-collapse peaks get RTHL = -1, zvir_half = -1 and zeros elsewhere
+ * (RadialShell.no_collapse, src/HaloFinder/RadialShell.jl:86-90). */
+typedef struct {
+    double* RTHL;              /* Lagrangian radius in lattice units */
+    double* Fbarx;             /* mean overdensity within RTHL */
+    double* e_v;
+    double* p_v;
+    double* strain;            /* [9*npeaks] */
+    double* eigs;              /* [3*npeaks] eigenvalues of the trace-matched strain */
+    double* Srb;
+    double* Sbar;              /* [3*npeaks] */
+    double* Sbar2;             /* [3*npeaks] */
+    double* gradpk;            /* [3*npeaks] */
+    double* gradpkf;           /* [3*npeaks] */
+    double* gradpkrf;          /* [3*npeaks] */
+    double* d2F;
+    double* zvir_half;
+    int32_t* nshells_used;     /* optional (NULL): shells gathered per peak, incl. the centre */
+} hnt_results;
+
+/*
+ * hnt_shell_analyse -- replaces analyse_peaks_gpu_cuda_multirf (ext/CUDAExt.jl:2085-2344,
+ * stub src/Huntian.jl:259-281) and analyse_peak_gpu_cuda (ext/CUDAExt.jl:1914-2078), i.e.
+ * the radial-shell collapse of every peak in `batch`: shell gather of delta, psi_1,
+ * psi_2, the transposed delta and lapd; Fbar crossing of fcrit; inward walk with
+ * kernel_strain, eigenvalues and the HomelTab lookup until zvir >= ZZon; RTHL and the
+ * record fields.  mask (device n^3 Int8 or NULL) gets 1 on non-buffer cells within RTHL
+ * of every collapsed peak.
+ */
+hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields, const hnt_batch* batch,
+                             const hnt_shell_tables* tables, const hnt_ctab* ctab,
+                             const hnt_params* params, hnt_results* results, int8_t* mask);
+
+/* ---------------------------------------------------------------- primitive entries */
+/* Single-primitive entry points; the reference exposes them as test harnesses
+ * (src/Huntian.jl:98-217). */
+
+/* Full per-shell gather (shell_gather_full_gpu, ext/CUDAExt.jl:961-1026) for all shells of
+ * every peak.  Device outputs, Float64, shell-major per peak:
+ * F, lapd [nshells*np]; n int32 [nshells*np]; S, S2, G, Gf [3*nshells*np]; SR [9*nshells*np]
+ * (component L fastest, then K for SR).  G, Gf, SR are not divided by the shell radius. */
+hnt_status hnt_shell_gather_full(hnt_ctx* ctx, const hnt_fields* fields, int64_t npeaks,
+                                 const int32_t* pi, const int32_t* pj, const int32_t* pk,
+                                 const hnt_shell_tables* tables,
+                                 double* F, int32_t* n, double* S, double* S2, double* G,
+                                 double* Gf, double* SR, double* lapd);
+
+/* Ascending eigenvalues of symmetric 3x3 matrices (eig3_symmetric_batch_gpu,
+ * ext/CUDAExt.jl:1030-1058; CPU get_evals, RadialShell.jl:215-220).  mats: device
+ * [9*n] column-major, upper triangle used; out: device [3*n]. */
+hnt_status hnt_eig3_symmetric_batch(hnt_ctx* ctx, const double* mats, int64_t n, double* out);
+
+/* Trilinear table lookup with out-of-box -> out_val (interp3_trilinear_gpu,
+ * ext/CUDAExt.jl:1060-1110; CollapseTable.interpolate).  x, y, z, out: device [n]. */
+hnt_status hnt_interp3_trilinear(hnt_ctx* ctx, const hnt_ctab* ctab, const double* x,
+                                 const double* y, const double* z, int64_t n, double* out);
+
+/* One kernel_strain evaluation (kernel_strain_gpu, ext/CUDAExt.jl:1113-1166;
+ * RadialShell.kernel_strain, RadialShell.jl:225-274).  rad [nshells], G, Gf [3*nshells],
+ * SR [9*nshells] are host arrays; mp, mlow, mupp are 1-based shell indices.
+ * out (host, 12): Ebar 11,12,13,22,23,33, grad xyz, gradf xyz. */
+hnt_status hnt_kernel_strain(hnt_ctx* ctx, const double* rad, const double* G,
+                             const double* Gf, const double* SR, int32_t nshells, int32_t mp,
+                             int32_t mlow, int32_t mupp, double wRnor, double aRnor,
+                             double hlatt_1, double hlatt_2, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HUNTIAN_ABI_H */

@@ -1,0 +1,67 @@
+/*
+ * huntian_oracle.h -- TEST INFRASTRUCTURE.  FP64 CPU restatement of Huntian.jl's
+ * peak-finding + radial-shell collapse path, used only as the parity checker by tests/,
+ * by __graft_entry__.smoke() and as the CPU baseline leg of bench.py.  It is never the
+ * product: libhuntian_hip.so does not link or call it.
+ *
+ * Each function cites the reference lines it restates (paths relative to the Huntian.jl
+ * source tree).  Types come from the public ABI header; here every pointer is host memory.
+ */
+#ifndef HUNTIAN_ORACLE_H
+#define HUNTIAN_ORACLE_H
+
+#include "../include/huntian_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* RadialShell.precompute_shells + ShellAnalysisGPU.build_shell_tables */
+int64_t hro_shell_cells(int32_t rmax);
+int32_t hro_shell_count(int32_t rmax);
+int32_t hro_build_shell_tables(int32_t rmax, int32_t* di, int32_t* dj, int32_t* dk,
+                               int32_t* shell_start, int32_t* shell_count, int32_t* shell_r2);
+
+double hro_hRinteg(double u0, double u12);          /* RadialShell.jl:95-121 */
+void hro_atab4(double* akk /* [1101] */);            /* RadialShell.jl:125-147 */
+/* RadialShell.kernel_strain (RadialShell.jl:225-274); 1-based mp/mlow/mupp; G, Gf
+ * [3*nshells] (shell-major: m*3 + L), SR [9*nshells] (m*9 + L*3 + K).  out: Ebar 11,12,13,
+ * 22,23,33, grad xyz, gradf xyz. */
+void hro_kernel_strain(const double* rad, const double* G, const double* Gf, const double* SR,
+                       int32_t mp, int32_t mlow, int32_t mupp, double wRnor, double aRnor,
+                       double hlatt_1, double hlatt_2, double* out);
+/* Ascending eigenvalues of a symmetric 3x3 (column-major, upper triangle used) */
+void hro_eig3(const double* A, double* lam);
+double hro_interp3(const hnt_ctab* ct, double x, double y, double z);
+double hro_fsc_of_z(const hnt_ctab* ct, double z);   /* RadialShell.jl:181-195 */
+
+/* PeakFind.find_peaks (PeakFind.jl:15-78), one filter.  Returns the number of peaks
+ * written (<= max_peaks); 1-based indices. */
+int64_t hro_find_peaks(const float* delta, int8_t* mask, int32_t n1, int32_t n2, int32_t n3,
+                       int32_t nbuff, float fcrit, int64_t max_peaks,
+                       int32_t* pi, int32_t* pj, int32_t* pk, float* pdelta);
+/* The per-filter loop of the drivers (MultiResolution.jl:1046-1066): filters in the given
+ * order, mask carried over.  Returns the total; writes at most cap entries. */
+int64_t hro_peak_find_multi(const float* const* fields, const float* fcrit, int32_t nfilt,
+                            int32_t n, int32_t nbuff, int8_t* mask, int64_t cap,
+                            int32_t* pi, int32_t* pj, int32_t* pk, int32_t* filt, float* pdelta);
+
+/* Per-shell sums of every shell for every peak (the brute-force loop of
+ * test/test_shell_gpu.jl:241-300, in the FP64 arithmetic of analyse_peak_gpu). */
+void hro_shell_gather_full(const hnt_fields* f, int64_t npeaks, const int32_t* pi,
+                           const int32_t* pj, const int32_t* pk, const hnt_shell_tables* t,
+                           double* F, int32_t* n, double* S, double* S2, double* G, double* Gf,
+                           double* SR, double* lapd);
+
+/* analyse_peak (variant A) / analyse_peak_gpu (B) / B + zvir_half (C) over a batch.
+ * nthreads > 1 uses OpenMP over peaks (as Threads.@threads, src/Pipeline.jl:235).
+ * Returns 0 on success, nonzero on invalid input. */
+int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_tables* t,
+                      const hnt_ctab* ct, const hnt_params* p, hnt_results* r, int8_t* mask,
+                      int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,279 @@
+"""TEST INFRASTRUCTURE: ctypes wrapper of the FP64 C oracle (libhuntian_oracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module, and
+only as the checker / CPU baseline.  The product (huntian_hip) never imports it.
+
+The oracle restates the reference CPU path (Huntian.jl src/HaloFinder/PeakFind.jl,
+src/HaloFinder/RadialShell.jl, src/ShellAnalysisGPU.jl,
+src/EllipsoidalCollapse/CollapseTable.jl); see huntian_oracle.c for line citations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(REPO, "huntian.jl_amd"))
+
+from huntian_hip import abi  # noqa: E402  (struct layouts only)
+
+LIB = os.path.join(HERE, "libhuntian_oracle.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_i8p = C.POINTER(C.c_int8)
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.hro_shell_cells.argtypes = [C.c_int32]
+        L.hro_shell_cells.restype = C.c_int64
+        L.hro_shell_count.argtypes = [C.c_int32]
+        L.hro_shell_count.restype = C.c_int32
+        L.hro_build_shell_tables.argtypes = [C.c_int32] + [_i32p] * 6
+        L.hro_build_shell_tables.restype = C.c_int32
+        L.hro_hRinteg.argtypes = [C.c_double, C.c_double]
+        L.hro_hRinteg.restype = C.c_double
+        L.hro_atab4.argtypes = [_f64p]
+        L.hro_kernel_strain.argtypes = [_f64p, _f64p, _f64p, _f64p, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _f64p]
+        L.hro_eig3.argtypes = [_f64p, _f64p]
+        L.hro_interp3.argtypes = [C.POINTER(abi.CTab), C.c_double, C.c_double, C.c_double]
+        L.hro_interp3.restype = C.c_double
+        L.hro_fsc_of_z.argtypes = [C.POINTER(abi.CTab), C.c_double]
+        L.hro_fsc_of_z.restype = C.c_double
+        L.hro_find_peaks.argtypes = [_f32p, _i8p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_float, C.c_int64, _i32p, _i32p, _i32p, _f32p]
+        L.hro_find_peaks.restype = C.c_int64
+        L.hro_peak_find_multi.argtypes = [C.POINTER(_f32p), _f32p, C.c_int32, C.c_int32,
+                                          C.c_int32, _i8p, C.c_int64, _i32p, _i32p, _i32p,
+                                          _i32p, _f32p]
+        L.hro_peak_find_multi.restype = C.c_int64
+        L.hro_shell_gather_full.argtypes = [C.POINTER(abi.Fields), C.c_int64, _i32p, _i32p,
+                                            _i32p, C.POINTER(abi.ShellTables)] + [_f64p, _i32p] + [_f64p] * 6
+        L.hro_analyse_peaks.argtypes = [C.POINTER(abi.Fields), C.POINTER(abi.Batch),
+                                        C.POINTER(abi.ShellTables), C.POINTER(abi.CTab),
+                                        C.POINTER(abi.Params), C.POINTER(abi.Results), _i8p,
+                                        C.c_int32]
+        L.hro_analyse_peaks.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else C.cast(None, C.POINTER(ct))
+
+
+# ---------------------------------------------------------------- shells / small kernels
+
+def shell_tables(rmax: int) -> dict:
+    L = lib()
+    nc = L.hro_shell_cells(rmax)
+    ns = L.hro_shell_count(rmax)
+    t = {k: np.zeros(nc, np.int32) for k in ("di", "dj", "dk")}
+    t.update({k: np.zeros(ns, np.int32) for k in ("shell_start", "shell_count", "shell_r2")})
+    got = L.hro_build_shell_tables(rmax, *[_p(t[k], C.c_int32) for k in
+                                           ("di", "dj", "dk", "shell_start", "shell_count",
+                                            "shell_r2")])
+    assert got == ns
+    t["rmax"] = rmax
+    t["nshells"] = ns
+    t["ncells"] = nc
+    return t
+
+
+def tables_struct(t: dict) -> abi.ShellTables:
+    return abi.ShellTables(t["rmax"], t["nshells"], t["ncells"],
+                           *[_p(t[k], C.c_int32) for k in ("di", "dj", "dk", "shell_start",
+                                                           "shell_count", "shell_r2")])
+
+
+def ctab_struct(ct: dict) -> abi.CTab:
+    tab = np.ascontiguousarray(ct["table"].ravel(order="F"), dtype=np.float32)
+    ct["_flat"] = tab  # keep alive
+    return abi.CTab(ct["nx"], ct["ny"], ct["nz"], ct["x1"], ct["x2"], ct["y1"], ct["y2"],
+                    ct["z1"], ct["z2"], ct.get("out_val", -1.0), _p(tab, C.c_float))
+
+
+def hRinteg(u0, u12):
+    return lib().hro_hRinteg(float(u0), float(u12))
+
+
+def atab4():
+    out = np.zeros(1101)
+    lib().hro_atab4(_p(out, C.c_double))
+    return out
+
+
+def kernel_strain(rad, G, Gf, SR, mp, mlow, mupp, wRnor, aRnor, h1=1.0, h2=1.0):
+    """rad [ns]; G, Gf [ns,3]; SR [ns,3,3] (L,K); 1-based mp/mlow/mupp.
+    Returns (Ebar 3x3, grad 3, gradf 3)."""
+    rad = np.ascontiguousarray(rad, np.float64)
+    G = np.ascontiguousarray(G, np.float64)
+    Gf = np.ascontiguousarray(Gf, np.float64)
+    SR = np.ascontiguousarray(SR, np.float64)
+    out = np.zeros(12)
+    lib().hro_kernel_strain(_p(rad, C.c_double), _p(G, C.c_double), _p(Gf, C.c_double),
+                            _p(SR, C.c_double), mp, mlow, mupp, wRnor, aRnor, h1, h2,
+                            _p(out, C.c_double))
+    e = out
+    E = np.array([[e[0], e[1], e[2]], [e[1], e[3], e[4]], [e[2], e[4], e[5]]])
+    return E, out[6:9].copy(), out[9:12].copy()
+
+
+def eig3(A):
+    A = np.asfortranarray(np.asarray(A, np.float64).reshape(3, 3))
+    flat = np.ascontiguousarray(A.ravel(order="F"))
+    lam = np.zeros(3)
+    lib().hro_eig3(_p(flat, C.c_double), _p(lam, C.c_double))
+    return lam
+
+
+def interp3(ct: dict, x, y, z):
+    s = ctab_struct(ct)
+    return lib().hro_interp3(C.byref(s), float(x), float(y), float(z))
+
+
+def fsc_of_z(ct: dict, z: float) -> float:
+    s = ctab_struct(ct)
+    return lib().hro_fsc_of_z(C.byref(s), float(z))
+
+
+# ---------------------------------------------------------------- peak finding
+
+def find_peaks(delta, mask, nbuff, fcrit, max_peaks=2_000_000):
+    """One filter, as PeakFind.find_peaks.  delta/mask are (n1,n2,n3) Fortran-ordered
+    arrays; mask is updated in place.  Returns (i, j, k, delta) 1-based arrays."""
+    d = np.asfortranarray(delta, dtype=np.float32)
+    assert mask.flags.f_contiguous and mask.dtype == np.int8
+    n1, n2, n3 = d.shape
+    cap = max(1, min(max_peaks, d.size))
+    pi = np.zeros(cap, np.int32); pj = np.zeros(cap, np.int32); pk = np.zeros(cap, np.int32)
+    pd = np.zeros(cap, np.float32)
+    np_ = lib().hro_find_peaks(_p(d, C.c_float), _p(mask, C.c_int8), n1, n2, n3, nbuff,
+                               float(fcrit), max_peaks, _p(pi, C.c_int32), _p(pj, C.c_int32),
+                               _p(pk, C.c_int32), _p(pd, C.c_float))
+    return pi[:np_], pj[:np_], pk[:np_], pd[:np_]
+
+
+def peak_find_multi(fields, fcrits, nbuff, mask, cap=None):
+    fs = [np.asfortranarray(f, dtype=np.float32) for f in fields]
+    n = fs[0].shape[0]
+    if cap is None:
+        cap = max(1, (n - 2 * nbuff) ** 3 // 8)
+    arr = (_f32p * len(fs))(*[_p(f, C.c_float) for f in fs])
+    fc = np.asarray(fcrits, np.float32)
+    out = {k: np.zeros(cap, np.int32) for k in ("i", "j", "k", "filt")}
+    out["delta"] = np.zeros(cap, np.float32)
+    tot = lib().hro_peak_find_multi(arr, _p(fc, C.c_float), len(fs), n, nbuff,
+                                    _p(mask, C.c_int8), cap, _p(out["i"], C.c_int32),
+                                    _p(out["j"], C.c_int32), _p(out["k"], C.c_int32),
+                                    _p(out["filt"], C.c_int32), _p(out["delta"], C.c_float))
+    cnt = min(tot, cap)
+    return {k: v[:cnt] for k, v in out.items()}, tot
+
+
+# ---------------------------------------------------------------- shell analysis
+
+def fields_struct(flds: dict) -> abi.Fields:
+    d = flds["delta"]
+    n1, n2, n3 = d.shape
+    keep = {}
+    ptrs = []
+    for k in ("delta", "etax", "etay", "etaz", "eta2x", "eta2y", "eta2z", "lapd"):
+        a = flds.get(k)
+        if a is None:
+            ptrs.append(C.cast(None, _f32p))
+        else:
+            a = np.asfortranarray(a, dtype=np.float32)
+            keep[k] = a
+            ptrs.append(_p(a, C.c_float))
+    s = abi.Fields(n1, n2, n3, *ptrs)
+    s._keep = keep
+    return s
+
+
+def shell_gather_full(flds: dict, pi, pj, pk, t: dict):
+    fs = fields_struct(flds)
+    ts = tables_struct(t)
+    npk = len(pi)
+    ns = t["nshells"]
+    pi = np.ascontiguousarray(pi, np.int32); pj = np.ascontiguousarray(pj, np.int32)
+    pk = np.ascontiguousarray(pk, np.int32)
+    F = np.zeros((npk, ns)); n = np.zeros((npk, ns), np.int32); lp = np.zeros((npk, ns))
+    S = np.zeros((npk, ns, 3)); S2 = np.zeros((npk, ns, 3)); G = np.zeros((npk, ns, 3))
+    Gf = np.zeros((npk, ns, 3)); SR = np.zeros((npk, ns, 3, 3))  # SR[..., K, L] (L fastest)
+    lib().hro_shell_gather_full(C.byref(fs), npk, _p(pi, C.c_int32), _p(pj, C.c_int32),
+                                _p(pk, C.c_int32), C.byref(ts), _p(F, C.c_double),
+                                _p(n, C.c_int32), _p(S, C.c_double), _p(S2, C.c_double),
+                                _p(G, C.c_double), _p(Gf, C.c_double), _p(SR, C.c_double),
+                                _p(lp, C.c_double))
+    SR = np.swapaxes(SR, -1, -2)  # -> SR[..., L, K]
+    return dict(F=F, n=n, S=S, S2=S2, G=G, Gf=Gf, SR=SR, lapd=lp)
+
+
+def alloc_results(npk: int, with_used=True):
+    r = {k: np.zeros(npk) for k in abi.RESULT_SCALARS}
+    r.update({k: np.zeros((npk, 3)) for k in abi.RESULT_VEC3})
+    r["strain"] = np.zeros((npk, 3, 3))  # [p, K, L] column-major per peak
+    r["nshells_used"] = np.zeros(npk, np.int32) if with_used else None
+    return r
+
+
+def results_struct(r: dict) -> abi.Results:
+    vals = []
+    for name, _ in abi.Results._fields_:
+        a = r.get(name)
+        if a is None:
+            vals.append(C.cast(None, _i32p if name == "nshells_used" else _f64p))
+        else:
+            vals.append(_p(a, C.c_int32 if name == "nshells_used" else C.c_double))
+    return abi.Results(*vals)
+
+
+def analyse_peaks(flds: dict, pi, pj, pk, Rf, ir2min, t: dict, ct: dict, *, alatt, ZZon,
+                  fcrit, nbuff=0, variant="A", early_exit=False, mask=None, nthreads=1,
+                  ZZon_pp=None, fcrit_pp=None):
+    """Variant "A" = RadialShell.analyse_peak, "B" = analyse_peak_gpu, "C" = B + zvir_half.
+    Returns a dict of per-peak arrays (strain as [p, L, K])."""
+    npk = len(pi)
+    fs = fields_struct(flds)
+    ts = tables_struct(t)
+    cs = ctab_struct(ct)
+    pi = np.ascontiguousarray(pi, np.int32); pj = np.ascontiguousarray(pj, np.int32)
+    pk = np.ascontiguousarray(pk, np.int32)
+    Rf = np.ascontiguousarray(np.broadcast_to(np.asarray(Rf, np.float64), (npk,)))
+    ir2 = np.ascontiguousarray(np.broadcast_to(np.asarray(ir2min, np.int32), (npk,)))
+    zz = np.ascontiguousarray(ZZon_pp, np.float64) if ZZon_pp is not None else None
+    fc = np.ascontiguousarray(fcrit_pp, np.float64) if fcrit_pp is not None else None
+    b = abi.Batch(npk, _p(pi, C.c_int32), _p(pj, C.c_int32), _p(pk, C.c_int32),
+                  _p(Rf, C.c_double), _p(ir2, C.c_int32), _p(zz, C.c_double),
+                  _p(fc, C.c_double), float(ZZon), float(fcrit))
+    prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)))
+    r = alloc_results(npk)
+    rs = results_struct(r)
+    if mask is not None:
+        assert mask.flags.f_contiguous and mask.dtype == np.int8
+    st = lib().hro_analyse_peaks(C.byref(fs), C.byref(b), C.byref(ts), C.byref(cs),
+                                 C.byref(prm), C.byref(rs), _p(mask, C.c_int8), nthreads)
+    if st != 0:
+        raise ValueError(f"oracle analyse_peaks failed with code {st}")
+    r["strain"] = np.swapaxes(r["strain"], 1, 2)  # -> [p, L, K]
+    return r
