@@ -51,6 +51,17 @@ hnt_status begin_call(hnt_ctx* ctx) {
     return HNT_OK;
 }
 
+void stage_begin(hnt_ctx* ctx, int stage) {
+    if (!ctx->profiling) return;
+    if (hipEventRecord(ctx->ev[2 * stage], ctx->stream) == hipSuccess) ctx->ev_used[stage] = true;
+}
+
+void stage_end(hnt_ctx* ctx, int stage) {
+    if (!ctx->profiling || !ctx->ev_used[stage]) return;
+    if (hipEventRecord(ctx->ev[2 * stage + 1], ctx->stream) != hipSuccess)
+        ctx->ev_used[stage] = false;
+}
+
 hnt_status finish_call(hnt_ctx* ctx, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -61,6 +72,17 @@ hnt_status finish_call(hnt_ctx* ctx, const char* what) {
     if (e != hipSuccess) {
         set_error("%s: %s", what, hipGetErrorString(e));
         return HNT_ERR_DEVICE;
+    }
+    if (ctx->profiling) {
+        for (int s = 0; s < HNT_NSTAGES; ++s) {
+            if (!ctx->ev_used[s]) continue;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ctx->ev[2 * s], ctx->ev[2 * s + 1]) == hipSuccess) {
+                ctx->stage_ms[s] += ms;
+                ctx->stage_calls[s] += 1;
+            }
+            ctx->ev_used[s] = false;
+        }
     }
     return HNT_OK;
 }
@@ -246,15 +268,40 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
                       &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
                       &ctx->tables.r2, &ctx->tables.rad, &ctx->ctab.table};
     for (DevBuf* b : bufs) release(*b);
+    for (hipEvent_t& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
+    return HNT_OK;
+}
+
+hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on) {
+    HNT_TRY(begin_call(ctx));
+    if (on && !ctx->ev[0])
+        for (hipEvent_t& e : ctx->ev) HNT_HIP(hipEventCreate(&e));
+    ctx->profiling = on != 0;
+    for (int s = 0; s < HNT_NSTAGES; ++s) {
+        ctx->stage_ms[s] = 0.0;
+        ctx->stage_calls[s] = 0;
+        ctx->ev_used[s] = false;
+    }
+    return HNT_OK;
+}
+
+hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t nstages) {
+    clear_error();
+    HNT_REQUIRE(ctx && ms && calls, "hnt_ctx_get_profile: NULL argument");
+    for (int s = 0; s < nstages; ++s) {
+        ms[s] = s < HNT_NSTAGES ? ctx->stage_ms[s] : 0.0;
+        calls[s] = s < HNT_NSTAGES ? ctx->stage_calls[s] : 0;
+    }
     return HNT_OK;
 }
 
 hnt_status hnt_ctx_set_stream(hnt_ctx* ctx, void* hip_stream) {
     clear_error();
     HNT_REQUIRE(ctx != nullptr, "NULL context");
-    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    ctx->stream = (hipStream_t)hip_stream;
     return HNT_OK;
 }
 

@@ -49,6 +49,12 @@ struct hnt_ctx {
         queue, fieldptrs, fcrit, akk, small;
     hnt::TablesDev tables;
     hnt::CTabDev ctab;
+    // stage timers (hnt_ctx_set_profiling)
+    bool profiling = false;
+    hipEvent_t ev[2 * HNT_NSTAGES] = {};
+    bool ev_used[HNT_NSTAGES] = {};
+    double stage_ms[HNT_NSTAGES] = {};
+    int64_t stage_calls[HNT_NSTAGES] = {};
 };
 
 namespace hnt {
@@ -59,6 +65,9 @@ hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);
 hnt_status begin_call(hnt_ctx* ctx);
 hnt_status finish_call(hnt_ctx* ctx, const char* what);
+// bracket a stage with events when profiling is on (no-ops otherwise)
+void stage_begin(hnt_ctx* ctx, int stage);
+void stage_end(hnt_ctx* ctx, int stage);
 
 }  // namespace hnt
 

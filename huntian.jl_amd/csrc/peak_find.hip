@@ -240,8 +240,11 @@ extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
     const float* const* dfields = (const float* const*)ctx->fieldptrs.p;
 
     dim3 g1((m + TI - 1) / TI, (m + TJ - 1) / TJ, (m + TK - 1) / TK);
+    stage_begin(ctx, HNT_STAGE_PF_MARK);
     hipLaunchKernelGGL(pf_mark, g1, dim3(256), 0, st, dfields, (const float*)ctx->fcrit.p, nfilt,
                        n, nbuff, (const int8_t*)mask, (uint8_t*)ctx->claim.p);
+    stage_end(ctx, HNT_STAGE_PF_MARK);
+    stage_begin(ctx, HNT_STAGE_PF_COMPACT);
     hipLaunchKernelGGL(pf_count, dim3((unsigned)nchunks), dim3(256), 0, st,
                        (const uint8_t*)ctx->claim.p, ncell, nfilt, nchunks,
                        (unsigned long long*)ctx->counts.p);
@@ -252,6 +255,7 @@ extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
                        (const uint8_t*)ctx->claim.p, ncell, nchunks,
                        (const unsigned long long*)ctx->offsets.p, n, nbuff, dfields, mask, out->i,
                        out->j, out->k, out->filt, out->delta, out->capacity);
+    stage_end(ctx, HNT_STAGE_PF_COMPACT);
     unsigned long long tail[2] = {0, 0};
     HNT_HIP(hipMemcpyAsync(&tail[0], (unsigned long long*)ctx->offsets.p + (ntab - 1),
                            sizeof(unsigned long long), hipMemcpyDeviceToHost, st));

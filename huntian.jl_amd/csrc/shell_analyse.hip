@@ -858,6 +858,7 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     hipStream_t st = ctx->stream;
 
     // ---- validate the peak cells and ZZon on the device before any gather ----
+    stage_begin(ctx, HNT_STAGE_SA_PREP);
     {
         HNT_TRY(reserve(ctx, ctx->small, 64));
         hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ctx->small.p, 1,
@@ -964,6 +965,9 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
         A.qend[x] = ngroups * (x + 1) / 8;
     }
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, A.qctr, 8, 0ull);
+    stage_end(ctx, HNT_STAGE_SA_PREP);
+    stage_begin(ctx, HNT_STAGE_SA_KERNEL);
     hipLaunchKernelGGL(shell_analyse_kernel, dim3((unsigned)nblocks), dim3(64), 0, st, A);
+    stage_end(ctx, HNT_STAGE_SA_KERNEL);
     return finish_call(ctx, "hnt_shell_analyse");
 }

@@ -19,6 +19,8 @@ HNT_ERR_CAPACITY = 3
 HNT_ERR_UNSUPPORTED = 4
 HNT_ERR_ALLOC = 5
 
+STAGES = ("pf_mark", "pf_compact", "sa_prep", "sa_kernel")
+
 HNT_VARIANT_A = 0
 HNT_VARIANT_B = 1
 HNT_VARIANT_C = 2
@@ -152,6 +154,8 @@ EXPORTED = (
     "hnt_ctx_set_stream",
     "hnt_last_error",
     "hnt_abi_version",
+    "hnt_ctx_set_profiling",
+    "hnt_ctx_get_profile",
     "hnt_shell_table_size",
     "hnt_build_shell_tables",
     "hnt_fsc_of_z",
@@ -187,6 +191,9 @@ def load(path: str | None = None):
     lib.hnt_last_error.argtypes = [C.c_char_p, C.c_size_t]
     lib.hnt_last_error.restype = C.c_size_t
     lib.hnt_abi_version.restype = C.c_int
+    lib.hnt_ctx_set_profiling.argtypes = [vp, C.c_int]
+    lib.hnt_ctx_get_profile.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                        C.c_int32]
     lib.hnt_shell_table_size.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     lib.hnt_build_shell_tables.argtypes = [C.c_int32] + [_i32p] * 6
     lib.hnt_fsc_of_z.argtypes = [C.POINTER(CTab), C.c_double]

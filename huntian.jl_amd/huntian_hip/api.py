@@ -46,7 +46,7 @@ class Context:
         h = C.c_void_p()
         abi.check(self.lib, self.lib.hnt_ctx_create(device, C.byref(h)))
         self.handle = h
-        self._stream_ptr = None
+        self._stream_ptr = -1  # nothing bound yet (0 is the default stream)
 
     def bind_stream(self, stream=None):
         """Run library work on `stream` (a torch.cuda.Stream) or torch's current stream."""
@@ -56,6 +56,17 @@ class Context:
         if ptr != self._stream_ptr:
             abi.check(self.lib, self.lib.hnt_ctx_set_stream(self.handle, C.c_void_p(ptr)))
             self._stream_ptr = ptr
+
+    def set_profiling(self, on: bool = True):
+        abi.check(self.lib, self.lib.hnt_ctx_set_profiling(self.handle, int(bool(on))))
+
+    def profile(self) -> dict:
+        """Summed device milliseconds and call counts per stage since set_profiling."""
+        n = len(abi.STAGES)
+        ms = (C.c_double * n)()
+        calls = (C.c_int64 * n)()
+        abi.check(self.lib, self.lib.hnt_ctx_get_profile(self.handle, ms, calls, n))
+        return {s: (ms[q], calls[q]) for q, s in enumerate(abi.STAGES)}
 
     def close(self):
         if self.handle:

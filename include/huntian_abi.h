@@ -55,11 +55,27 @@ typedef struct hnt_ctx hnt_ctx;
 /* Context on HIP device `device` (0-based).  Replaces set_cuda_device! (src/Huntian.jl:414-424). */
 hnt_status hnt_ctx_create(int device, hnt_ctx** out);
 hnt_status hnt_ctx_destroy(hnt_ctx* ctx);
-/* Run subsequent calls on the caller's hipStream_t (NULL = the context's own stream). */
+/* Run subsequent calls on the caller's hipStream_t, used as given (NULL is the device's
+ * default stream).  A new context uses a stream of its own until this is called. */
 hnt_status hnt_ctx_set_stream(hnt_ctx* ctx, void* hip_stream);
 /* Copy the calling thread's last error message into buf (NUL-terminated); returns its length. */
 size_t hnt_last_error(char* buf, size_t len);
 int hnt_abi_version(void);
+
+/* Per-stage device timers, the counterpart of the reference's profile=true stage timers
+ * ("08_peak_find", "09_shell_analysis", src/MultiResolution.jl:862-875,1068,1149), split
+ * per kernel.  hnt_ctx_set_profiling(ctx, 1) zeroes the accumulators and brackets every
+ * stage of later calls with HIP events on the context's stream;
+ * hnt_ctx_get_profile fills ms[s] (summed device time) and calls[s] for s < nstages. */
+enum {
+    HNT_STAGE_PF_MARK = 0,      /* peak finding: filter sweep */
+    HNT_STAGE_PF_COMPACT = 1,   /* peak finding: count, scan, scatter */
+    HNT_STAGE_SA_PREP = 2,      /* shell analysis: validation, record repack, peak sort */
+    HNT_STAGE_SA_KERNEL = 3,    /* shell analysis: gather and radial logic */
+    HNT_NSTAGES = 4
+};
+hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on);
+hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t nstages);
 
 /* ---------------------------------------------------------------- shells and tables */
 

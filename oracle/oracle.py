@@ -177,7 +177,7 @@ def peak_find_multi(fields, fcrits, nbuff, mask, cap=None):
     fs = [np.asfortranarray(f, dtype=np.float32) for f in fields]
     n = fs[0].shape[0]
     if cap is None:
-        cap = max(1, (n - 2 * nbuff) ** 3 // 8)
+        cap = max(1, min((n - 2 * nbuff) ** 3, 1 << 26))
     arr = (_f32p * len(fs))(*[_p(f, C.c_float) for f in fs])
     fc = np.asarray(fcrits, np.float32)
     out = {k: np.zeros(cap, np.int32) for k in ("i", "j", "k", "filt")}

@@ -1,10 +1,10 @@
 """GPU hot path vs the FP64 oracle on the same seeded inputs.
 
 Bars: peak lists (indices, filter, value) and masks equal; collapse flags and halo counts
-equal; float results within 1e-12 relative of the oracle (the tolerance the north star
-asks for is 1e-5 for RTHL / overdensity and 1e-4 for strain, e_v, p_v; the two sides run
-the same FP64 operations and differ only where device and host log10 / pow differ by an
-ulp)."""
+equal; float results within 1e-10 relative of the oracle.  The north star's tolerance is
+1e-5 for RTHL / overdensity and 1e-4 for strain, e_v, p_v; the two sides run the same FP64
+operations in the same order and differ only where the device and host log10 / pow differ
+by an ulp, which reaches ~1e-12 after the RTHL interpolation and the eigen-solve."""
 import numpy as np
 import pytest
 
@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 FLOAT_KEYS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half", "eigs", "Sbar",
               "Sbar2", "gradpk", "gradpkf", "gradpkrf", "strain")
-RTOL = 1e-12
+RTOL = 1e-10
 
 
 def to_host(r):
@@ -32,12 +32,20 @@ def compare(gpu, ref, label=""):
     for k in FLOAT_KEYS:
         a = gpu[k][:n].reshape(n, -1)
         b = ref[k].reshape(n, -1)
+        # NaN where the reference itself makes one (gradpkrf when the filter-scale shell is
+        # the centre: kernel_strain divides by rad[1] = 0, RadialShell.jl:259-260)
+        assert np.array_equal(np.isnan(a), np.isnan(b)), (label, k, "NaN pattern")
+        a = np.where(np.isnan(a), 0.0, a)
+        b = np.where(np.isnan(b), 0.0, b)
         scale = np.maximum(np.abs(b), 1e-300)
         # entries that are tiny against the rest of their vector are compared absolutely
         vscale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-30)
         err = np.abs(a - b) / np.maximum(scale, 1e-9 * vscale)
         worst[k] = float(err.max()) if err.size else 0.0
         assert worst[k] <= RTOL, (label, k, worst[k])
+    exact = sum(np.array_equal(gpu[k][:n], ref[k], equal_nan=True) for k in FLOAT_KEYS)
+    print(f"{label}: {n} peaks, {exact}/{len(FLOAT_KEYS)} fields bit-identical, worst rel "
+          f"{max(worst.values()):.2e} ({max(worst, key=worst.get)})")
     return worst
 
 
@@ -104,10 +112,10 @@ def test_peak_find_rejects_bad_arguments():
 
 @pytest.fixture(scope="module")
 def bumps():
-    return bumps_field(48, 31415, nbumps=20, margin=10)
+    return bumps_field(56, 31415, nbumps=40, margin=10)
 
 
-def _bump_peaks(oracle, f, nbuff=1, fcrit=0.8):
+def _bump_peaks(oracle, f, nbuff=1, fcrit=0.12):
     m = _mask(f["delta"].shape[0])
     return oracle.find_peaks(f["delta"], m, nbuff, fcrit)
 
@@ -126,10 +134,10 @@ def test_shell_analyse_bumps(oracle, bumps, variant, early):
     rng = np.random.default_rng(5)
     Rf = rng.choice([1.5, 2.2, 3.0, 4.0, np.sqrt(2.5)], size=len(pi))  # incl. an mrf tie
     ir2 = np.array([4 if r < 3 else 9 for r in Rf], np.int32)
-    mref = _mask(48)
+    mref = _mask(56)
     ref = oracle_run(oracle, bumps, pi, pj, pk, Rf, ir2, t, ct, alatt=1.0, ZZon=0.0, fcrit=1.0,
                      variant=variant, early_exit=early, mask=mref, nbuff=2)
-    md = api.to_device_field(_mask(48), dtype=torch.int8)
+    md = api.to_device_field(_mask(56), dtype=torch.int8)
     r = api.shell_analyse(bumps, pi, pj, pk, Rf, ir2, stab, ct, alatt=1.0, ZZon=0.0, fcrit=1.0,
                           variant=variant, early_exit=early, mask=md, nbuff=2)
     g = to_host(r)
@@ -150,9 +158,9 @@ def test_shell_analyse_boundary_and_optional_fields(oracle, bumps):
     t = oracle.shell_tables(rmax)
     stab = api.build_shell_tables(rmax)
     pi, pj, pk, _ = _bump_peaks(oracle, bumps)
-    pi = np.concatenate([pi, [1, 48, 3, 47]]).astype(np.int32)
-    pj = np.concatenate([pj, [1, 20, 46, 2]]).astype(np.int32)
-    pk = np.concatenate([pk, [24, 48, 2, 30]]).astype(np.int32)
+    pi = np.concatenate([pi, [1, 56, 3, 55]]).astype(np.int32)
+    pj = np.concatenate([pj, [1, 20, 54, 2]]).astype(np.int32)
+    pk = np.concatenate([pk, [24, 56, 2, 30]]).astype(np.int32)
     rng = np.random.default_rng(8)
     zz = rng.uniform(-0.5, 0.3, len(pi))
     fc = rng.uniform(0.7, 1.3, len(pi))
@@ -255,12 +263,12 @@ def test_multirf_mirror(oracle, bumps):
     pi, pj, pk, _ = _bump_peaks(oracle, bumps)
     h = len(pi) // 2
     batches = [(pi[:h], pj[:h], pk[:h], 3.0, 9), (pi[h:], pj[h:], pk[h:], 2.0, 4)]
-    mask = _mask(48)
+    mask = _mask(56)
     res, m = api.analyse_peaks_gpu_cuda_multirf(
         bumps["delta"], bumps["etax"], bumps["etay"], bumps["etaz"], bumps["eta2x"],
         bumps["eta2y"], bumps["eta2z"], batches, stab, ct_np["table"], -2.0, 2.0, 0.0, 1.0,
         0.0, 2.0, 1.0, 0.0, fcrit_override=1.0, lapd=bumps["lapd"], mask=mask, nbuff=2)
-    assert len(res) == 2 and m.shape == (48, 48, 48)
+    assert len(res) == 2 and m.shape == (56, 56, 56)
     t = oracle.shell_tables(8)
     for (bi, bj, bk, Rf, ir2), d in zip(batches, res):
         ref = oracle_run(oracle, bumps, bi, bj, bk, Rf, ir2, t, ct_np, alatt=1.0, ZZon=0.0,
