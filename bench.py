@@ -1,0 +1,244 @@
+"""Benchmark of the halo-finding hot path on one MI355X.
+
+Workload (default, the headline configuration): a 1024^3 Float32 tile on a 2048 Mpc/h box
+(alatt 2 Mpc/h), Threefry seed 2024, BBKS P(k) with sigma8 0.81, 1LPT + 2LPT displacements,
+24 top-hat filters log-spaced 1.5..34 Mpc/h processed largest first, peak threshold 1.5,
+64-cell buffer, collapse at z = 0 with HomelTab_websky.dat, shells to 60 Mpc/h (hunt radius
+30 cells, 753 distinct-r^2 shells).  Inputs are synthesised on the GPU before timing and
+stay resident in HBM.
+
+One step = one pass of the hot path: mask reset, multi-filter peak finding over the 24
+smoothed fields (hnt_peak_find), mask reset, radial-shell collapse of every candidate
+(hnt_shell_analyse: validation, record repack, peak sort, gather + radial logic).
+value = candidates / step time.  The CPU baseline is the oracle's FP64 restatement of the
+reference CPU path (RadialShell.analyse_peak, no early exit) on a bounded random sample of
+the same candidates, OpenMP over peaks on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "huntian.jl_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "peaks resolved/s at 1024^3 tile: learned-bracket vs plain shell scan, same GPU"
+
+CONFIGS = {
+    # name: n, box, seed, filters (rmin, rmax, count), nbuff, nhunt
+    "tile1024": dict(n=1024, box=2048.0, seed=2024, rmin=1.5, rmax=34.0, nfilt=24, nbuff=64,
+                     nhunt=30),
+    "tile512": dict(n=512, box=1024.0, seed=7, rmin=1.5, rmax=20.0, nfilt=16, nbuff=32,
+                    nhunt=20),
+    "tile128": dict(n=128, box=256.0, seed=12345, rmin=1.5, rmax=20.0, nfilt=16, nbuff=16,
+                    nhunt=15),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="tile1024", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the oracle baseline sample (0 = skip)")
+    ap.add_argument("--variant", default="C", choices=["A", "B", "C"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 4 GiB device copy (3 times), the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): this build has no "
+              "multi-GPU path; the per-tile work would shard by tiles, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from huntian_hip import api, synth
+    from huntian_hip.homeltab import WEBSKY_TABLE, read_homeltab
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_dispatch": a.numel() * 4}), flush=True)
+        return
+
+    cfg = CONFIGS[args.config]
+    n, box, nbuff, nhunt = cfg["n"], cfg["box"], cfg["nbuff"], cfg["nhunt"]
+    alatt = box / n
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(0)
+
+    # ---------------- inputs (untimed)
+    t0 = time.time()
+    fields, dk = synth.fields_torch(n, box, cfg["seed"], dev)
+    radii = synth.filter_radii(cfg["rmin"], cfg["rmax"], cfg["nfilt"])
+    smoothed = [synth.smooth_tophat_torch(dk, n, box, R) for R in radii]
+    del dk
+    torch.cuda.synchronize()
+    log(f"synthesised {n}^3 fields + {len(radii)} smoothed fields in {time.time() - t0:.1f} s; "
+        f"GPU memory {torch.cuda.memory_allocated() / 2**30:.1f} GiB")
+    ct = read_homeltab(WEBSKY_TABLE)
+    fcrit = api.fsc_of_z(ct, 0.0)
+    fc_peak = [1.5] * len(radii)
+    stab = api.build_shell_tables(nhunt)
+    Rf_tab = torch.tensor(radii, dtype=torch.float64, device=dev)
+    ir2_tab = torch.tensor([api.ir2min_of(R, alatt) for R in radii], dtype=torch.int32,
+                           device=dev)
+    mask = api.fortran_empty((n, n, n), torch.int8, dev)
+    cap = 48_000_000
+    ctx = api.context(0)
+
+    results = None
+
+    def step():
+        nonlocal results
+        mask.zero_()
+        pf = api.peak_find(smoothed, fc_peak, nbuff, mask, capacity=cap)
+        npk = len(pf["i"])
+        if results is None or len(results["RTHL"]) < npk:
+            results = api.alloc_results(int(npk * 1.25) + 1024, dev)
+        Rf = Rf_tab[pf["filt"].long()]
+        ir2 = ir2_tab[pf["filt"].long()]
+        mask.zero_()  # the driver clears the mask between peak finding and shells
+        r = api.shell_analyse(fields, pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct, alatt=alatt,
+                              ZZon=1.0, fcrit=fcrit, nbuff=nbuff, variant=args.variant,
+                              early_exit=True, mask=mask, results=results)
+        return pf, r, npk
+
+    for w in range(args.warmup):
+        t1 = time.time()
+        pf, r, npk = step()
+        torch.cuda.synchronize()
+        log(f"warmup {w}: {npk} candidates, {time.time() - t1:.2f} s")
+
+    ctx.set_profiling(True)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for s in range(args.steps):
+        pf, r, npk = step()
+    ev1.record()
+    torch.cuda.synchronize()
+    total_ms = ev0.elapsed_time(ev1)
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    ms_step = total_ms / args.steps
+    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {npk} candidates; stages {prof}")
+
+    # ---------------- results summary and algorithmic bytes
+    RTHL = r["RTHL"][:npk].cpu().numpy()
+    used = r["nshells_used"][:npk].cpu().numpy().astype(np.int64)
+    nhalo = int((RTHL > 0).sum())
+    cum_cells = np.concatenate([[0], np.cumsum(stab.shell_count)])  # cells in shells 1..s
+    cells = cum_cells[used]
+    rec_bytes = 32  # delta, delta^T, psi1 xyz, psi2 xyz (no lapd in this workload)
+    sa_bytes = float(cells.sum()) * rec_bytes
+    m = n - 2 * nbuff
+    pf_bytes = float(len(radii)) * m ** 3 * 4 + 2.0 * m ** 3  # fields + mask read + claims
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
+    t_pf = stage_ms["pf_mark"] + stage_ms["pf_compact"]
+    t_sa = stage_ms["sa_prep"] + stage_ms["sa_kernel"]
+    kern = {
+        "shell_analyse_kernel": dict(ms=stage_ms["sa_kernel"], bytes=sa_bytes),
+        "pf_mark": dict(ms=stage_ms["pf_mark"], bytes=float(len(radii)) * m ** 3 * 4 + m ** 3),
+    }
+    for k, v in kern.items():
+        v["GBs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "hbm_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(args.config, {}).get(dom)
+        except Exception:
+            traffic = None
+    roof = dict(bound="hbm", achieved=round(kern[dom]["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(kern[dom]["GBs"] / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel=dom, kernel_ms=round(kern[dom]["ms"], 3),
+                algorithmic_bytes=kern[dom]["bytes"])
+
+    # ---------------- CPU baseline: oracle, variant A (the reference CPU path), sample
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle import oracle as O
+
+        cores = min(16, len(os.sched_getaffinity(0)))
+        host = {k: v.cpu().numpy() for k, v in fields.items()}
+        pi = pf["i"].cpu().numpy()
+        pj = pf["j"].cpu().numpy()
+        pk_ = pf["k"].cpu().numpy()
+        Rf_h = np.array(radii)[pf["filt"].cpu().numpy()]
+        ir2_h = np.array([api.ir2min_of(R, alatt) for R in radii], np.int32)[pf["filt"].cpu().numpy()]
+        t_oracle = O.shell_tables(nhunt)
+        rng = np.random.default_rng(0)
+        order = rng.permutation(npk)
+        done, elapsed, nsamp = 0, 0.0, max(cores * 4, 64)
+        while elapsed < args.cpu_seconds and done < npk:
+            sel = order[done:done + nsamp]
+            t1 = time.time()
+            O.analyse_peaks(host, pi[sel], pj[sel], pk_[sel], Rf_h[sel], ir2_h[sel], t_oracle, ct,
+                            alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff, variant="A",
+                            early_exit=False, nthreads=cores)
+            dt = time.time() - t1
+            elapsed += dt
+            done += len(sel)
+            rate = len(sel) / max(dt, 1e-9)
+            nsamp = int(min(npk - done, max(16, rate * (args.cpu_seconds - elapsed))))
+            if nsamp <= 0:
+                break
+        cpu = dict(value=round(done / elapsed, 2), unit="peaks/s", cores=cores, kind="port",
+                   sample=f"{done} of {npk} candidates drawn uniformly (seed 0); FP64 oracle of "
+                          "RadialShell.analyse_peak (variant A, full hunt radius, no early "
+                          f"exit), OpenMP over peaks; {elapsed:.1f} s")
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(npk / (ms_step * 1e-3), 1),
+        "unit": "peaks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (sums of f32 inputs and f32 products, as the reference CPU path)",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {n}^3 Float32 tile, box {box:g} Mpc/h, seed "
+                               f"{cfg['seed']}, {len(radii)} top-hat filters "
+                               f"{cfg['rmin']}-{cfg['rmax']} Mpc/h, threshold 1.5, nbuff "
+                               f"{nbuff}, hunt radius {nhunt} cells ({stab.nshells} shells), "
+                               "z = 0, psi1 + psi2, no lapd",
+                   "method": "plain shell scan (learned bracket not built yet)",
+                   "variant": args.variant, "candidates": npk, "halos": nhalo,
+                   "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
+                   "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
+                   "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
+                   "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+                   "kernels_GBs": {k: round(v["GBs"], 1) for k, v in kern.items()},
+                   "parallelism": "1 GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -183,6 +183,12 @@ def load(path: str | None = None):
     if not os.path.exists(p):
         raise NativeLibraryMissing(
             f"{p} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64, and the library must
+    # bind to that copy (same soname) rather than pull in the system ROCm one beside it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     vp = C.c_void_p
     lib.hnt_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]

@@ -253,3 +253,111 @@ def ir2min_of(Rf, alatt):
     """ir2min = min(floor((1.75 Rf/alatt)^2), floor((40/alatt - 1)^2))
     (src/MultiResolution.jl:575-576)."""
     return min(int(math.floor((1.75 * Rf / alatt) ** 2)), int(math.floor((40.0 / alatt - 1) ** 2)))
+
+
+# ---------------------------------------------------------------- fields (torch, on the GPU)
+
+def _kgrid_torch(n, boxsize, device):
+    import torch
+
+    dk = 2.0 * np.pi / boxsize
+    kx = torch.arange(n // 2 + 1, device=device, dtype=torch.float32) * dk
+    f = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float32) * dk
+    return kx[None, None, :], f[None, :, None], f[:, None, None]
+
+
+def _power_torch(pk: PowerSpectrum, k):
+    import torch
+
+    q = torch.clamp(k / pk.gamma, min=1e-30)
+    T = torch.log(1.0 + 2.34 * q) / (2.34 * q) * (
+        1.0 + 3.89 * q + (16.1 * q) ** 2 + (5.46 * q) ** 3 + (6.71 * q) ** 4) ** -0.25
+    return pk.amp * k ** pk.ns * T * T / (2.0 * np.pi) ** 3
+
+
+def fields_torch(n, boxsize, seed, device, second_order=True, pk: PowerSpectrum | None = None):
+    """delta, psi_1 (and psi_2) on the GPU in float32 / complex64, same recipe as the numpy
+    path.  Returns (dict of column-major (n,n,n) tensors, delta_k (C-order rfft))."""
+    import torch
+
+    pk = pk or PowerSpectrum()
+    noise = white_noise_torch(n, seed, device)          # [i,j,k] view of C-order [k,j,i]
+    fk = torch.fft.rfftn(noise.permute(2, 1, 0))
+    del noise
+    KX, KY, KZ = _kgrid_torch(n, boxsize, device)
+    dk = 2.0 * np.pi / boxsize
+    for z0 in range(0, n, 64):  # slab-wise to bound temporaries
+        sl = slice(z0, min(n, z0 + 64))
+        k2 = KX ** 2 + KY ** 2 + KZ[sl] ** 2
+        k = torch.sqrt(k2)
+        amp = torch.sqrt(_power_torch(pk, torch.where(k2 > 0, k, torch.ones_like(k)))
+                         * dk ** 3 * float(n) ** 3)
+        fk[sl] *= torch.where(k2 > 0, amp, torch.zeros_like(amp))
+    delta = torch.fft.irfftn(fk, s=(n, n, n))
+    del fk
+    out = {"delta": delta.permute(2, 1, 0)}
+    dkk = torch.fft.rfftn(delta)
+    nyq = n // 2
+    ix = torch.arange(n // 2 + 1, device=device)[None, None, :]
+    iy = torch.arange(n, device=device)[None, :, None]
+    iz = torch.arange(n, device=device)[:, None, None]
+
+    def kfilter(fn):
+        res = torch.empty_like(dkk)
+        for z0 in range(0, n, 64):
+            sl = slice(z0, min(n, z0 + 64))
+            k2 = KX ** 2 + KY ** 2 + KZ[sl] ** 2
+            inv = torch.where(k2 > 0, 1.0 / torch.where(k2 > 0, k2, torch.ones_like(k2)),
+                              torch.zeros_like(k2))
+            res[sl] = fn(sl, inv)
+        return res
+
+    def nyq_zero(sl, v):
+        z = (ix == nyq) | (iy == nyq) | (iz[sl] == nyq)
+        return torch.where(z, torch.zeros_like(v), v)
+
+    for name, kc in (("etax", 0), ("etay", 1), ("etaz", 2)):
+        comp = (KX, KY, KZ)[kc]
+        pkk = kfilter(lambda sl, inv: nyq_zero(sl, 1j * (comp if kc < 2 else comp[sl]) * inv * dkk[sl]))
+        out[name] = torch.fft.irfftn(pkk, s=(n, n, n)).permute(2, 1, 0)
+        del pkk
+    if second_order:
+        def phi(a, b):
+            ka, kb = (KX, KY, KZ)[a], (KX, KY, KZ)[b]
+            pkk = kfilter(lambda sl, inv: -(ka if a < 2 else ka[sl]) * (kb if b < 2 else kb[sl])
+                          * inv * dkk[sl])
+            return torch.fft.irfftn(pkk, s=(n, n, n))
+
+        p11, p22, p33 = phi(0, 0), phi(1, 1), phi(2, 2)
+        src = p11 * p22 + p11 * p33 + p22 * p33
+        del p11, p22, p33
+        for a, b in ((0, 1), (0, 2), (1, 2)):
+            p = phi(a, b)
+            src -= p * p
+            del p
+        sk = torch.fft.rfftn(src)
+        del src
+        for name, kc in (("eta2x", 0), ("eta2y", 1), ("eta2z", 2)):
+            comp = (KX, KY, KZ)[kc]
+            pkk = kfilter(lambda sl, inv: nyq_zero(sl, -1j * (comp if kc < 2 else comp[sl]) * inv * sk[sl]))
+            out[name] = torch.fft.irfftn(pkk, s=(n, n, n)).permute(2, 1, 0)
+            del pkk
+        del sk
+    return out, dkk
+
+
+def smooth_tophat_torch(delta_k, n, boxsize, R):
+    """Top-hat smoothing on the GPU (k = 0 untouched), column-major float32 result."""
+    import torch
+
+    KX, KY, KZ = _kgrid_torch(n, boxsize, delta_k.device)
+    out = torch.empty_like(delta_k)
+    for z0 in range(0, n, 64):
+        sl = slice(z0, min(n, z0 + 64))
+        k = torch.sqrt(KX ** 2 + KY ** 2 + KZ[sl] ** 2)
+        x = k * R
+        xs = torch.where(x > 0, x, torch.ones_like(x))
+        w = 3.0 * (torch.sin(xs) - xs * torch.cos(xs)) / xs ** 3
+        w = torch.where(x > 0, w, torch.ones_like(w))
+        out[sl] = delta_k[sl] * w
+    return torch.fft.irfftn(out, s=(n, n, n)).permute(2, 1, 0)
