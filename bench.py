@@ -144,11 +144,22 @@ def main():
     # ---------------- results summary and algorithmic bytes
     RTHL = r["RTHL"][:npk].cpu().numpy()
     used = r["nshells_used"][:npk].cpu().numpy().astype(np.int64)
+    cross = r["crossing_shell"][:npk].cpu().numpy().astype(np.int64)
     nhalo = int((RTHL > 0).sum())
     cum_cells = np.concatenate([[0], np.cumsum(stab.shell_count)])  # cells in shells 1..s
     cells = cum_cells[used]
-    rec_bytes = 32  # delta, delta^T, psi1 xyz, psi2 xyz (no lapd in this workload)
-    sa_bytes = float(cells.sum()) * rec_bytes
+    # algorithmic bytes follow the reference's gather extent (ext/CUDAExt.jl:896-917): every
+    # shell up to and including the first with rad >= rad[crossing] + 3, all shells if there
+    # is no crossing; 32 B per cell (delta, delta^T, psi1 xyz, psi2 xyz; no lapd here)
+    rad = np.sqrt(np.asarray(stab.shell_r2, np.float64))  # rad[s-1] of 1-based shell s
+    ns = len(rad)
+    ref_stop = np.where(cross > 0,
+                        np.minimum(np.searchsorted(rad, rad[np.maximum(cross, 1) - 1] + 3.0,
+                                                   side="left") + 1, ns),
+                        ns)
+    ref_cells = cum_cells[ref_stop]
+    rec_bytes = 32
+    sa_bytes = float(ref_cells.sum()) * rec_bytes
     m = n - 2 * nbuff
     pf_bytes = float(len(radii)) * m ** 3 * 4 + 2.0 * m ** 3  # fields + mask read + claims
     stage_ms = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
@@ -231,6 +242,8 @@ def main():
                    "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
+                   "mean_cells_reference_extent": round(float(ref_cells.mean()), 1)
+                   if npk else 0.0,
                    "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
                    "kernels_GBs": {k: round(v["GBs"], 1) for k, v in kern.items()},
                    "parallelism": "1 GPU"},

@@ -734,7 +734,7 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     bool active = valid;
     bool past = false, crossed = false, stop_set = false, oobA = false;
     double rstop = 0.0;
-    int stop = m;
+    int stop = m, scross = 0;
     for (int s = 2; s <= m; ++s) {
         if (__ballot(active) == 0ull) break;
         const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
@@ -775,13 +775,16 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
             X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
             X[K_LAP] = lap;
             X[K_LAPN] = (double)lapn;
-            // early exit: nothing beyond r_cross + 3 (and the gradient window) is read later
+            // early exit: with s_c the first shell past ir2min where Fbar < fcrit, the radial
+            // logic reads Fbar up to s_c + 1 and kernel_strain inputs of shells with
+            // rad < rad[s_c + 1] + 2 (and the filter-scale window up to mupp_rf)
             if (!past && a.shell_r2[s - 1] > ir2min) past = true;
             if (past && !crossed && Fb < fcrit) {
                 crossed = true;
-                rstop = radS + 3.0;
+                scross = s;
+                rstop = rad[s + 1] + 2.0;
             }
-            if (!stop_set && crossed && radS >= rstop && s >= mupp_rf) {
+            if (!stop_set && crossed && rad[s + 1] >= rstop && s >= mupp_rf) {
                 stop = s;
                 stop_set = true;
                 if (a.early) active = false;
@@ -790,6 +793,7 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     }
     if (!valid) return;
     if (a.res.nshells_used) a.res.nshells_used[p] = stop;
+    if (a.res.crossing_shell) a.res.crossing_shell[p] = scross;
     post_process(a, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
 }
 

@@ -118,6 +118,7 @@ class Params(C.Structure):
 
 RESULT_SCALARS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half")
 RESULT_VEC3 = ("eigs", "Sbar", "Sbar2", "gradpk", "gradpkf", "gradpkrf")
+RESULT_INT = ("nshells_used", "crossing_shell")
 
 
 class Results(C.Structure):
@@ -137,6 +138,7 @@ class Results(C.Structure):
         ("d2F", _f64p),
         ("zvir_half", _f64p),
         ("nshells_used", _i32p),
+        ("crossing_shell", _i32p),
     ]
 
 

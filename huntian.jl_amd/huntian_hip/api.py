@@ -327,7 +327,8 @@ def alloc_results(npk: int, device="cuda"):
     r.update({k: torch.empty((max(npk, 1), 3), dtype=torch.float64, device=device)
               for k in abi.RESULT_VEC3})
     r["strain"] = torch.empty((max(npk, 1), 3, 3), dtype=torch.float64, device=device)
-    r["nshells_used"] = torch.empty(max(npk, 1), dtype=torch.int32, device=device)
+    for k in abi.RESULT_INT:
+        r[k] = torch.empty(max(npk, 1), dtype=torch.int32, device=device)
     return r
 
 
@@ -335,7 +336,7 @@ def _results_struct(r):
     vals = []
     for name, _ in abi.Results._fields_:
         t = r.get(name)
-        ct = C.c_int32 if name == "nshells_used" else C.c_double
+        ct = C.c_int32 if name in abi.RESULT_INT else C.c_double
         vals.append(abi.as_ptr(_ptr(t), ct))
     return abi.Results(*vals)
 

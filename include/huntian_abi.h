@@ -201,6 +201,8 @@ typedef struct {
     double* d2F;
     double* zvir_half;
     int32_t* nshells_used;     /* optional (NULL): shells gathered per peak, incl. the centre */
+    int32_t* crossing_shell;   /* optional (NULL): first shell (1-based) past ir2min whose Fbar
+                                  is below fcrit, 0 if there is none within the hunt radius */
 } hnt_results;
 
 /*

@@ -490,7 +490,8 @@ static inline void eig_full(const double E[9], double lam[3]) {
 static void analyse_one(const hnt_fields* f, const hnt_shell_tables* t, const hnt_ctab* ct,
                         const double* akk, const hnt_params* prm, int32_t ci, int32_t cj,
                         int32_t ck, double Rf, int32_t ir2min, double ZZon, double fcrit,
-                        prof_t* w, res_t* out, int8_t* mask, int32_t* nused) {
+                        prof_t* w, res_t* out, int8_t* mask, int32_t* nused,
+                        int32_t* ncross) {
     const int variant = prm->variant;
     const double alatt = prm->alatt;
     const int32_t n1 = f->n1, n2 = f->n2, n3 = f->n3;
@@ -626,19 +627,26 @@ static void analyse_one(const hnt_fields* f, const hnt_shell_tables* t, const hn
         for (int q = 0; q < 9; ++q) w->SR[s * 9 + q] = srv[q] / rad[s];
         w->lapA[s] = lapA; w->cntA[s] = cntA; w->lapB[s] = lapB; w->cntB[s] = cntB;
 
-        /* early-exit rule: after the first shell past ir2min whose Fbar drops below fcrit,
-         * nothing the radial logic reads lies beyond radius r_cross + 2, nor beyond the
-         * window of the filter-scale gradient (ext/CUDAExt.jl:774-781,896-917 states the
-         * same rule with a Float32 margin; here Fbar is the exact FP64 value both sides use) */
+        /* early-exit rule.  Let s_c be the first shell past ir2min whose Fbar drops below
+         * fcrit.  The radial logic then has m0 <= s_c; it reads Fbar up to s_c (s_c + 1 for
+         * the formation-redshift radius when RTHL rounds up to rad[m0]) and the kernel_strain
+         * inputs of shells with rad < rad[s_c + 1] + 2, plus the filter-scale window up to
+         * mupp_rf.  The gather stops before the first shell beyond all of these.  The
+         * reference states the same rule with a Float32 margin (ext/CUDAExt.jl:774-781,
+         * 896-917); here Fbar is the exact FP64 value both sides use. */
         if (!past_m0 && t->shell_r2[s - 1] > ir2min) past_m0 = 1;
-        if (past_m0 && !crossed && w->Fbar[s] < fcrit) { crossed = 1; rstop = rad[s] + 3.0; }
-        if (!stop_set && crossed && rad[s] >= rstop && s >= mupp_rf) {
+        if (past_m0 && !crossed && w->Fbar[s] < fcrit) {
+            crossed = s;
+            rstop = rad[s + 1] + 2.0;
+        }
+        if (!stop_set && crossed && rad[s + 1] >= rstop && s >= mupp_rf) {
             stop = s;
             stop_set = 1;
             if (prm->early_exit) { last = s; break; }
         }
     }
     if (nused) *nused = stop;
+    if (ncross) *ncross = crossed;
     /* shells not gathered must never be read: poison them */
     for (int s = last + 1; s <= m + 1 && s <= NS; ++s) {
         w->Fbar[s] = NAN;
@@ -982,11 +990,12 @@ int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_t
                 double ZZ = b->ZZon ? b->ZZon[q] : b->ZZon_scalar;
                 double fc = b->fcrit ? b->fcrit[q] : b->fcrit_scalar;
                 if (!(ZZ > -1.0)) { bad |= 16; no_collapse(&v); store(r, q, &v); continue; }
-                int32_t nu = 0;
+                int32_t nu = 0, nx = 0;
                 analyse_one(f, t, ct, akk, p, b->pi[q], b->pj[q], b->pk[q], b->Rf[q],
-                            b->ir2min[q], ZZ, fc, &w, &v, mask, &nu);
+                            b->ir2min[q], ZZ, fc, &w, &v, mask, &nu, &nx);
                 store(r, q, &v);
                 if (r->nshells_used) r->nshells_used[q] = nu;
+                if (r->crossing_shell) r->crossing_shell[q] = nx;
             }
             prof_free(&w);
         }

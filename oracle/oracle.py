@@ -233,7 +233,8 @@ def alloc_results(npk: int, with_used=True):
     r = {k: np.zeros(npk) for k in abi.RESULT_SCALARS}
     r.update({k: np.zeros((npk, 3)) for k in abi.RESULT_VEC3})
     r["strain"] = np.zeros((npk, 3, 3))  # [p, K, L] column-major per peak
-    r["nshells_used"] = np.zeros(npk, np.int32) if with_used else None
+    for k in abi.RESULT_INT:
+        r[k] = np.zeros(npk, np.int32) if with_used else None
     return r
 
 
@@ -242,9 +243,9 @@ def results_struct(r: dict) -> abi.Results:
     for name, _ in abi.Results._fields_:
         a = r.get(name)
         if a is None:
-            vals.append(C.cast(None, _i32p if name == "nshells_used" else _f64p))
+            vals.append(C.cast(None, _i32p if name in abi.RESULT_INT else _f64p))
         else:
-            vals.append(_p(a, C.c_int32 if name == "nshells_used" else C.c_double))
+            vals.append(_p(a, C.c_int32 if name in abi.RESULT_INT else C.c_double))
     return abi.Results(*vals)
 
 

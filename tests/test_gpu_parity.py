@@ -143,6 +143,7 @@ def test_shell_analyse_bumps(oracle, bumps, variant, early):
     g = to_host(r)
     compare(g, ref, f"{variant} early={early}")
     assert np.array_equal(g["nshells_used"], ref["nshells_used"])
+    assert np.array_equal(g["crossing_shell"], ref["crossing_shell"])
     assert np.array_equal(md.cpu().numpy(), mref)
     assert (ref["RTHL"] > 0).sum() >= 3
 
