@@ -264,7 +264,7 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->recs, &ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp,
                       &ctx->keys_a, &ctx->keys_b, &ctx->vals_a, &ctx->vals_b, &ctx->prof,
-                      &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small,
+                      &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small, &ctx->stops,
                       &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
                       &ctx->tables.r2, &ctx->tables.rad, &ctx->ctab.table};
     for (DevBuf* b : bufs) release(*b);

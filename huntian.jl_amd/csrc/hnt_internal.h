@@ -46,7 +46,7 @@ struct hnt_ctx {
     int num_cus = 0;
     // buffers reused across calls (grow only)
     hnt::DevBuf recs, claim, counts, offsets, cub_tmp, keys_a, keys_b, vals_a, vals_b, prof,
-        queue, fieldptrs, fcrit, akk, small;
+        queue, fieldptrs, fcrit, akk, small, stops;
     hnt::TablesDev tables;
     hnt::CTabDev ctab;
     // stage timers (hnt_ctx_set_profiling)

@@ -266,6 +266,32 @@ __device__ __forceinline__ int closest_shell(const int32_t* r2, int lo, int hi, 
     return best;
 }
 
+// Filter-scale gradient shell mrf and its window (RadialShell.jl:450-482 for A,
+// ShellAnalysisGPU.jl:233-266 for B/C): the first shell whose r^2 is nearest to
+// (Rf/alatt)^2, if within 10, recorded two shells below (A; one below for r^2 = 1) or one
+// below (B, C).
+__device__ void rf_window(const SAArgs& a, double Rf, int& mrf, int& mlow_rf, int& mupp_rf) {
+    const double* __restrict__ rad = a.rad;
+    const int NS = a.NS;
+    const int m = NS - 1;
+    mrf = 1;
+    const double rf = Rf / a.alatt;
+    const double rf2 = rf * rf;
+    double dist;
+    if (a.variant == HNT_VARIANT_A) {
+        int s = closest_shell(a.shell_r2, 2, NS, rf2, &dist);
+        if (s >= 2 && dist < 10.0) mrf = (s == 2) ? 1 : s - 2;
+    } else {
+        int s = closest_shell(a.shell_r2, 2, m, rf2, &dist);
+        if (s >= 2 && dist < 10.0) mrf = s - 1;
+    }
+    double rlow_rf = rad[mrf] - 2.0;
+    if (rlow_rf < 0.0) rlow_rf = 0.0;
+    mlow_rf = first_gt(rad, 1, mrf - 1, rlow_rf);
+    if (mlow_rf > mrf - 1) mlow_rf = mrf;
+    mupp_rf = first_ge(rad, mrf + 1, m, rad[mrf] + 2.0) - 1;
+}
+
 struct Acc {
     double F, S0, S1, S2, T0, T1, T2, G0, G1, G2, H0, H1, H2;
     double R00, R01, R02, R10, R11, R12, R20, R21, R22;
@@ -671,28 +697,8 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     const int NS = a.NS;
     const int m = NS - 1;
 
-    // filter-scale gradient shell mrf and its window (RadialShell.jl:450-482 for A,
-    // ShellAnalysisGPU.jl:233-266 for B/C): the first shell whose r^2 is nearest to
-    // (Rf/alatt)^2, if within 10, recorded two shells below (A; one below for r^2 = 1)
-    // or one below (B, C)
     int mrf = 1, mlow_rf = 1, mupp_rf = 1;
-    if (valid) {
-        const double rf = Rf / a.alatt;
-        const double rf2 = rf * rf;
-        double dist;
-        if (a.variant == HNT_VARIANT_A) {
-            int s = closest_shell(a.shell_r2, 2, NS, rf2, &dist);
-            if (s >= 2 && dist < 10.0) mrf = (s == 2) ? 1 : s - 2;
-        } else {
-            int s = closest_shell(a.shell_r2, 2, m, rf2, &dist);
-            if (s >= 2 && dist < 10.0) mrf = s - 1;
-        }
-        double rlow_rf = rad[mrf] - 2.0;
-        if (rlow_rf < 0.0) rlow_rf = 0.0;
-        mlow_rf = first_gt(rad, 1, mrf - 1, rlow_rf);
-        if (mlow_rf > mrf - 1) mlow_rf = mrf;
-        mupp_rf = first_ge(rad, mrf + 1, m, rad[mrf] + 2.0) - 1;
-    }
+    if (valid) rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
 
     // ---- gather ----
     const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
@@ -795,6 +801,121 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     if (a.res.nshells_used) a.res.nshells_used[p] = stop;
     if (a.res.crossing_shell) a.res.crossing_shell[p] = scross;
     post_process(a, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
+}
+
+// ------------------------------------------------------------------ crossing pass
+// Work of a peak = shells gathered, fixed by its crossing shell s_c (first shell past ir2min
+// with Fbar < fcrit) and its filter-scale window.  This pass finds s_c from delta alone,
+// with the main gather's arithmetic (same cell order, same FP64 sums and Fbar recurrence),
+// so its stop shell is the one the main gather will reach; the main gather then runs over
+// peaks ordered by that stop shell and a wavefront's 64 lanes finish together.
+// Workgroup b takes group (b % 8) * G8 + b / 8: each XCD walks its own contiguous eighth of
+// the (ir2min, Morton) order.
+__global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, const float* __restrict__ delta,
+                                                      int64_t ngroups, int64_t G8,
+                                                      int32_t* __restrict__ stop_out) {
+    const int64_t g = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
+    if (g >= ngroups) return;
+    const int64_t idx = g * 64 + threadIdx.x;
+    const bool valid = idx < a.npeaks;
+    const int p = valid ? a.order[idx] : 0;
+    int ci = 1, cj = 1, ck = 1, ir2min = 0;
+    double fcrit = 0.0, Rf = 0.0;
+    if (valid) {
+        ci = a.pi[p];
+        cj = a.pj[p];
+        ck = a.pk[p];
+        ir2min = a.ir2min[p];
+        Rf = a.Rf[p];
+        fcrit = a.fcrit ? a.fcrit[p] : a.fcrit_s;
+    }
+    const double* __restrict__ rad = a.rad;
+    const int m = a.NS - 1;
+    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
+    const int R = a.rmax;
+    const bool inside = ci - R >= 1 && ci + R <= a.n1 && cj - R >= 1 && cj + R <= a.n2 &&
+                        ck - R >= 1 && ck + R <= a.n3;
+    const bool wave_inside = __ballot(!valid || inside) == ~0ull;
+
+    double Fprev = valid ? (double)delta[c0] : 0.0;
+    double dFbarp = Fprev;
+    bool active = valid, past = false;
+    int scross = 0;
+    for (int s = 2; s <= m; ++s) {
+        if (__ballot(active) == 0ull) break;
+        const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
+        if (active) {
+            double F = 0.0;
+            int64_t nin = 0;
+            if (wave_inside) {
+                int c = 0;
+                for (; c + 4 <= nc; c += 4) {
+                    float v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int4 o = a.cells[st + c + u];
+                        v[u] = delta[c0 + o.x + o.y * a.s1 + o.z * a.s2];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) F += (double)v[u];
+                }
+                for (; c < nc; ++c) {
+                    const int4 o = a.cells[st + c];
+                    F += (double)delta[c0 + o.x + o.y * a.s1 + o.z * a.s2];
+                }
+                nin = nc;
+            } else {
+                for (int c = 0; c < nc; ++c) {
+                    const int4 o = a.cells[st + c];
+                    const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
+                    if (i1 < 1 || i1 > a.n1 || i2 < 1 || i2 > a.n2 || i3 < 1 || i3 > a.n3)
+                        continue;
+                    F += (double)delta[c0 + o.x + o.y * a.s1 + o.z * a.s2];
+                    nin += 1;
+                }
+            }
+            const double dFbar = nin > 0 ? F / (double)nin : 0.0;
+            const double rad3p = cube(rad[s - 1]), rad3 = cube(rad[s]);
+            const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
+            Fprev = Fb;
+            dFbarp = dFbar;
+            if (!past && a.shell_r2[s - 1] > ir2min) past = true;
+            if (past && Fb < fcrit) {
+                scross = s;
+                active = false;
+            }
+        }
+    }
+    if (!valid) return;
+    // the main gather stops at the first s >= max(s_c, mupp_rf) with rad[s+1] >= rad[s_c+1]+2
+    int stop = m;
+    if (scross > 0) {
+        int mrf, mlow_rf, mupp_rf;
+        rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
+        const int s0 = scross > mupp_rf ? scross : mupp_rf;
+        if (s0 <= m) {
+            // first s in [s0, m] with rad[s + 1] >= rstop, i.e. first shell index t = s + 1
+            // in [s0 + 1, m + 1] with rad[t] >= rstop
+            const int t = first_ge(rad, s0 + 1, m + 1, rad[scross + 1] + 2.0);
+            if (t <= m + 1) stop = t - 1;
+        }
+    }
+    stop_out[p] = stop;
+}
+
+// Sort key of the main pass: stop shell, then Morton code of the peak cell.
+__global__ void stop_key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+                                const int32_t* __restrict__ pk, const int32_t* __restrict__ stop,
+                                int64_t np, uint64_t* __restrict__ keys,
+                                int32_t* __restrict__ vals) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    uint64_t mort = spread3((uint32_t)(pi[q] - 1) & 0x1fff) |
+                    (spread3((uint32_t)(pj[q] - 1) & 0x1fff) << 1) |
+                    (spread3((uint32_t)(pk[q] - 1) & 0x1fff) << 2);
+    uint64_t g = (uint64_t)min(max(stop[q], 0), (1 << 24) - 1);
+    keys[q] = (g << 39) | mort;
+    vals[q] = (int32_t)q;
 }
 
 __global__ __launch_bounds__(64) void shell_analyse_kernel(SAArgs a) {
@@ -970,6 +1091,24 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     }
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, A.qctr, 8, 0ull);
     stage_end(ctx, HNT_STAGE_SA_PREP);
+
+    // ---- crossing pass: each peak's stop shell, then order the peaks by it ----
+    if (A.early) {
+        stage_begin(ctx, HNT_STAGE_SA_CROSS);
+        HNT_TRY(reserve(ctx, ctx->stops, (size_t)np * 4));
+        const int64_t G8 = (ngroups + 7) / 8;
+        hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A,
+                           fields->delta, ngroups, G8, (int32_t*)ctx->stops.p);
+        hipLaunchKernelGGL(stop_key_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
+                           batch->pi, batch->pj, batch->pk, (const int32_t*)ctx->stops.p, np,
+                           (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);
+        HNT_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp.p, tmp_bytes,
+                                                   (uint64_t*)ctx->keys_a.p,
+                                                   (uint64_t*)ctx->keys_b.p,
+                                                   (int32_t*)ctx->vals_a.p,
+                                                   (int32_t*)ctx->vals_b.p, (int)np, 0, 64, st));
+        stage_end(ctx, HNT_STAGE_SA_CROSS);
+    }
     stage_begin(ctx, HNT_STAGE_SA_KERNEL);
     hipLaunchKernelGGL(shell_analyse_kernel, dim3((unsigned)nblocks), dim3(64), 0, st, A);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);

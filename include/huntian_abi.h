@@ -72,7 +72,8 @@ enum {
     HNT_STAGE_PF_COMPACT = 1,   /* peak finding: count, scan, scatter */
     HNT_STAGE_SA_PREP = 2,      /* shell analysis: validation, record repack, peak sort */
     HNT_STAGE_SA_KERNEL = 3,    /* shell analysis: gather and radial logic */
-    HNT_NSTAGES = 4
+    HNT_STAGE_SA_CROSS = 4,     /* shell analysis: crossing pass and work-ordered peak sort */
+    HNT_NSTAGES = 5
 };
 hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on);
 hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t nstages);
