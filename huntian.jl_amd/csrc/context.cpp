@@ -145,11 +145,35 @@ hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t) {
     }
     std::vector<double> rad((size_t)ns + 2, 0.0);
     for (int32_t s = 1; s <= ns; ++s) rad[s] = std::sqrt((double)t->shell_r2[s - 1]);
+    // kernel_strain weights (RadialShell.jl:236-244): the same IEEE operations the device
+    // would do per term, done once per table
+    std::vector<int32_t> kidx((size_t)3 * (ns + 2), 0);
+    std::vector<double> kw;
+    for (int32_t mp = 2; mp <= ns; ++mp) {
+        const double rmp = rad[mp];
+        int32_t lo = mp, hi = mp;
+        while (lo > 1 && (rmp - rad[lo - 1]) * (rmp - rad[lo - 1]) < 4.0) --lo;
+        while (hi < ns && (rmp - rad[hi + 1]) * (rmp - rad[hi + 1]) < 4.0) ++hi;
+        kidx[3 * mp] = lo;
+        kidx[3 * mp + 1] = hi;
+        kidx[3 * mp + 2] = (int32_t)kw.size();
+        for (int32_t m1 = lo; m1 <= hi; ++m1) {
+            const double rm1 = rad[m1];
+            const double u0 = rmp - rm1;
+            const double u02 = u0 * u0;
+            const double u12 = rmp * rmp + rm1 * rm1;
+            const double dd = u12 - u02;
+            kw.push_back(hRinteg(u0, u12) / (dd * dd));
+        }
+    }
+    if (kw.empty()) kw.push_back(0.0);
     HNT_TRY(reserve(ctx, T.cells, cells4.size() * sizeof(int32_t)));
     HNT_TRY(reserve(ctx, T.start, (size_t)ns * sizeof(int32_t)));
     HNT_TRY(reserve(ctx, T.count, (size_t)ns * sizeof(int32_t)));
     HNT_TRY(reserve(ctx, T.r2, (size_t)ns * sizeof(int32_t)));
     HNT_TRY(reserve(ctx, T.rad, rad.size() * sizeof(double)));
+    HNT_TRY(reserve(ctx, T.kw, kw.size() * sizeof(double)));
+    HNT_TRY(reserve(ctx, T.kw_idx, kidx.size() * sizeof(int32_t)));
     HNT_HIP(hipMemcpyAsync(T.cells.p, cells4.data(), cells4.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, ctx->stream));
     HNT_HIP(hipMemcpyAsync(T.start.p, t->shell_start, ns * sizeof(int32_t), hipMemcpyHostToDevice,
@@ -159,6 +183,10 @@ hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t) {
     HNT_HIP(hipMemcpyAsync(T.r2.p, t->shell_r2, ns * sizeof(int32_t), hipMemcpyHostToDevice,
                            ctx->stream));
     HNT_HIP(hipMemcpyAsync(T.rad.p, rad.data(), rad.size() * sizeof(double),
+                           hipMemcpyHostToDevice, ctx->stream));
+    HNT_HIP(hipMemcpyAsync(T.kw.p, kw.data(), kw.size() * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HNT_HIP(hipMemcpyAsync(T.kw_idx.p, kidx.data(), kidx.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, ctx->stream));
     HNT_HIP(hipStreamSynchronize(ctx->stream));
     T.rmax = t->rmax;
@@ -266,7 +294,8 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
                       &ctx->keys_a, &ctx->keys_b, &ctx->vals_a, &ctx->vals_b, &ctx->prof,
                       &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small, &ctx->stops,
                       &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
-                      &ctx->tables.r2, &ctx->tables.rad, &ctx->ctab.table};
+                      &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
+                      &ctx->tables.kw_idx, &ctx->ctab.table};
     for (DevBuf* b : bufs) release(*b);
     for (hipEvent_t& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

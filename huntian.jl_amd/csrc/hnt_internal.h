@@ -28,7 +28,12 @@ struct TablesDev {
     std::vector<int32_t> host;  // di, dj, dk, start, count, r2 concatenated
     DevBuf cells;               // int4 {di, dj, dk, 0} per cell
     DevBuf start, count, r2;    // int32 [nshells]
-    DevBuf rad;                 // double [nshells + 1], rad[s] = sqrt(r2 of shell s), 1-based
+    DevBuf rad;                 // double [nshells + 2], rad[s] = sqrt(r2 of shell s), 1-based
+    // kernel_strain weights hRinteg(u0, u12) / (u12 - u0^2)^2 of shell m1 around shell mp,
+    // for the shells with u0^2 < 4 (u0 = rad[mp] - rad[m1]): m1 in [kw_lo[mp], kw_hi[mp]],
+    // weight at kw[kw_off[mp] + m1 - kw_lo[mp]]; mp = 2 .. nshells
+    DevBuf kw;                  // double
+    DevBuf kw_idx;              // int32 [3 * (nshells + 2)]: lo, hi, off of each mp
 };
 
 struct CTabDev {

@@ -63,6 +63,8 @@ struct SAArgs {
     const int32_t* shell_count;
     const int32_t* shell_r2;
     const double* rad;  // 1-based
+    const double* kw;   // kernel_strain weight table
+    const int32_t* kidx;
     int NS, rmax;
     CTabView ct;
     const double* akk;
@@ -159,35 +161,35 @@ struct KS {
 };
 
 // kernel_strain (RadialShell.jl:225-274) over the lane's profile slot P.
-__device__ KS kstrain(const double* __restrict__ rad, const double* __restrict__ P,
-                      const double* __restrict__ akk, int mp, int mlow, int mupp, double wRnor,
-                      double aRnor) {
+// The shells with u0^2 < 4 form the contiguous range [kidx[3mp], kidx[3mp+1]]; their
+// weights hRinteg(u0, u12) / (u12 - u0^2)^2 come precomputed from the table kw.
+__device__ KS kstrain(const SAArgs& a, const double* __restrict__ P, int mp, int mlow,
+                      int mupp) {
+    const double* __restrict__ rad = a.rad;
+    const double* __restrict__ akk = a.akk;
+    const double wRnor = a.wRnor, aRnor = a.aRnor;
     KS r = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (mp > 1) {
         const double rmp = rad[mp];
-        for (int m1 = mlow; m1 <= mupp; ++m1) {
-            const double rm1 = rad[m1];
-            const double u0 = rmp - rm1;
-            const double u02 = u0 * u0;
-            if (u02 < 4.0) {
-                const double u12 = rmp * rmp + rm1 * rm1;
-                const double dd = u12 - u02;
-                const double wt = hRinteg(u0, u12) / (dd * dd);
-                const double* k = P + (int64_t)m1 * PW;
-                const double hw = 0.5 * wt;
-                r.g0 += wt * k[0];
-                r.g1 += wt * k[1];
-                r.g2 += wt * k[2];
-                r.f0 += wt * k[3];
-                r.f1 += wt * k[4];
-                r.f2 += wt * k[5];
-                r.E11 -= hw * k[6];
-                r.E12 -= hw * k[7];
-                r.E13 -= hw * k[8];
-                r.E22 -= hw * k[9];
-                r.E23 -= hw * k[10];
-                r.E33 -= hw * k[11];
-            }
+        const int lo0 = a.kidx[3 * mp], hi0 = a.kidx[3 * mp + 1];
+        const double* __restrict__ W = a.kw + (a.kidx[3 * mp + 2] - lo0);
+        const int lo = mlow > lo0 ? mlow : lo0, hi = mupp < hi0 ? mupp : hi0;
+        for (int m1 = lo; m1 <= hi; ++m1) {
+            const double wt = W[m1];
+            const double* k = P + (int64_t)m1 * PW;
+            const double hw = 0.5 * wt;
+            r.g0 += wt * k[0];
+            r.g1 += wt * k[1];
+            r.g2 += wt * k[2];
+            r.f0 += wt * k[3];
+            r.f1 += wt * k[4];
+            r.f2 += wt * k[5];
+            r.E11 -= hw * k[6];
+            r.E12 -= hw * k[7];
+            r.E13 -= hw * k[8];
+            r.E22 -= hw * k[9];
+            r.E23 -= hw * k[10];
+            r.E33 -= hw * k[11];
         }
         const double sc = wRnor / rmp;
         r.E11 *= sc; r.E12 *= sc; r.E13 *= sc; r.E22 *= sc; r.E23 *= sc; r.E33 *= sc;
@@ -421,7 +423,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
     const int variant = a.variant;
     auto Fb = [&](int s) -> double { return P[(int64_t)s * PW + K_F]; };
 
-    const KS krf = kstrain(rad, P, a.akk, mrf, mlow_rf, mupp_rf, a.wRnor, a.aRnor);
+    const KS krf = kstrain(a, P, mrf, mlow_rf, mupp_rf);
 
     // m0: first shell with r^2 > ir2min, else the last finalised shell
     int m0 = first_r2_gt(a.shell_r2, 2, m, ir2min);
@@ -463,7 +465,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
     int mlow = first_gt(rad, 1, m0 - 1, rlow);
     if (mlow > m0 - 1) mlow = m0;
 
-    const KS k0 = kstrain(rad, P, a.akk, m0, mlow, mupp, a.wRnor, a.aRnor);
+    const KS k0 = kstrain(a, P, m0, mlow, mupp);
     double lam[3], Fr, ev, pv;
     eig_ks(k0, lam);
     ev_from_lam(lam, &Fr, &ev, &pv);
@@ -491,7 +493,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
                 if (rad[m1] > rlow_mp) mlownew = m1;
             mlow = mlownew;
         }
-        const KS kk = kstrain(rad, P, a.akk, mp, mlow, mupp, a.wRnor, a.aRnor);
+        const KS kk = kstrain(a, P, mp, mlow, mupp);
         double e_mp, p_mp, Fr_mp;
         eig_ks(kk, lam);
         ev_from_lam(lam, &Fr_mp, &e_mp, &p_mp);
@@ -643,7 +645,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
                 int f = first_gt(rad, 1, m - 1, rcur);
                 m0_rc = f <= m - 1 ? f : m - 1;
             }
-            const KS kr = kstrain(rad, P, a.akk, m0_rc, mlow_rc, mupp_rc, a.wRnor, a.aRnor);
+            const KS kr = kstrain(a, P, m0_rc, mlow_rc, mupp_rc);
             double Frr, e_rc, p_rc;
             eig_ks(kr, lam);
             ev_from_lam(lam, &Frr, &e_rc, &p_rc);
@@ -1050,6 +1052,8 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     A.shell_count = (const int32_t*)ctx->tables.count.p;
     A.shell_r2 = (const int32_t*)ctx->tables.r2.p;
     A.rad = (const double*)ctx->tables.rad.p;
+    A.kw = (const double*)ctx->tables.kw.p;
+    A.kidx = (const int32_t*)ctx->tables.kw_idx.p;
     A.NS = tables->nshells;
     A.rmax = tables->rmax;
     A.ct = CTabView{(const float*)ctx->ctab.table.p, ctab->nx, ctab->ny, ctab->nz, ctab->x1,
