@@ -292,10 +292,12 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->recs, &ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp,
                       &ctx->keys_a, &ctx->keys_b, &ctx->vals_a, &ctx->vals_b, &ctx->prof,
-                      &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small, &ctx->stops,
+                      &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small, &ctx->stops, &ctx->feats,
                       &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
                       &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
-                      &ctx->tables.kw_idx, &ctx->ctab.table};
+                      &ctx->tables.kw_idx, &ctx->ctab.table, &ctx->mlp.wt, &ctx->mlp.x0,
+                      &ctx->mlp.z, &ctx->mlp.h, &ctx->mlp.y, &ctx->mlp.dy, &ctx->mlp.dyf,
+                      &ctx->mlp.dz, &ctx->mlp.gw, &ctx->mlp.part, &ctx->mlp.gradv};
     for (DevBuf* b : bufs) release(*b);
     for (hipEvent_t& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

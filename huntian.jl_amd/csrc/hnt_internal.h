@@ -36,6 +36,12 @@ struct TablesDev {
     DevBuf kw_idx;              // int32 [3 * (nshells + 2)]: lo, hi, off of each mp
 };
 
+// scratch of the bracket MLP (bf16 weights and activations, gradients)
+struct MlpScratch {
+    DevBuf wt, x0, z, h, y, dy, dyf, dz, gw, part, gradv;
+    int64_t Bp = 0;
+};
+
 struct CTabDev {
     int32_t nx = 0, ny = 0, nz = 0;
     std::vector<float> host;
@@ -51,9 +57,10 @@ struct hnt_ctx {
     int num_cus = 0;
     // buffers reused across calls (grow only)
     hnt::DevBuf recs, claim, counts, offsets, cub_tmp, keys_a, keys_b, vals_a, vals_b, prof,
-        queue, fieldptrs, fcrit, akk, small, stops;
+        queue, fieldptrs, fcrit, akk, small, stops, feats;
     hnt::TablesDev tables;
     hnt::CTabDev ctab;
+    hnt::MlpScratch mlp;
     // stage timers (hnt_ctx_set_profiling)
     bool profiling = false;
     hipEvent_t ev[2 * HNT_NSTAGES] = {};
@@ -70,6 +77,9 @@ hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);
 hnt_status begin_call(hnt_ctx* ctx);
 hnt_status finish_call(hnt_ctx* ctx, const char* what);
+hnt_status mlp_predict(hnt_ctx* ctx, const hnt_mlp* net, const float* x, int64_t B, float* out);
+hnt_status bracket_features(hnt_ctx* ctx, const hnt_fields* fields, const hnt_batch* batch,
+                            double alatt, float* out);
 // bracket a stage with events when profiling is on (no-ops otherwise)
 void stage_begin(hnt_ctx* ctx, int stage);
 void stage_end(hnt_ctx* ctx, int stage);

@@ -905,6 +905,17 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, const float* __r
     stop_out[p] = stop;
 }
 
+// Predicted gather extent (network output 1 = log(1 + shells)), clamped to [1, m].
+__global__ void pred_stop_kernel(const float* __restrict__ pred, int64_t np, int m,
+                                 int32_t* __restrict__ stop) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    float s = expf(pred[q * HNT_MLP_OUTPUTS + 1]) - 1.0f;
+    if (!(s >= 1.0f)) s = 1.0f;  // NaN too
+    if (s > (float)m) s = (float)m;
+    stop[q] = (int32_t)(s + 0.5f);
+}
+
 // Sort key of the main pass: stop shell, then Morton code of the peak cell.
 __global__ void stop_key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
                                 const int32_t* __restrict__ pk, const int32_t* __restrict__ stop,
@@ -958,6 +969,13 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
                     (fields->eta2y == nullptr) == (fields->eta2z == nullptr),
                 "hnt_shell_analyse: eta2x/y/z must be all set or all NULL");
     HNT_REQUIRE(params->variant >= 0 && params->variant <= 2, "hnt_shell_analyse: bad variant");
+    HNT_REQUIRE(params->order >= HNT_ORDER_CROSSING && params->order <= HNT_ORDER_LEARNED,
+                "hnt_shell_analyse: bad order %d", params->order);
+    HNT_REQUIRE(params->order != HNT_ORDER_LEARNED ||
+                    (params->bracket && params->bracket->params && params->bracket->norm),
+                "hnt_shell_analyse: HNT_ORDER_LEARNED needs the bracket network");
+    HNT_REQUIRE(params->order != HNT_ORDER_LEARNED || tables->nshells > 12,
+                "hnt_shell_analyse: HNT_ORDER_LEARNED needs more than 12 shells");
     if (params->rmax2rs != 0.0) {
         set_error("hnt_shell_analyse: rmax2rs > 0 is not supported by this build");
         return HNT_ERR_UNSUPPORTED;
@@ -1096,13 +1114,23 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, A.qctr, 8, 0ull);
     stage_end(ctx, HNT_STAGE_SA_PREP);
 
-    // ---- crossing pass: each peak's stop shell, then order the peaks by it ----
-    if (A.early) {
+    // ---- schedule: order the peaks by their (exact or predicted) gather extent ----
+    if (A.early && params->order != HNT_ORDER_PLAIN) {
         stage_begin(ctx, HNT_STAGE_SA_CROSS);
         HNT_TRY(reserve(ctx, ctx->stops, (size_t)np * 4));
-        const int64_t G8 = (ngroups + 7) / 8;
-        hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A,
-                           fields->delta, ngroups, G8, (int32_t*)ctx->stops.p);
+        if (params->order == HNT_ORDER_CROSSING) {
+            const int64_t G8 = (ngroups + 7) / 8;
+            hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A,
+                               fields->delta, ngroups, G8, (int32_t*)ctx->stops.p);
+        } else {
+            HNT_TRY(reserve(ctx, ctx->feats, (size_t)np * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4));
+            float* feats = (float*)ctx->feats.p;
+            float* pred = feats + np * HNT_MLP_INPUTS;
+            HNT_TRY(bracket_features(ctx, fields, batch, params->alatt, feats));
+            HNT_TRY(mlp_predict(ctx, params->bracket, feats, np, pred));
+            hipLaunchKernelGGL(pred_stop_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
+                               st, pred, np, A.NS - 1, (int32_t*)ctx->stops.p);
+        }
         hipLaunchKernelGGL(stop_key_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
                            batch->pi, batch->pj, batch->pk, (const int32_t*)ctx->stops.p, np,
                            (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);

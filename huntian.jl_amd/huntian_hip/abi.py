@@ -106,6 +106,23 @@ class Batch(C.Structure):
     ]
 
 
+MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS = 18, 128, 3
+MLP_NPARAM = 18 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 3 + 3
+HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
+ORDERS = {"crossing": HNT_ORDER_CROSSING, "plain": HNT_ORDER_PLAIN,
+          "learned": HNT_ORDER_LEARNED}
+
+
+class Mlp(C.Structure):
+    _fields_ = [
+        ("params", _f32p),
+        ("norm", _f32p),
+        ("adam_m", _f32p),
+        ("adam_v", _f32p),
+        ("step", C.c_int64),
+    ]
+
+
 class Params(C.Structure):
     _fields_ = [
         ("alatt", C.c_double),
@@ -113,6 +130,8 @@ class Params(C.Structure):
         ("rmax2rs", C.c_double),
         ("variant", C.c_int32),
         ("early_exit", C.c_int32),
+        ("order", C.c_int32),
+        ("bracket", C.POINTER(Mlp)),
     ]
 
 
@@ -167,6 +186,10 @@ EXPORTED = (
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
     "hnt_kernel_strain",
+    "hnt_bracket_features",
+    "hnt_mlp_forward",
+    "hnt_mlp_gradient",
+    "hnt_mlp_train_step",
 )
 
 _lib = None
@@ -220,6 +243,13 @@ def load(path: str | None = None):
     lib.hnt_kernel_strain.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _f64p]
+    lib.hnt_bracket_features.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
+                                         C.POINTER(ShellTables), C.c_double, _f32p]
+    lib.hnt_mlp_forward.argtypes = [vp, C.POINTER(Mlp), _f32p, C.c_int64, _f32p]
+    lib.hnt_mlp_gradient.argtypes = [vp, C.POINTER(Mlp), _f32p, _f32p, C.c_int64, _f32p,
+                                     C.POINTER(C.c_double)]
+    lib.hnt_mlp_train_step.argtypes = [vp, C.POINTER(Mlp), _f32p, _f32p, C.c_int64, C.c_float,
+                                       C.POINTER(C.c_double)]
     for name in EXPORTED:
         if name not in ("hnt_last_error", "hnt_abi_version", "hnt_fsc_of_z"):
             getattr(lib, name).restype = C.c_int

@@ -344,9 +344,12 @@ def _results_struct(r):
 def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct: dict, *,
                   alatt: float, ZZon: float, fcrit: float, nbuff: int = 0, variant="C",
                   early_exit: bool = True, mask=None, ZZon_pp=None, fcrit_pp=None,
-                  results=None):
+                  results=None, order: str = "crossing", bracket=None):
     """hnt_shell_analyse over one batch of peaks (device arrays or numpy).  Returns a dict of
-    device tensors: scalars [np], vectors [np, 3], strain [np, 3, 3] ([p, L, K])."""
+    device tensors: scalars [np], vectors [np, 3], strain [np, 3, 3] ([p, L, K]).
+
+    order: wavefront schedule ("crossing", "plain" or "learned" with `bracket`, a
+    huntian_hip.bracket.BracketNet); it changes the speed, never a result."""
     torch = _require_gpu()
     ctx = context()
     fs, keep_f = _fields_struct(fields)
@@ -365,7 +368,13 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
                   abi.as_ptr(_ptr(pk_), C.c_int32), abi.as_ptr(_ptr(Rf_), C.c_double),
                   abi.as_ptr(_ptr(ir_), C.c_int32), abi.as_ptr(_ptr(zz), C.c_double),
                   abi.as_ptr(_ptr(fc), C.c_double), float(ZZon), float(fcrit))
-    prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)))
+    net = None
+    if order == "learned":
+        if bracket is None:
+            raise ValueError("order='learned' needs a bracket network")
+        net = bracket.struct()
+    prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)),
+                     abi.ORDERS[order], C.pointer(net) if net is not None else None)
     r = results if results is not None else alloc_results(npk, dev)
     rs = _results_struct(r)
     cs, keep_t = _ctab_struct(ct)

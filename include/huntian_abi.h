@@ -180,7 +180,17 @@ typedef struct {
     double rmax2rs;            /* only 0 is supported (the BASELINE setting) */
     int32_t variant;           /* HNT_VARIANT_A / _B / _C */
     int32_t early_exit;        /* 1: stop gathering once no later shell can change a result */
+    int32_t order;             /* wavefront schedule of the peaks, HNT_ORDER_* (0: crossing pass);
+                                  it changes the speed, never a result */
+    const struct hnt_mlp* bracket;  /* network for HNT_ORDER_LEARNED, else unused */
 } hnt_params;
+
+/* Peak schedules of hnt_shell_analyse (early_exit = 1).
+ *  CROSSING: a delta-only pass finds each peak's crossing shell, hence its exact gather
+ *            extent; wavefronts hold peaks of equal extent.
+ *  PLAIN:    (ir2min, Morton code) order, no extra pass.
+ *  LEARNED:  the bracket network predicts the extent from the first 12 shells. */
+enum { HNT_ORDER_CROSSING = 0, HNT_ORDER_PLAIN = 1, HNT_ORDER_LEARNED = 2 };
 
 /* Per-peak results, device arrays of Float64 (caller-allocated, npeaks entries each;
  * vector fields hold 3 consecutive values per peak, strain 9 in column-major order).
@@ -251,6 +261,41 @@ hnt_status hnt_kernel_strain(hnt_ctx* ctx, const double* rad, const double* G,
                              const double* Gf, const double* SR, int32_t nshells, int32_t mp,
                              int32_t mlow, int32_t mupp, double wRnor, double aRnor,
                              double hlatt_1, double hlatt_2, double* out);
+
+/* ---------------------------------------------------------------- bracket network */
+/* No counterpart in the reference.  An MLP 18 -> 128 -> 128 -> 128 -> 3 with SiLU, run in
+ * BF16 on the matrix cores with FP32 accumulation.  Inputs (hnt_bracket_features): mean
+ * delta of shells 1..12, (e, p) of the symmetrised psi1 gradient over shells 2..6 and
+ * 2..12, mean delta over shells 1..12, Rf / alatt.  Outputs: log(1 + crossing shell),
+ * log(1 + gather extent in shells), collapse logit.  Parameters: one FP32 vector, row-major
+ * W1 [18x128], b1, W2 [128x128], b2, W3 [128x128], b3, W4 [128x3], b4. */
+#define HNT_MLP_INPUTS 18
+#define HNT_MLP_HIDDEN 128
+#define HNT_MLP_OUTPUTS 3
+#define HNT_MLP_NPARAM (18 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 3 + 3)
+
+typedef struct hnt_mlp {
+    float* params;             /* device [HNT_MLP_NPARAM] FP32 master weights */
+    const float* norm;         /* device [2 * 18]: input mean, then input scale */
+    float* adam_m;             /* device [HNT_MLP_NPARAM] Adam moments (training only) */
+    float* adam_v;
+    int64_t step;              /* Adam step count */
+} hnt_mlp;
+
+/* Features of every peak in batch (batch->Rf used, ZZon / fcrit ignored): out device
+ * [npeaks * 18] FP32. */
+hnt_status hnt_bracket_features(hnt_ctx* ctx, const hnt_fields* fields, const hnt_batch* batch,
+                                const hnt_shell_tables* tables, double alatt, float* out);
+/* out device [B * 3] = network(x), x device [B * 18]. */
+hnt_status hnt_mlp_forward(hnt_ctx* ctx, const hnt_mlp* net, const float* x, int64_t B,
+                           float* out);
+/* Gradient of the mean loss over the batch, loss = (y0 - t0)^2 + (y1 - t1)^2 +
+ * BCE-with-logits(y2, t2): grad device [HNT_MLP_NPARAM]; *loss (host, may be NULL). */
+hnt_status hnt_mlp_gradient(hnt_ctx* ctx, const hnt_mlp* net, const float* x,
+                            const float* target, int64_t B, float* grad, double* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; advances net->step. */
+hnt_status hnt_mlp_train_step(hnt_ctx* ctx, hnt_mlp* net, const float* x, const float* target,
+                              int64_t B, float lr, double* loss);
 
 #ifdef __cplusplus
 }
