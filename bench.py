@@ -52,6 +52,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0 = skip)")
     ap.add_argument("--variant", default="C", choices=["A", "B", "C"])
+    ap.add_argument("--order", default="crossing", choices=["crossing", "plain", "learned"],
+                    help="wavefront schedule of the shell analysis (never changes a result)")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip timing the shell analysis under the other schedules")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 4 GiB device copy (3 times), the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -102,6 +106,13 @@ def main():
     mask = api.fortran_empty((n, n, n), torch.int8, dev)
     cap = 48_000_000
     ctx = api.context(0)
+    from huntian_hip import bracket
+
+    net = None
+    if os.path.exists(bracket.DEFAULT_WEIGHTS):
+        net = bracket.BracketNet.load(bracket.DEFAULT_WEIGHTS, device=dev)
+    elif args.order == "learned":
+        raise SystemExit(f"--order learned needs trained weights at {bracket.DEFAULT_WEIGHTS}")
 
     results = None
 
@@ -117,7 +128,8 @@ def main():
         mask.zero_()  # the driver clears the mask between peak finding and shells
         r = api.shell_analyse(fields, pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct, alatt=alatt,
                               ZZon=1.0, fcrit=fcrit, nbuff=nbuff, variant=args.variant,
-                              early_exit=True, mask=mask, results=results)
+                              early_exit=True, mask=mask, results=results, order=args.order,
+                              bracket=net)
         return pf, r, npk
 
     for w in range(args.warmup):
@@ -140,6 +152,33 @@ def main():
     ctx.set_profiling(False)
     ms_step = total_ms / args.steps
     log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {npk} candidates; stages {prof}")
+
+    # ---------------- schedule ablation: shell analysis alone under each schedule
+    ablation = {}
+    if not args.no_ablation:
+        Rf = Rf_tab[pf["filt"].long()]
+        ir2 = ir2_tab[pf["filt"].long()]
+        for o in ("crossing", "plain", "learned"):
+            if o == "learned" and net is None:
+                continue
+            res2 = api.alloc_results(npk, dev)
+            for rep in range(2):  # first call warms the schedule's buffers
+                mask.zero_()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                r2 = api.shell_analyse(fields, pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct,
+                                       alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff,
+                                       variant=args.variant, early_exit=True, mask=mask,
+                                       results=res2, order=o, bracket=net)
+                e1.record()
+                torch.cuda.synchronize()
+            same = bool(torch.equal(r2["RTHL"][:npk], r["RTHL"][:npk]) and
+                        torch.equal(r2["nshells_used"][:npk], r["nshells_used"][:npk]))
+            ablation[o] = dict(ms=round(e0.elapsed_time(e1), 3), same_results=same)
+            del r2, res2
+            log(f"schedule {o}: {ablation[o]}")
 
     # ---------------- results summary and algorithmic bytes
     RTHL = r["RTHL"][:npk].cpu().numpy()
@@ -237,7 +276,8 @@ def main():
                                f"{cfg['rmin']}-{cfg['rmax']} Mpc/h, threshold 1.5, nbuff "
                                f"{nbuff}, hunt radius {nhunt} cells ({stab.nshells} shells), "
                                "z = 0, psi1 + psi2, no lapd",
-                   "method": "plain shell scan (learned bracket not built yet)",
+                   "method": f"shell scan with exact early exit, schedule '{args.order}'",
+                   "schedule_ablation_shell_ms": ablation,
                    "variant": args.variant, "candidates": npk, "halos": nhalo,
                    "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,

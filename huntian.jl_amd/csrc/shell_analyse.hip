@@ -294,46 +294,78 @@ __device__ void rf_window(const SAArgs& a, double Rf, int& mrf, int& mlow_rf, in
     mupp_rf = first_ge(rad, mrf + 1, m, rad[mrf] + 2.0) - 1;
 }
 
-struct Acc {
-    double F, S0, S1, S2, T0, T1, T2, G0, G1, G2, H0, H1, H2;
-    double R00, R01, R02, R10, R11, R12, R20, R21, R22;
-    double lap;
-    int64_t lapn, nin;
+// Per-shell sums of Float32 terms.  SumF64: FP64 accumulation, the reference CPU
+// arithmetic.  SumKahan: FP32 accumulation with Kahan compensation (the running error c is
+// subtracted from each new term), read out as s - c in FP64.
+struct SumF64 {
+    double s;
+    __device__ __forceinline__ void zero() { s = 0.0; }
+    __device__ __forceinline__ void add(float v) { s += (double)v; }
+    __device__ __forceinline__ double val() const { return s; }
+};
+struct SumKahan {
+    float s, c;
+    __device__ __forceinline__ void zero() {
+        s = 0.0f;
+        c = 0.0f;
+    }
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double val() const { return (double)s - (double)c; }
 };
 
-__device__ __forceinline__ void acc_cell(Acc& A, const float4 v0, const float4 v1, float fx,
-                                         float fy, float fz, bool eta2) {
-    A.F += (double)v0.x;
-    A.S0 += (double)v0.z;
-    A.S1 += (double)v0.w;
-    A.S2 += (double)v1.x;
-    if (eta2) {
-        A.T0 += (double)v1.y;
-        A.T1 += (double)v1.z;
-        A.T2 += (double)v1.w;
+template <class S>
+struct Acc {
+    S F, S0, S1, S2, T0, T1, T2, G0, G1, G2, H0, H1, H2;
+    S R00, R01, R02, R10, R11, R12, R20, R21, R22;
+    double lap;  // running over all shells so far, FP64 in both modes
+    int64_t lapn, nin;
+    __device__ __forceinline__ void zero() {
+        F.zero(); S0.zero(); S1.zero(); S2.zero(); T0.zero(); T1.zero(); T2.zero();
+        G0.zero(); G1.zero(); G2.zero(); H0.zero(); H1.zero(); H2.zero();
+        R00.zero(); R01.zero(); R02.zero(); R10.zero(); R11.zero(); R12.zero();
+        R20.zero(); R21.zero(); R22.zero();
     }
-    // Float32 products (Float32 * Int32 is Float32 in the reference), summed in FP64
-    A.G0 += (double)(v0.x * fx);
-    A.G1 += (double)(v0.x * fy);
-    A.G2 += (double)(v0.x * fz);
-    A.H0 += (double)(v0.y * fx);
-    A.H1 += (double)(v0.y * fy);
-    A.H2 += (double)(v0.y * fz);
-    A.R00 += (double)(v0.z * fx);
-    A.R01 += (double)(v0.z * fy);
-    A.R02 += (double)(v0.z * fz);
-    A.R10 += (double)(v0.w * fx);
-    A.R11 += (double)(v0.w * fy);
-    A.R12 += (double)(v0.w * fz);
-    A.R20 += (double)(v1.x * fx);
-    A.R21 += (double)(v1.x * fy);
-    A.R22 += (double)(v1.x * fz);
+};
+
+template <class S>
+__device__ __forceinline__ void acc_cell(Acc<S>& A, const float4 v0, const float4 v1, float fx,
+                                         float fy, float fz, bool eta2) {
+    A.F.add(v0.x);
+    A.S0.add(v0.z);
+    A.S1.add(v0.w);
+    A.S2.add(v1.x);
+    if (eta2) {
+        A.T0.add(v1.y);
+        A.T1.add(v1.z);
+        A.T2.add(v1.w);
+    }
+    // Float32 products (Float32 * Int32 is Float32 in the reference)
+    A.G0.add(v0.x * fx);
+    A.G1.add(v0.x * fy);
+    A.G2.add(v0.x * fz);
+    A.H0.add(v0.y * fx);
+    A.H1.add(v0.y * fy);
+    A.H2.add(v0.y * fz);
+    A.R00.add(v0.z * fx);
+    A.R01.add(v0.z * fy);
+    A.R02.add(v0.z * fz);
+    A.R10.add(v0.w * fx);
+    A.R11.add(v0.w * fy);
+    A.R12.add(v0.w * fz);
+    A.R20.add(v1.x * fx);
+    A.R21.add(v1.x * fy);
+    A.R22.add(v1.x * fz);
 }
 
 // Gather one shell for one lane.  CHECK = the wave holds a peak whose ball leaves the grid.
-template <bool CHECK>
+template <bool CHECK, class S>
 __device__ __forceinline__ void gather_shell(const SAArgs& a, int st, int nc, int ci, int cj,
-                                             int ck, int64_t c0, Acc& A, bool& oobA) {
+                                             int ck, int64_t c0, Acc<S>& A, bool& oobA) {
     const bool eta2 = a.has_eta2;
     const bool lapd = a.has_lapd;
     const bool varA = a.variant == HNT_VARIANT_A;
@@ -679,6 +711,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
     R.gradpkrf[o3] = krf.g0; R.gradpkrf[o3 + 1] = krf.g1; R.gradpkrf[o3 + 2] = krf.g2;
 }
 
+template <class S>
 __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__ P) {
     const int lane = threadIdx.x;
     const int64_t idx = grp * 64 + lane;
@@ -748,33 +781,33 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
         const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
         const double radS = rad[s], radP = rad[s - 1];
         if (active) {
-            Acc A;
-            A.F = 0.0; A.S0 = 0.0; A.S1 = 0.0; A.S2 = 0.0; A.T0 = 0.0; A.T1 = 0.0; A.T2 = 0.0;
-            A.G0 = 0.0; A.G1 = 0.0; A.G2 = 0.0; A.H0 = 0.0; A.H1 = 0.0; A.H2 = 0.0;
-            A.R00 = 0.0; A.R01 = 0.0; A.R02 = 0.0; A.R10 = 0.0; A.R11 = 0.0; A.R12 = 0.0;
-            A.R20 = 0.0; A.R21 = 0.0; A.R22 = 0.0;
+            Acc<S> A;
+            A.zero();
             A.lap = lap; A.lapn = lapn; A.nin = 0;
-            if (wave_inside) gather_shell<false>(a, st, nc, ci, cj, ck, c0, A, oobA);
-            else gather_shell<true>(a, st, nc, ci, cj, ck, c0, A, oobA);
+            if (wave_inside) gather_shell<false, S>(a, st, nc, ci, cj, ck, c0, A, oobA);
+            else gather_shell<true, S>(a, st, nc, ci, cj, ck, c0, A, oobA);
             lap = A.lap;
             lapn = A.lapn;
             const int64_t nin = A.nin;
-            const double dFbar = nin > 0 ? A.F / (double)nin : 0.0;
+            const double dFbar = nin > 0 ? A.F.val() / (double)nin : 0.0;
             const double rad3p = cube(radP), rad3 = cube(radS);
             const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
             Fprev = Fb;
             dFbarp = dFbar;
             if (nin > 0) {
                 cumN += (double)nin;
-                cS0 += A.S0; cS1 += A.S1; cS2 += A.S2;
-                cT0 += A.T0; cT1 += A.T1; cT2 += A.T2;
+                cS0 += A.S0.val(); cS1 += A.S1.val(); cS2 += A.S2.val();
+                cT0 += A.T0.val(); cT1 += A.T1.val(); cT2 += A.T2.val();
             }
-            const double r00 = A.R00 / radS, r01 = A.R01 / radS, r02 = A.R02 / radS;
-            const double r10 = A.R10 / radS, r11 = A.R11 / radS, r12 = A.R12 / radS;
-            const double r20 = A.R20 / radS, r21 = A.R21 / radS, r22 = A.R22 / radS;
+            const double r00 = A.R00.val() / radS, r01 = A.R01.val() / radS,
+                         r02 = A.R02.val() / radS;
+            const double r10 = A.R10.val() / radS, r11 = A.R11.val() / radS,
+                         r12 = A.R12.val() / radS;
+            const double r20 = A.R20.val() / radS, r21 = A.R21.val() / radS,
+                         r22 = A.R22.val() / radS;
             double* X = P + (int64_t)s * PW;
-            X[0] = A.G0 / radS; X[1] = A.G1 / radS; X[2] = A.G2 / radS;
-            X[3] = A.H0 / radS; X[4] = A.H1 / radS; X[5] = A.H2 / radS;
+            X[0] = A.G0.val() / radS; X[1] = A.G1.val() / radS; X[2] = A.G2.val() / radS;
+            X[3] = A.H0.val() / radS; X[4] = A.H1.val() / radS; X[5] = A.H2.val() / radS;
             X[6] = r00 + r00; X[7] = r01 + r10; X[8] = r02 + r20;
             X[9] = r11 + r11; X[10] = r12 + r21; X[11] = r22 + r22;
             X[K_F] = Fb;
@@ -931,6 +964,7 @@ __global__ void stop_key_kernel(const int32_t* __restrict__ pi, const int32_t* _
     vals[q] = (int32_t)q;
 }
 
+template <class S>
 __global__ __launch_bounds__(64) void shell_analyse_kernel(SAArgs a) {
     double* P = a.prof + ((int64_t)blockIdx.x * 64 + threadIdx.x) * a.slot_doubles;
     int q = blockIdx.x & 7;  // workgroups are dispatched round-robin over the 8 XCDs
@@ -944,7 +978,7 @@ __global__ __launch_bounds__(64) void shell_analyse_kernel(SAArgs a) {
             q = (q + 1) & 7;
             continue;
         }
-        analyse_group(a, grp, P);
+        analyse_group<S>(a, grp, P);
     }
 }
 
@@ -971,6 +1005,8 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     HNT_REQUIRE(params->variant >= 0 && params->variant <= 2, "hnt_shell_analyse: bad variant");
     HNT_REQUIRE(params->order >= HNT_ORDER_CROSSING && params->order <= HNT_ORDER_LEARNED,
                 "hnt_shell_analyse: bad order %d", params->order);
+    HNT_REQUIRE(params->precision == HNT_PREC_FP64 || params->precision == HNT_PREC_FP32_KAHAN,
+                "hnt_shell_analyse: bad precision %d", params->precision);
     HNT_REQUIRE(params->order != HNT_ORDER_LEARNED ||
                     (params->bracket && params->bracket->params && params->bracket->norm),
                 "hnt_shell_analyse: HNT_ORDER_LEARNED needs the bracket network");
@@ -1091,7 +1127,13 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
 
     const int64_t ngroups = (np + 63) / 64;
     int per_cu = 0;
-    HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shell_analyse_kernel, 64, 0));
+    const bool kahan = params->precision == HNT_PREC_FP32_KAHAN;
+    if (kahan)
+        HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, shell_analyse_kernel<SumKahan>, 64, 0));
+    else
+        HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, shell_analyse_kernel<SumF64>, 64, 0));
     if (per_cu < 1) per_cu = 1;
     A.slot_doubles = (int64_t)(A.NS + 1) * PW;
     const size_t slot_bytes = (size_t)A.slot_doubles * 8;
@@ -1142,7 +1184,12 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
         stage_end(ctx, HNT_STAGE_SA_CROSS);
     }
     stage_begin(ctx, HNT_STAGE_SA_KERNEL);
-    hipLaunchKernelGGL(shell_analyse_kernel, dim3((unsigned)nblocks), dim3(64), 0, st, A);
+    if (kahan)
+        hipLaunchKernelGGL(shell_analyse_kernel<SumKahan>, dim3((unsigned)nblocks), dim3(64), 0,
+                           st, A);
+    else
+        hipLaunchKernelGGL(shell_analyse_kernel<SumF64>, dim3((unsigned)nblocks), dim3(64), 0, st,
+                           A);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);
     return finish_call(ctx, "hnt_shell_analyse");
 }

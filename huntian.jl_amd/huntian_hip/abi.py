@@ -111,6 +111,7 @@ MLP_NPARAM = 18 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 3 + 3
 HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
 ORDERS = {"crossing": HNT_ORDER_CROSSING, "plain": HNT_ORDER_PLAIN,
           "learned": HNT_ORDER_LEARNED}
+PRECISIONS = {"fp64": 0, "fp32_kahan": 1}
 
 
 class Mlp(C.Structure):
@@ -131,6 +132,7 @@ class Params(C.Structure):
         ("variant", C.c_int32),
         ("early_exit", C.c_int32),
         ("order", C.c_int32),
+        ("precision", C.c_int32),
         ("bracket", C.POINTER(Mlp)),
     ]
 

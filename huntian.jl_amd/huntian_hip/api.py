@@ -344,12 +344,14 @@ def _results_struct(r):
 def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct: dict, *,
                   alatt: float, ZZon: float, fcrit: float, nbuff: int = 0, variant="C",
                   early_exit: bool = True, mask=None, ZZon_pp=None, fcrit_pp=None,
-                  results=None, order: str = "crossing", bracket=None):
+                  results=None, order: str = "crossing", bracket=None,
+                  precision: str = "fp64"):
     """hnt_shell_analyse over one batch of peaks (device arrays or numpy).  Returns a dict of
     device tensors: scalars [np], vectors [np, 3], strain [np, 3, 3] ([p, L, K]).
 
     order: wavefront schedule ("crossing", "plain" or "learned" with `bracket`, a
-    huntian_hip.bracket.BracketNet); it changes the speed, never a result."""
+    huntian_hip.bracket.BracketNet); it changes the speed, never a result.
+    precision: per-shell sums in "fp64" (the reference CPU arithmetic) or "fp32_kahan"."""
     torch = _require_gpu()
     ctx = context()
     fs, keep_f = _fields_struct(fields)
@@ -374,7 +376,8 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
             raise ValueError("order='learned' needs a bracket network")
         net = bracket.struct()
     prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)),
-                     abi.ORDERS[order], C.pointer(net) if net is not None else None)
+                     abi.ORDERS[order], abi.PRECISIONS[precision],
+                     C.pointer(net) if net is not None else None)
     r = results if results is not None else alloc_results(npk, dev)
     rs = _results_struct(r)
     cs, keep_t = _ctab_struct(ct)

@@ -182,8 +182,14 @@ typedef struct {
     int32_t early_exit;        /* 1: stop gathering once no later shell can change a result */
     int32_t order;             /* wavefront schedule of the peaks, HNT_ORDER_* (0: crossing pass);
                                   it changes the speed, never a result */
+    int32_t precision;         /* per-shell sums: HNT_PREC_FP64 (0, the reference CPU arithmetic)
+                                  or HNT_PREC_FP32_KAHAN */
     const struct hnt_mlp* bracket;  /* network for HNT_ORDER_LEARNED, else unused */
 } hnt_params;
+
+/* Per-shell sums of the gather.  FP64: FP64 sums of the Float32 terms, as the reference CPU
+ * path.  FP32_KAHAN: FP32 sums with Kahan compensation; the radial logic stays FP64. */
+enum { HNT_PREC_FP64 = 0, HNT_PREC_FP32_KAHAN = 1 };
 
 /* Peak schedules of hnt_shell_analyse (early_exit = 1).
  *  CROSSING: a delta-only pass finds each peak's crossing shell, hence its exact gather

@@ -278,3 +278,34 @@ def test_multirf_mirror(oracle, bumps):
         np.testing.assert_allclose(d["RTHL"], ref["RTHL"].astype(np.float32), rtol=1e-6)
         np.testing.assert_allclose(d["strain_final"], np.transpose(ref["strain"], (1, 2, 0)),
                                    rtol=1e-5, atol=1e-6)
+
+
+def test_fp32_kahan_sums_within_tolerance(oracle, websky, cosmo64):
+    """Per-shell sums in FP32 with Kahan compensation against the FP64 oracle: same halo
+    set, RTHL and overdensity within 1e-5 relative, strain, e_v, p_v within 1e-4."""
+    f, radii, sm = cosmo64
+    n, alatt, nbuff, nhunt = 64, 2.0, 11, 10
+    ref_pk, tot = oracle.peak_find_multi(sm, [1.5] * len(radii), nbuff, _mask(n))
+    Rf = np.array([radii[q] for q in ref_pk["filt"]])
+    ir2 = np.array([api.ir2min_of(r, alatt) for r in Rf], np.int32)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    t = oracle.shell_tables(nhunt)
+    ref = oracle_run(oracle, f, ref_pk["i"], ref_pk["j"], ref_pk["k"], Rf, ir2, t, websky,
+                     alatt=alatt, ZZon=1.0, fcrit=fcrit, variant="C", nbuff=nbuff)
+    r = api.shell_analyse(f, ref_pk["i"], ref_pk["j"], ref_pk["k"], Rf, ir2,
+                          api.build_shell_tables(nhunt), websky, alatt=alatt, ZZon=1.0,
+                          fcrit=fcrit, variant="C", nbuff=nbuff, precision="fp32_kahan")
+    g = to_host(r)
+    assert np.array_equal(g["RTHL"] > 0, ref["RTHL"] > 0)
+    halo = ref["RTHL"] > 0
+    worst = {}
+    for k, tol in (("RTHL", 1e-5), ("Fbarx", 1e-5), ("strain", 1e-4), ("e_v", 1e-4),
+                   ("p_v", 1e-4)):
+        a = g[k][halo].reshape(halo.sum(), -1)
+        b = ref[k][halo].reshape(halo.sum(), -1)
+        vs = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-30)
+        floor = 1e-3 * vs if k in ("RTHL", "Fbarx", "strain") else 1e-3  # e_v, p_v near 0
+        err = np.abs(a - b) / np.maximum(np.abs(b), floor)
+        worst[k] = float(err.max())
+        assert worst[k] <= tol, (k, worst[k])
+    print(f"fp32_kahan vs FP64 oracle over {halo.sum()} halos: worst relative {worst}")
