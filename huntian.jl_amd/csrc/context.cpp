@@ -293,6 +293,7 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->recs, &ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp,
                       &ctx->keys_a, &ctx->keys_b, &ctx->vals_a, &ctx->vals_b, &ctx->prof,
                       &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small, &ctx->stops, &ctx->feats,
+                      &ctx->tiles, &ctx->tilemeta, &ctx->slots, &ctx->gtiles,
                       &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
                       &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
                       &ctx->tables.kw_idx, &ctx->ctab.table, &ctx->mlp.wt, &ctx->mlp.x0,

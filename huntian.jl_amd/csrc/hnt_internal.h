@@ -36,6 +36,28 @@ struct TablesDev {
     DevBuf kw_idx;              // int32 [3 * (nshells + 2)]: lo, hi, off of each mp
 };
 
+// One tile of a call, in device memory.  A wavefront's peaks all belong to one tile, so a
+// wave reads its tile's descriptor with scalar loads.
+struct TileDev {
+    const float4* recs;   // 32-byte cell records of this tile
+    const float* delta;   // the caller's delta (crossing pass)
+    const float* lapd;
+    int n, has_eta2, has_lapd;
+    int64_t s1, s2;
+    const int32_t* pi;
+    const int32_t* pj;
+    const int32_t* pk;
+    const int32_t* ir2min;
+    const double* Rf;
+    const double* ZZon;
+    const double* fcrit;
+    double ZZon_s, fcrit_s;
+    int8_t* mask;
+    hnt_results res;
+    int64_t npeaks;
+    int64_t base;  // global id of the tile's first peak
+};
+
 // scratch of the bracket MLP (bf16 weights and activations, gradients)
 struct MlpScratch {
     DevBuf wt, x0, z, h, y, dy, dyf, dz, gw, part, gradv;
@@ -61,6 +83,9 @@ struct hnt_ctx {
     hnt::TablesDev tables;
     hnt::CTabDev ctab;
     hnt::MlpScratch mlp;
+    hnt::DevBuf tiles, tilemeta, slots, gtiles;  // multi-tile shell analysis
+    std::vector<hnt::TileDev> tile_host;
+    std::vector<int64_t> meta_host;
     // stage timers (hnt_ctx_set_profiling)
     bool profiling = false;
     hipEvent_t ev[2 * HNT_NSTAGES] = {};

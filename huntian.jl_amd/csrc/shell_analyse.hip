@@ -44,20 +44,10 @@ constexpr int PW = 24;  // doubles per shell in a profile slot
 constexpr int K_F = 12, K_N = 13, K_S = 14, K_S2 = 17, K_LAP = 20, K_LAPN = 21;
 
 struct SAArgs {
-    const float4* recs;
-    const float* lapd;
-    int n1, n2, n3;
-    int64_t s1, s2;
-    const int32_t* pi;
-    const int32_t* pj;
-    const int32_t* pk;
-    const int32_t* ir2min;
-    const double* Rf;
-    const double* ZZon;
-    const double* fcrit;
-    double ZZon_s, fcrit_s;
-    const int32_t* order;
-    int64_t npeaks;
+    const TileDev* tiles;
+    const int32_t* slot;   // [ngroups * 64]: peak index within its tile, -1 = empty lane
+    const int32_t* gtile;  // [ngroups]: tile of each group
+    int64_t ngroups;
     const int4* cells;
     const int32_t* shell_start;
     const int32_t* shell_count;
@@ -69,13 +59,11 @@ struct SAArgs {
     CTabView ct;
     const double* akk;
     double alatt, wRnor, aRnor;
-    int nbuff, variant, early, has_eta2, has_lapd;
+    int nbuff, variant, early;
     double* prof;
     int64_t slot_doubles;
     unsigned long long* qctr;
     int64_t qbeg[8], qend[8];
-    hnt_results res;
-    int8_t* mask;
 };
 
 // ------------------------------------------------------------------ repack
@@ -124,17 +112,51 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
     return x;
 }
 
-__global__ void key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
-                           const int32_t* __restrict__ pk, const int32_t* __restrict__ ir2min,
-                           int64_t np, uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+// Sort key of a tile's peaks: (tile, primary, Morton code of the peak cell), value = global
+// peak id base + q.  primary is ir2min for the first order and the stop shell for the
+// second; it is clamped to 11 bits (it only orders).
+__global__ void tile_key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+                                const int32_t* __restrict__ pk, const int32_t* __restrict__ prim,
+                                int64_t np, uint64_t tile, int64_t base,
+                                uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     uint64_t mort = spread3((uint32_t)(pi[q] - 1) & 0x1fff) |
                     (spread3((uint32_t)(pj[q] - 1) & 0x1fff) << 1) |
                     (spread3((uint32_t)(pk[q] - 1) & 0x1fff) << 2);
-    uint64_t g = (uint64_t)min(max(ir2min[q], 0), (1 << 24) - 1);
-    keys[q] = (g << 39) | mort;
-    vals[q] = (int32_t)q;
+    uint64_t g = (uint64_t)min(max(prim[q], 0), 2047);
+    keys[base + q] = (tile << 50) | (g << 39) | mort;
+    vals[base + q] = (int32_t)(base + q);
+}
+
+// Sorted position q -> lane slot: tile t's peaks occupy sorted positions [base_t, base_t +
+// n_t) and slots [pad_t, pad_t + n_t); slots up to the next multiple of 64 stay -1.
+__global__ void slot_kernel(const uint64_t* __restrict__ keys, const int32_t* __restrict__ vals,
+                            int64_t ntot, const int64_t* __restrict__ tile_base,
+                            const int64_t* __restrict__ tile_pad, int32_t* __restrict__ slot) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ntot) return;
+    const int t = (int)(keys[q] >> 50);
+    slot[tile_pad[t] + (q - tile_base[t])] = (int32_t)(vals[q] - tile_base[t]);
+}
+
+// Tile of every group of 64 slots.
+__global__ void group_tile_kernel(const int64_t* __restrict__ tile_pad, int ntiles,
+                                  int64_t ngroups, int32_t* __restrict__ gtile) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    int lo = 0, hi = ntiles - 1;  // last tile with tile_pad <= 64 g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_pad[mid] <= g * 64) lo = mid;
+        else hi = mid - 1;
+    }
+    gtile[g] = lo;
+}
+
+__global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
 }
 
 // Flags peaks outside the grid or with ZZon <= -1 (bit 0 / bit 1 of *bad).
@@ -364,10 +386,10 @@ __device__ __forceinline__ void acc_cell(Acc<S>& A, const float4 v0, const float
 
 // Gather one shell for one lane.  CHECK = the wave holds a peak whose ball leaves the grid.
 template <bool CHECK, class S>
-__device__ __forceinline__ void gather_shell(const SAArgs& a, int st, int nc, int ci, int cj,
+__device__ __forceinline__ void gather_shell(const SAArgs& a, const TileDev& T, int st, int nc, int ci, int cj,
                                              int ck, int64_t c0, Acc<S>& A, bool& oobA) {
-    const bool eta2 = a.has_eta2;
-    const bool lapd = a.has_lapd;
+    const bool eta2 = T.has_eta2;
+    const bool lapd = T.has_lapd;
     const bool varA = a.variant == HNT_VARIANT_A;
     if (!CHECK) {
         int c = 0;
@@ -378,10 +400,10 @@ __device__ __forceinline__ void gather_shell(const SAArgs& a, int st, int nc, in
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 o[u] = a.cells[st + c + u];
-                const int64_t off = c0 + o[u].x + o[u].y * a.s1 + o[u].z * a.s2;
-                v0[u] = a.recs[2 * off];
-                v1[u] = a.recs[2 * off + 1];
-                lv[u] = lapd ? a.lapd[off] : 0.0f;
+                const int64_t off = c0 + o[u].x + o[u].y * T.s1 + o[u].z * T.s2;
+                v0[u] = T.recs[2 * off];
+                v1[u] = T.recs[2 * off + 1];
+                lv[u] = lapd ? T.lapd[off] : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -394,11 +416,11 @@ __device__ __forceinline__ void gather_shell(const SAArgs& a, int st, int nc, in
         }
         for (; c < nc; ++c) {
             const int4 o = a.cells[st + c];
-            const int64_t off = c0 + o.x + o.y * a.s1 + o.z * a.s2;
-            const float4 v0 = a.recs[2 * off], v1 = a.recs[2 * off + 1];
+            const int64_t off = c0 + o.x + o.y * T.s1 + o.z * T.s2;
+            const float4 v0 = T.recs[2 * off], v1 = T.recs[2 * off + 1];
             acc_cell(A, v0, v1, (float)o.x, (float)o.y, (float)o.z, eta2);
             if (lapd) {
-                A.lap += (double)a.lapd[off];
+                A.lap += (double)T.lapd[off];
                 A.lapn += 1;
             }
         }
@@ -407,24 +429,24 @@ __device__ __forceinline__ void gather_shell(const SAArgs& a, int st, int nc, in
         for (int c = 0; c < nc; ++c) {
             const int4 o = a.cells[st + c];
             const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
-            if (i1 < 1 || i1 > a.n1 || i2 < 1 || i2 > a.n2 || i3 < 1 || i3 > a.n3) {
+            if (i1 < 1 || i1 > T.n || i2 < 1 || i2 > T.n || i3 < 1 || i3 > T.n) {
                 oobA = true;
                 continue;
             }
-            const int64_t off = c0 + o.x + o.y * a.s1 + o.z * a.s2;
-            const float4 v0 = a.recs[2 * off], v1 = a.recs[2 * off + 1];
+            const int64_t off = c0 + o.x + o.y * T.s1 + o.z * T.s2;
+            const float4 v0 = T.recs[2 * off], v1 = T.recs[2 * off + 1];
             acc_cell(A, v0, v1, (float)o.x, (float)o.y, (float)o.z, eta2);
             A.nin += 1;
             if (lapd && (!varA || !oobA)) {
-                A.lap += (double)a.lapd[off];
+                A.lap += (double)T.lapd[off];
                 A.lapn += 1;
             }
         }
     }
 }
 
-__device__ void write_no_collapse(const SAArgs& a, int p) {
-    const hnt_results& r = a.res;
+__device__ void write_no_collapse(const TileDev& T, int p) {
+    const hnt_results& r = T.res;
     r.RTHL[p] = -1.0;
     r.Fbarx[p] = 0.0;
     r.e_v[p] = 0.0;
@@ -446,7 +468,7 @@ __device__ void write_no_collapse(const SAArgs& a, int p) {
 
 // The radial logic of one peak from its profile slot P (RadialShell.jl:449-865 for A,
 // ShellAnalysisGPU.jl:231-535 for B; C = B + zvir_half of A).
-__device__ void post_process(const SAArgs& a, int p, const double* __restrict__ P, int ci, int cj,
+__device__ void post_process(const SAArgs& a, const TileDev& T, int p, const double* __restrict__ P, int ci, int cj,
                              int ck, double ZZon, double fcrit, int ir2min, int mrf,
                              int mlow_rf, int mupp_rf) {
     const double* __restrict__ rad = a.rad;
@@ -486,7 +508,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
             m0 = mp;
         }
         if (!found) {
-            write_no_collapse(a, p);
+            write_no_collapse(T, p);
             return;
         }
         mupp = first_ge(rad, m0 + 1, m, rad[m0] + 2.0) - 1;
@@ -505,7 +527,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
     double Frhoh = Fr;
     KS last = k0, prev = k0;  // Ebar_last/gradpk_last and strain_mat/gradpk_prev
     if (m0 == 1) {
-        write_no_collapse(a, p);
+        write_no_collapse(T, p);
         return;
     }
     bool collapsed = false;
@@ -549,7 +571,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
         last = kk;
     }
     if (!collapsed) {
-        write_no_collapse(a, p);
+        write_no_collapse(T, p);
         return;
     }
 
@@ -563,7 +585,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
         RTHL = rad[m0 - 1];
     }
     if (RTHL <= 0.0) {
-        write_no_collapse(a, p);
+        write_no_collapse(T, p);
         return;
     }
     const double RTHL3 = cube(RTHL);
@@ -635,20 +657,20 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
     // last shell within RTHL: d2F and the mask
     const int sR = first_gt(rad, 1, NS, RTHL) - 1;
     double d2F = 0.0;
-    if (a.has_lapd) {
+    if (T.has_lapd) {
         const double* Y = P + (int64_t)sR * PW;
         if (Y[K_LAPN] > 0.0) d2F = Y[K_LAP] / Y[K_LAPN];
     }
-    if (a.mask) {
+    if (T.mask) {
         const int nb = a.nbuff;
         for (int s = 1; s <= sR; ++s) {
             const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
             for (int c = 0; c < nc; ++c) {
                 const int4 o = a.cells[st + c];
                 const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
-                if (i1 > nb && i1 <= a.n1 - nb && i2 > nb && i2 <= a.n2 - nb && i3 > nb &&
-                    i3 <= a.n3 - nb && i1 >= 1 && i2 >= 1 && i3 >= 1)
-                    a.mask[(int64_t)(i1 - 1) + (i2 - 1) * a.s1 + (i3 - 1) * a.s2] = 1;
+                if (i1 > nb && i1 <= T.n - nb && i2 > nb && i2 <= T.n - nb && i3 > nb &&
+                    i3 <= T.n - nb && i1 >= 1 && i2 >= 1 && i3 >= 1)
+                    T.mask[(int64_t)(i1 - 1) + (i2 - 1) * T.s1 + (i3 - 1) * T.s2] = 1;
             }
         }
     }
@@ -690,7 +712,7 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
         }
     }
 
-    const hnt_results& R = a.res;
+    const hnt_results& R = T.res;
     R.RTHL[p] = RTHL;
     R.Fbarx[p] = Fbarx;
     R.e_v[p] = e_f;
@@ -714,19 +736,19 @@ __device__ void post_process(const SAArgs& a, int p, const double* __restrict__ 
 template <class S>
 __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__ P) {
     const int lane = threadIdx.x;
-    const int64_t idx = grp * 64 + lane;
-    const bool valid = idx < a.npeaks;
-    const int p = valid ? a.order[idx] : 0;
+    const TileDev& T = a.tiles[a.gtile[grp]];
+    const int p = a.slot[grp * 64 + lane];
+    const bool valid = p >= 0;
     int ci = 1, cj = 1, ck = 1, ir2min = 0;
     double ZZon = 0.0, fcrit = 0.0, Rf = 0.0;
     if (valid) {
-        ci = a.pi[p];
-        cj = a.pj[p];
-        ck = a.pk[p];
-        ir2min = a.ir2min[p];
-        Rf = a.Rf[p];
-        ZZon = a.ZZon ? a.ZZon[p] : a.ZZon_s;
-        fcrit = a.fcrit ? a.fcrit[p] : a.fcrit_s;
+        ci = T.pi[p];
+        cj = T.pj[p];
+        ck = T.pk[p];
+        ir2min = T.ir2min[p];
+        Rf = T.Rf[p];
+        ZZon = T.ZZon ? T.ZZon[p] : T.ZZon_s;
+        fcrit = T.fcrit ? T.fcrit[p] : T.fcrit_s;
     }
     const double* __restrict__ rad = a.rad;
     const int NS = a.NS;
@@ -736,30 +758,30 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     if (valid) rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
 
     // ---- gather ----
-    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
+    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
     const int R = a.rmax;
-    const bool inside = ci - R >= 1 && ci + R <= a.n1 && cj - R >= 1 && cj + R <= a.n2 &&
-                        ck - R >= 1 && ck + R <= a.n3;
+    const bool inside = ci - R >= 1 && ci + R <= T.n && cj - R >= 1 && cj + R <= T.n &&
+                        ck - R >= 1 && ck + R <= T.n;
     const bool wave_inside = __ballot(!valid || inside) == ~0ull;
 
     double Fprev = 0.0, dFbarp = 0.0, cumN = 0.0, cS0 = 0.0, cS1 = 0.0, cS2 = 0.0, cT0 = 0.0,
            cT1 = 0.0, cT2 = 0.0, lap = 0.0;
     int64_t lapn = 0;
     if (valid) {
-        const float4 v0 = a.recs[2 * c0], v1 = a.recs[2 * c0 + 1];
+        const float4 v0 = T.recs[2 * c0], v1 = T.recs[2 * c0 + 1];
         Fprev = (double)v0.x;
         dFbarp = Fprev;
         cumN = 1.0;
         cS0 = 0.0 + (double)v0.z;
         cS1 = 0.0 + (double)v0.w;
         cS2 = 0.0 + (double)v1.x;
-        if (a.has_eta2) {
+        if (T.has_eta2) {
             cT0 = 0.0 + (double)v1.y;
             cT1 = 0.0 + (double)v1.z;
             cT2 = 0.0 + (double)v1.w;
         }
-        if (a.has_lapd && a.variant != HNT_VARIANT_A) {
-            lap = 0.0 + (double)a.lapd[c0];
+        if (T.has_lapd && a.variant != HNT_VARIANT_A) {
+            lap = 0.0 + (double)T.lapd[c0];
             lapn = 1;
         }
         double* X = P + PW;
@@ -784,8 +806,8 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
             Acc<S> A;
             A.zero();
             A.lap = lap; A.lapn = lapn; A.nin = 0;
-            if (wave_inside) gather_shell<false, S>(a, st, nc, ci, cj, ck, c0, A, oobA);
-            else gather_shell<true, S>(a, st, nc, ci, cj, ck, c0, A, oobA);
+            if (wave_inside) gather_shell<false, S>(a, T, st, nc, ci, cj, ck, c0, A, oobA);
+            else gather_shell<true, S>(a, T, st, nc, ci, cj, ck, c0, A, oobA);
             lap = A.lap;
             lapn = A.lapn;
             const int64_t nin = A.nin;
@@ -833,9 +855,9 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
         }
     }
     if (!valid) return;
-    if (a.res.nshells_used) a.res.nshells_used[p] = stop;
-    if (a.res.crossing_shell) a.res.crossing_shell[p] = scross;
-    post_process(a, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
+    if (T.res.nshells_used) T.res.nshells_used[p] = stop;
+    if (T.res.crossing_shell) T.res.crossing_shell[p] = scross;
+    post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
 }
 
 // ------------------------------------------------------------------ crossing pass
@@ -846,30 +868,30 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
 // peaks ordered by that stop shell and a wavefront's 64 lanes finish together.
 // Workgroup b takes group (b % 8) * G8 + b / 8: each XCD walks its own contiguous eighth of
 // the (ir2min, Morton) order.
-__global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, const float* __restrict__ delta,
-                                                      int64_t ngroups, int64_t G8,
+__global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, int64_t G8,
                                                       int32_t* __restrict__ stop_out) {
     const int64_t g = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
-    if (g >= ngroups) return;
-    const int64_t idx = g * 64 + threadIdx.x;
-    const bool valid = idx < a.npeaks;
-    const int p = valid ? a.order[idx] : 0;
+    if (g >= a.ngroups) return;
+    const TileDev& T = a.tiles[a.gtile[g]];
+    const float* __restrict__ delta = T.delta;
+    const int p = a.slot[g * 64 + threadIdx.x];
+    const bool valid = p >= 0;
     int ci = 1, cj = 1, ck = 1, ir2min = 0;
     double fcrit = 0.0, Rf = 0.0;
     if (valid) {
-        ci = a.pi[p];
-        cj = a.pj[p];
-        ck = a.pk[p];
-        ir2min = a.ir2min[p];
-        Rf = a.Rf[p];
-        fcrit = a.fcrit ? a.fcrit[p] : a.fcrit_s;
+        ci = T.pi[p];
+        cj = T.pj[p];
+        ck = T.pk[p];
+        ir2min = T.ir2min[p];
+        Rf = T.Rf[p];
+        fcrit = T.fcrit ? T.fcrit[p] : T.fcrit_s;
     }
     const double* __restrict__ rad = a.rad;
     const int m = a.NS - 1;
-    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
+    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
     const int R = a.rmax;
-    const bool inside = ci - R >= 1 && ci + R <= a.n1 && cj - R >= 1 && cj + R <= a.n2 &&
-                        ck - R >= 1 && ck + R <= a.n3;
+    const bool inside = ci - R >= 1 && ci + R <= T.n && cj - R >= 1 && cj + R <= T.n &&
+                        ck - R >= 1 && ck + R <= T.n;
     const bool wave_inside = __ballot(!valid || inside) == ~0ull;
 
     double Fprev = valid ? (double)delta[c0] : 0.0;
@@ -889,23 +911,23 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, const float* __r
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int4 o = a.cells[st + c + u];
-                        v[u] = delta[c0 + o.x + o.y * a.s1 + o.z * a.s2];
+                        v[u] = delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) F += (double)v[u];
                 }
                 for (; c < nc; ++c) {
                     const int4 o = a.cells[st + c];
-                    F += (double)delta[c0 + o.x + o.y * a.s1 + o.z * a.s2];
+                    F += (double)delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
                 }
                 nin = nc;
             } else {
                 for (int c = 0; c < nc; ++c) {
                     const int4 o = a.cells[st + c];
                     const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
-                    if (i1 < 1 || i1 > a.n1 || i2 < 1 || i2 > a.n2 || i3 < 1 || i3 > a.n3)
+                    if (i1 < 1 || i1 > T.n || i2 < 1 || i2 > T.n || i3 < 1 || i3 > T.n)
                         continue;
-                    F += (double)delta[c0 + o.x + o.y * a.s1 + o.z * a.s2];
+                    F += (double)delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
                     nin += 1;
                 }
             }
@@ -935,7 +957,7 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, const float* __r
             if (t <= m + 1) stop = t - 1;
         }
     }
-    stop_out[p] = stop;
+    stop_out[T.base + p] = stop;
 }
 
 // Predicted gather extent (network output 1 = log(1 + shells)), clamped to [1, m].
@@ -949,20 +971,7 @@ __global__ void pred_stop_kernel(const float* __restrict__ pred, int64_t np, int
     stop[q] = (int32_t)(s + 0.5f);
 }
 
-// Sort key of the main pass: stop shell, then Morton code of the peak cell.
-__global__ void stop_key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
-                                const int32_t* __restrict__ pk, const int32_t* __restrict__ stop,
-                                int64_t np, uint64_t* __restrict__ keys,
-                                int32_t* __restrict__ vals) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
-    uint64_t mort = spread3((uint32_t)(pi[q] - 1) & 0x1fff) |
-                    (spread3((uint32_t)(pj[q] - 1) & 0x1fff) << 1) |
-                    (spread3((uint32_t)(pk[q] - 1) & 0x1fff) << 2);
-    uint64_t g = (uint64_t)min(max(stop[q], 0), (1 << 24) - 1);
-    keys[q] = (g << 39) | mort;
-    vals[q] = (int32_t)q;
-}
+
 
 template <class S>
 __global__ __launch_bounds__(64) void shell_analyse_kernel(SAArgs a) {
@@ -986,22 +995,52 @@ __global__ __launch_bounds__(64) void shell_analyse_kernel(SAArgs a) {
 
 using namespace hnt;
 
-extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
-                                        const hnt_batch* batch, const hnt_shell_tables* tables,
-                                        const hnt_ctab* ctab, const hnt_params* params,
-                                        hnt_results* results, int8_t* mask) {
-    HNT_TRY(begin_call(ctx));
-    HNT_REQUIRE(fields && batch && tables && ctab && params && results,
+namespace {
+
+hnt_status check_tile(const hnt_fields* f, const hnt_batch* b, const hnt_results* r,
+                      const hnt_shell_tables* tables, int t) {
+    HNT_REQUIRE(f && b && r, "hnt_shell_analyse: NULL argument (tile %d)", t);
+    const int n = f->n1;
+    HNT_REQUIRE(f->n1 == f->n2 && f->n2 == f->n3 && n >= 1 && n <= 8192,
+                "hnt_shell_analyse: the transposed read needs a cubic grid, got %d x %d x %d "
+                "(tile %d)", f->n1, f->n2, f->n3, t);
+    HNT_REQUIRE(f->delta && f->etax && f->etay && f->etaz,
+                "hnt_shell_analyse: delta and eta fields are required (tile %d)", t);
+    HNT_REQUIRE((f->eta2x == nullptr) == (f->eta2y == nullptr) &&
+                    (f->eta2y == nullptr) == (f->eta2z == nullptr),
+                "hnt_shell_analyse: eta2x/y/z must be all set or all NULL (tile %d)", t);
+    HNT_REQUIRE(b->npeaks >= 0 && b->npeaks < (1LL << 31), "hnt_shell_analyse: npeaks");
+    if (b->npeaks > 0) {
+        HNT_REQUIRE(b->pi && b->pj && b->pk && b->Rf && b->ir2min,
+                    "hnt_shell_analyse: NULL peak array (tile %d)", t);
+        HNT_REQUIRE(r->RTHL && r->Fbarx && r->e_v && r->p_v && r->strain && r->eigs && r->Srb &&
+                        r->Sbar && r->Sbar2 && r->gradpk && r->gradpkf && r->gradpkrf &&
+                        r->d2F && r->zvir_half,
+                    "hnt_shell_analyse: NULL result array (tile %d)", t);
+        HNT_REQUIRE(b->ZZon != nullptr || b->ZZon_scalar > -1.0,
+                    "hnt_shell_analyse: ZZon must be > -1");
+        HNT_REQUIRE(tables->rmax < n,
+                    "hnt_shell_analyse: shell radius %d not below the grid size %d (tile %d)",
+                    tables->rmax, n, t);
+    }
+    return HNT_OK;
+}
+
+void sort_pairs(hnt_ctx* ctx, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipError_t* e) {
+    *e = hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp.p, tmp_bytes, (uint64_t*)ctx->keys_a.p,
+                                            (uint64_t*)ctx->keys_b.p, (int32_t*)ctx->vals_a.p,
+                                            (int32_t*)ctx->vals_b.p, (int)ntot, 0, 64, st);
+}
+
+// All tiles of a call in one persistent launch: peaks of every tile share the eight queues;
+// a wavefront's 64 lanes belong to one tile.
+hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
+                         const hnt_batch* batches, const hnt_shell_tables* tables,
+                         const hnt_ctab* ctab, const hnt_params* params, hnt_results* results,
+                         int8_t* const* masks) {
+    HNT_REQUIRE(ntiles >= 1 && ntiles <= 8192, "hnt_shell_analyse: %d tiles", ntiles);
+    HNT_REQUIRE(fields && batches && tables && ctab && params && results,
                 "hnt_shell_analyse: NULL argument");
-    const int n = fields->n1;
-    HNT_REQUIRE(fields->n1 == fields->n2 && fields->n2 == fields->n3 && n >= 1 && n <= 8192,
-                "hnt_shell_analyse: the transposed read needs a cubic grid, got %d x %d x %d",
-                fields->n1, fields->n2, fields->n3);
-    HNT_REQUIRE(fields->delta && fields->etax && fields->etay && fields->etaz,
-                "hnt_shell_analyse: delta and eta fields are required");
-    HNT_REQUIRE((fields->eta2x == nullptr) == (fields->eta2y == nullptr) &&
-                    (fields->eta2y == nullptr) == (fields->eta2z == nullptr),
-                "hnt_shell_analyse: eta2x/y/z must be all set or all NULL");
     HNT_REQUIRE(params->variant >= 0 && params->variant <= 2, "hnt_shell_analyse: bad variant");
     HNT_REQUIRE(params->order >= HNT_ORDER_CROSSING && params->order <= HNT_ORDER_LEARNED,
                 "hnt_shell_analyse: bad order %d", params->order);
@@ -1017,90 +1056,140 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
         return HNT_ERR_UNSUPPORTED;
     }
     HNT_REQUIRE(params->alatt > 0.0, "hnt_shell_analyse: alatt must be > 0");
-    HNT_REQUIRE(batch->npeaks >= 0 && batch->npeaks < (1LL << 31), "hnt_shell_analyse: npeaks");
-    const int64_t np = batch->npeaks;
-    if (np > 0) {
-        HNT_REQUIRE(batch->pi && batch->pj && batch->pk && batch->Rf && batch->ir2min,
-                    "hnt_shell_analyse: NULL peak array");
-        HNT_REQUIRE(results->RTHL && results->Fbarx && results->e_v && results->p_v &&
-                        results->strain && results->eigs && results->Srb && results->Sbar &&
-                        results->Sbar2 && results->gradpk && results->gradpkf &&
-                        results->gradpkrf && results->d2F && results->zvir_half,
-                    "hnt_shell_analyse: NULL result array");
-        HNT_REQUIRE(batch->ZZon != nullptr || batch->ZZon_scalar > -1.0,
-                    "hnt_shell_analyse: ZZon must be > -1");
+    int64_t ntot = 0, ngroups = 0, ncell = 0;
+    std::vector<int64_t> meta(2 * (size_t)ntiles);  // base, then padded slot offset
+    for (int t = 0; t < ntiles; ++t) {
+        HNT_TRY(check_tile(&fields[t], &batches[t], &results[t], tables, t));
+        meta[t] = ntot;
+        meta[ntiles + t] = ngroups * 64;
+        ntot += batches[t].npeaks;
+        ngroups += (batches[t].npeaks + 63) / 64;
+        ncell += (int64_t)fields[t].n1 * fields[t].n1 * fields[t].n1;
     }
+    HNT_REQUIRE(ntot < (1LL << 31), "hnt_shell_analyse: %lld peaks in one call",
+                (long long)ntot);
     HNT_TRY(upload_tables(ctx, tables));
     HNT_TRY(upload_ctab(ctx, ctab));
     HNT_TRY(ensure_akk(ctx));
-    if (np == 0) return HNT_OK;
-    HNT_REQUIRE(tables->rmax < n, "hnt_shell_analyse: shell radius %d not below the grid size %d",
-                tables->rmax, n);
+    if (ntot == 0) return HNT_OK;
     hipStream_t st = ctx->stream;
 
     // ---- validate the peak cells and ZZon on the device before any gather ----
     stage_begin(ctx, HNT_STAGE_SA_PREP);
-    {
-        HNT_TRY(reserve(ctx, ctx->small, 64));
-        hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ctx->small.p, 1,
-                           0ull);
-        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                           batch->pi, batch->pj, batch->pk, batch->ZZon, np, n,
+    HNT_TRY(reserve(ctx, ctx->small, 64));
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ctx->small.p, 1,
+                       0ull);
+    for (int t = 0; t < ntiles; ++t) {
+        const hnt_batch& b = batches[t];
+        if (b.npeaks == 0) continue;
+        hipLaunchKernelGGL(validate_kernel, dim3((unsigned)((b.npeaks + 255) / 256)), dim3(256),
+                           0, st, b.pi, b.pj, b.pk, b.ZZon, b.npeaks, fields[t].n1,
                            (unsigned int*)ctx->small.p);
+    }
+    {
         unsigned long long bad = 0;
         HNT_HIP(hipMemcpyAsync(&bad, ctx->small.p, sizeof(bad), hipMemcpyDeviceToHost, st));
         HNT_HIP(hipStreamSynchronize(st));
-        HNT_REQUIRE(!(bad & 1ull), "hnt_shell_analyse: a peak lies outside the %d^3 grid", n);
+        HNT_REQUIRE(!(bad & 1ull), "hnt_shell_analyse: a peak lies outside its grid");
         HNT_REQUIRE(!(bad & 2ull), "hnt_shell_analyse: per-peak ZZon must be > -1");
     }
 
-    // ---- repack the fields into 32-byte cell records ----
-    const int64_t ncell = (int64_t)n * n * n;
+    // ---- repack every tile's fields into 32-byte cell records ----
     HNT_TRY(reserve(ctx, ctx->recs, (size_t)ncell * 32));
+    std::vector<TileDev>& TD = ctx->tile_host;
+    TD.assign((size_t)ntiles, TileDev{});
     {
-        dim3 g((n + 63) / 64, (n + 63) / 64, n);
-        hipLaunchKernelGGL(repack_kernel, g, dim3(256), 0, st, fields->delta, fields->etax,
-                           fields->etay, fields->etaz, fields->eta2x, fields->eta2y,
-                           fields->eta2z, n, (float4*)ctx->recs.p);
+        int64_t off = 0;
+        for (int t = 0; t < ntiles; ++t) {
+            const hnt_fields& f = fields[t];
+            const int n = f.n1;
+            float4* recs = (float4*)ctx->recs.p + 2 * off;
+            if (batches[t].npeaks > 0) {
+                dim3 g((n + 63) / 64, (n + 63) / 64, n);
+                hipLaunchKernelGGL(repack_kernel, g, dim3(256), 0, st, f.delta, f.etax, f.etay,
+                                   f.etaz, f.eta2x, f.eta2y, f.eta2z, n, recs);
+            }
+            TileDev& T = TD[t];
+            T.recs = recs;
+            T.delta = f.delta;
+            T.lapd = f.lapd;
+            T.n = n;
+            T.has_eta2 = f.eta2x != nullptr;
+            T.has_lapd = f.lapd != nullptr;
+            T.s1 = n;
+            T.s2 = (int64_t)n * n;
+            const hnt_batch& b = batches[t];
+            T.pi = b.pi;
+            T.pj = b.pj;
+            T.pk = b.pk;
+            T.ir2min = b.ir2min;
+            T.Rf = b.Rf;
+            T.ZZon = b.ZZon;
+            T.fcrit = b.fcrit;
+            T.ZZon_s = b.ZZon_scalar;
+            T.fcrit_s = b.fcrit_scalar;
+            T.mask = masks ? masks[t] : nullptr;
+            T.res = results[t];
+            T.npeaks = b.npeaks;
+            T.base = meta[t];
+            off += (int64_t)n * n * n;
+        }
     }
+    HNT_TRY(reserve(ctx, ctx->tiles, TD.size() * sizeof(TileDev)));
+    HNT_TRY(reserve(ctx, ctx->tilemeta, meta.size() * sizeof(int64_t)));
+    ctx->meta_host = meta;
+    HNT_HIP(hipMemcpyAsync(ctx->tiles.p, TD.data(), TD.size() * sizeof(TileDev),
+                           hipMemcpyHostToDevice, st));
+    HNT_HIP(hipMemcpyAsync(ctx->tilemeta.p, ctx->meta_host.data(), meta.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, st));
+    const int64_t* tile_base = (const int64_t*)ctx->tilemeta.p;
+    const int64_t* tile_pad = tile_base + ntiles;
 
-    // ---- peak order: (ir2min, Morton) ----
-    HNT_TRY(reserve(ctx, ctx->keys_a, (size_t)np * 8));
-    HNT_TRY(reserve(ctx, ctx->keys_b, (size_t)np * 8));
-    HNT_TRY(reserve(ctx, ctx->vals_a, (size_t)np * 4));
-    HNT_TRY(reserve(ctx, ctx->vals_b, (size_t)np * 4));
-    hipLaunchKernelGGL(key_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                       batch->pi, batch->pj, batch->pk, batch->ir2min, np,
-                       (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);
+    // ---- first order: (tile, ir2min, Morton) ----
+    HNT_TRY(reserve(ctx, ctx->keys_a, (size_t)ntot * 8));
+    HNT_TRY(reserve(ctx, ctx->keys_b, (size_t)ntot * 8));
+    HNT_TRY(reserve(ctx, ctx->vals_a, (size_t)ntot * 4));
+    HNT_TRY(reserve(ctx, ctx->vals_b, (size_t)ntot * 4));
+    HNT_TRY(reserve(ctx, ctx->slots, (size_t)ngroups * 64 * 4));
+    HNT_TRY(reserve(ctx, ctx->gtiles, (size_t)ngroups * 4));
+    int32_t* slot = (int32_t*)ctx->slots.p;
     size_t tmp_bytes = 0;
     HNT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint64_t*)ctx->keys_a.p,
                                                (uint64_t*)ctx->keys_b.p, (int32_t*)ctx->vals_a.p,
-                                               (int32_t*)ctx->vals_b.p, (int)np, 0, 64, st));
+                                               (int32_t*)ctx->vals_b.p, (int)ntot, 0, 64, st));
     HNT_TRY(reserve(ctx, ctx->cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
-    HNT_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp.p, tmp_bytes,
-                                               (uint64_t*)ctx->keys_a.p, (uint64_t*)ctx->keys_b.p,
-                                               (int32_t*)ctx->vals_a.p, (int32_t*)ctx->vals_b.p,
-                                               (int)np, 0, 64, st));
+    auto order_by = [&](const int32_t* const* prim) -> hnt_status {
+        for (int t = 0; t < ntiles; ++t) {
+            const hnt_batch& b = batches[t];
+            if (b.npeaks == 0) continue;
+            hipLaunchKernelGGL(tile_key_kernel, dim3((unsigned)((b.npeaks + 255) / 256)),
+                               dim3(256), 0, st, b.pi, b.pj, b.pk, prim[t], b.npeaks,
+                               (uint64_t)t, meta[t], (uint64_t*)ctx->keys_a.p,
+                               (int32_t*)ctx->vals_a.p);
+        }
+        hipError_t e;
+        sort_pairs(ctx, tmp_bytes, ntot, st, &e);
+        HNT_HIP(e);
+        hipLaunchKernelGGL(fill_i32, dim3((unsigned)((ngroups * 64 + 255) / 256)), dim3(256), 0,
+                           st, slot, ngroups * 64, -1);
+        hipLaunchKernelGGL(slot_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st,
+                           (const uint64_t*)ctx->keys_b.p, (const int32_t*)ctx->vals_b.p, ntot,
+                           tile_base, tile_pad, slot);
+        return HNT_OK;
+    };
+    std::vector<const int32_t*> prim((size_t)ntiles);
+    for (int t = 0; t < ntiles; ++t) prim[t] = batches[t].ir2min;
+    HNT_TRY(order_by(prim.data()));
+    hipLaunchKernelGGL(group_tile_kernel, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0,
+                       st, tile_pad, ntiles, ngroups, (int32_t*)ctx->gtiles.p);
 
     // ---- persistent grid and profile scratch ----
     SAArgs A;
     memset(&A, 0, sizeof(A));
-    A.recs = (const float4*)ctx->recs.p;
-    A.lapd = fields->lapd;
-    A.n1 = A.n2 = A.n3 = n;
-    A.s1 = n;
-    A.s2 = (int64_t)n * n;
-    A.pi = batch->pi;
-    A.pj = batch->pj;
-    A.pk = batch->pk;
-    A.ir2min = batch->ir2min;
-    A.Rf = batch->Rf;
-    A.ZZon = batch->ZZon;
-    A.fcrit = batch->fcrit;
-    A.ZZon_s = batch->ZZon_scalar;
-    A.fcrit_s = batch->fcrit_scalar;
-    A.order = (const int32_t*)ctx->vals_b.p;
-    A.npeaks = np;
+    A.tiles = (const TileDev*)ctx->tiles.p;
+    A.slot = slot;
+    A.gtile = (const int32_t*)ctx->gtiles.p;
+    A.ngroups = ngroups;
     A.cells = (const int4*)ctx->tables.cells.p;
     A.shell_start = (const int32_t*)ctx->tables.start.p;
     A.shell_count = (const int32_t*)ctx->tables.count.p;
@@ -1120,14 +1209,9 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     A.nbuff = params->nbuff;
     A.variant = params->variant;
     A.early = params->early_exit ? 1 : 0;
-    A.has_eta2 = fields->eta2x != nullptr;
-    A.has_lapd = fields->lapd != nullptr;
-    A.res = *results;
-    A.mask = mask;
 
-    const int64_t ngroups = (np + 63) / 64;
-    int per_cu = 0;
     const bool kahan = params->precision == HNT_PREC_FP32_KAHAN;
+    int per_cu = 0;
     if (kahan)
         HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, shell_analyse_kernel<SumKahan>, 64, 0));
@@ -1159,30 +1243,29 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
     // ---- schedule: order the peaks by their (exact or predicted) gather extent ----
     if (A.early && params->order != HNT_ORDER_PLAIN) {
         stage_begin(ctx, HNT_STAGE_SA_CROSS);
-        HNT_TRY(reserve(ctx, ctx->stops, (size_t)np * 4));
+        HNT_TRY(reserve(ctx, ctx->stops, (size_t)ntot * 4));
+        int32_t* stops = (int32_t*)ctx->stops.p;
         if (params->order == HNT_ORDER_CROSSING) {
             const int64_t G8 = (ngroups + 7) / 8;
-            hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A,
-                               fields->delta, ngroups, G8, (int32_t*)ctx->stops.p);
+            hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A, G8,
+                               stops);
         } else {
-            HNT_TRY(reserve(ctx, ctx->feats, (size_t)np * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4));
+            HNT_TRY(reserve(ctx, ctx->feats,
+                            (size_t)ntot * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4));
             float* feats = (float*)ctx->feats.p;
-            float* pred = feats + np * HNT_MLP_INPUTS;
-            HNT_TRY(bracket_features(ctx, fields, batch, params->alatt, feats));
-            HNT_TRY(mlp_predict(ctx, params->bracket, feats, np, pred));
-            hipLaunchKernelGGL(pred_stop_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
-                               st, pred, np, A.NS - 1, (int32_t*)ctx->stops.p);
+            float* pred = feats + ntot * HNT_MLP_INPUTS;
+            for (int t = 0; t < ntiles; ++t)
+                HNT_TRY(bracket_features(ctx, &fields[t], &batches[t], params->alatt,
+                                         feats + meta[t] * HNT_MLP_INPUTS));
+            HNT_TRY(mlp_predict(ctx, params->bracket, feats, ntot, pred));
+            hipLaunchKernelGGL(pred_stop_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256),
+                               0, st, pred, ntot, A.NS - 1, stops);
         }
-        hipLaunchKernelGGL(stop_key_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                           batch->pi, batch->pj, batch->pk, (const int32_t*)ctx->stops.p, np,
-                           (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);
-        HNT_HIP(hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp.p, tmp_bytes,
-                                                   (uint64_t*)ctx->keys_a.p,
-                                                   (uint64_t*)ctx->keys_b.p,
-                                                   (int32_t*)ctx->vals_a.p,
-                                                   (int32_t*)ctx->vals_b.p, (int)np, 0, 64, st));
+        for (int t = 0; t < ntiles; ++t) prim[t] = stops + meta[t];
+        HNT_TRY(order_by(prim.data()));
         stage_end(ctx, HNT_STAGE_SA_CROSS);
     }
+
     stage_begin(ctx, HNT_STAGE_SA_KERNEL);
     if (kahan)
         hipLaunchKernelGGL(shell_analyse_kernel<SumKahan>, dim3((unsigned)nblocks), dim3(64), 0,
@@ -1191,5 +1274,27 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
         hipLaunchKernelGGL(shell_analyse_kernel<SumF64>, dim3((unsigned)nblocks), dim3(64), 0, st,
                            A);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);
+    return HNT_OK;
+}
+
+}  // namespace
+
+extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
+                                        const hnt_batch* batch, const hnt_shell_tables* tables,
+                                        const hnt_ctab* ctab, const hnt_params* params,
+                                        hnt_results* results, int8_t* mask) {
+    HNT_TRY(begin_call(ctx));
+    HNT_TRY(analyse_tiles(ctx, 1, fields, batch, tables, ctab, params, results, &mask));
     return finish_call(ctx, "hnt_shell_analyse");
+}
+
+extern "C" hnt_status hnt_shell_analyse_tiles(hnt_ctx* ctx, int32_t ntiles,
+                                              const hnt_fields* fields,
+                                              const hnt_batch* batches,
+                                              const hnt_shell_tables* tables,
+                                              const hnt_ctab* ctab, const hnt_params* params,
+                                              hnt_results* results, int8_t* const* masks) {
+    HNT_TRY(begin_call(ctx));
+    HNT_TRY(analyse_tiles(ctx, ntiles, fields, batches, tables, ctab, params, results, masks));
+    return finish_call(ctx, "hnt_shell_analyse_tiles");
 }

@@ -184,6 +184,7 @@ EXPORTED = (
     "hnt_fsc_of_z",
     "hnt_peak_find",
     "hnt_shell_analyse",
+    "hnt_shell_analyse_tiles",
     "hnt_shell_gather_full",
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
@@ -245,6 +246,10 @@ def load(path: str | None = None):
     lib.hnt_kernel_strain.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _f64p]
+    lib.hnt_shell_analyse_tiles.argtypes = [vp, C.c_int32, C.POINTER(Fields), C.POINTER(Batch),
+                                            C.POINTER(ShellTables), C.POINTER(CTab),
+                                            C.POINTER(Params), C.POINTER(Results),
+                                            C.POINTER(C.POINTER(C.c_int8))]
     lib.hnt_bracket_features.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                          C.POINTER(ShellTables), C.c_double, _f32p]
     lib.hnt_mlp_forward.argtypes = [vp, C.POINTER(Mlp), _f32p, C.c_int64, _f32p]

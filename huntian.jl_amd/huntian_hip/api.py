@@ -341,6 +341,37 @@ def _results_struct(r):
     return abi.Results(*vals)
 
 
+def _batch_struct(pi, pj, pk, Rf, ir2min, ZZon, fcrit, ZZon_pp, fcrit_pp, dev):
+    """abi.Batch over device copies of the peak arrays (kept alive by the returned list)."""
+    torch = _torch()
+    npk = int(len(pi))
+    pi_ = _dev1(pi, torch.int32, dev)
+    pj_ = _dev1(pj, torch.int32, dev)
+    pk_ = _dev1(pk, torch.int32, dev)
+    Rf_ = _dev1(np.broadcast_to(np.asarray(Rf, np.float64), (npk,)) if not isinstance(
+        Rf, torch.Tensor) else Rf, torch.float64, dev)
+    ir_ = _dev1(np.broadcast_to(np.asarray(ir2min, np.int32), (npk,)) if not isinstance(
+        ir2min, torch.Tensor) else ir2min, torch.int32, dev)
+    zz = _dev1(ZZon_pp, torch.float64, dev) if ZZon_pp is not None else None
+    fc = _dev1(fcrit_pp, torch.float64, dev) if fcrit_pp is not None else None
+    b = abi.Batch(npk, abi.as_ptr(_ptr(pi_), C.c_int32), abi.as_ptr(_ptr(pj_), C.c_int32),
+                  abi.as_ptr(_ptr(pk_), C.c_int32), abi.as_ptr(_ptr(Rf_), C.c_double),
+                  abi.as_ptr(_ptr(ir_), C.c_int32), abi.as_ptr(_ptr(zz), C.c_double),
+                  abi.as_ptr(_ptr(fc), C.c_double), float(ZZon), float(fcrit))
+    return b, [pi_, pj_, pk_, Rf_, ir_, zz, fc]
+
+
+def _params(alatt, nbuff, variant, early_exit, order, bracket, precision):
+    net = None
+    if order == "learned":
+        if bracket is None:
+            raise ValueError("order='learned' needs a bracket network")
+        net = bracket.struct()
+    return abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)),
+                      abi.ORDERS[order], abi.PRECISIONS[precision],
+                      C.pointer(net) if net is not None else None)
+
+
 def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct: dict, *,
                   alatt: float, ZZon: float, fcrit: float, nbuff: int = 0, variant="C",
                   early_exit: bool = True, mask=None, ZZon_pp=None, fcrit_pp=None,
@@ -357,27 +388,8 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     fs, keep_f = _fields_struct(fields)
     npk = int(len(pi))
     dev = torch.device("cuda", ctx.device)
-    pi_ = _dev1(pi, torch.int32, dev)
-    pj_ = _dev1(pj, torch.int32, dev)
-    pk_ = _dev1(pk, torch.int32, dev)
-    Rf_ = _dev1(np.broadcast_to(np.asarray(Rf, np.float64), (npk,)) if not isinstance(
-        Rf, torch.Tensor) else Rf, torch.float64, dev)
-    ir_ = _dev1(np.broadcast_to(np.asarray(ir2min, np.int32), (npk,)) if not isinstance(
-        ir2min, torch.Tensor) else ir2min, torch.int32, dev)
-    zz = _dev1(ZZon_pp, torch.float64, dev) if ZZon_pp is not None else None
-    fc = _dev1(fcrit_pp, torch.float64, dev) if fcrit_pp is not None else None
-    b = abi.Batch(npk, abi.as_ptr(_ptr(pi_), C.c_int32), abi.as_ptr(_ptr(pj_), C.c_int32),
-                  abi.as_ptr(_ptr(pk_), C.c_int32), abi.as_ptr(_ptr(Rf_), C.c_double),
-                  abi.as_ptr(_ptr(ir_), C.c_int32), abi.as_ptr(_ptr(zz), C.c_double),
-                  abi.as_ptr(_ptr(fc), C.c_double), float(ZZon), float(fcrit))
-    net = None
-    if order == "learned":
-        if bracket is None:
-            raise ValueError("order='learned' needs a bracket network")
-        net = bracket.struct()
-    prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)),
-                     abi.ORDERS[order], abi.PRECISIONS[precision],
-                     C.pointer(net) if net is not None else None)
+    b, keep_b = _batch_struct(pi, pj, pk, Rf, ir2min, ZZon, fcrit, ZZon_pp, fcrit_pp, dev)
+    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
     r = results if results is not None else alloc_results(npk, dev)
     rs = _results_struct(r)
     cs, keep_t = _ctab_struct(ct)
@@ -386,10 +398,45 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     st = ctx.lib.hnt_shell_analyse(ctx.handle, C.byref(fs), C.byref(b), C.byref(ts),
                                    C.byref(cs), C.byref(prm), C.byref(rs), m)
     abi.check(ctx.lib, st)
-    del keep_f, keep_t
+    del keep_f, keep_t, keep_b
     if npk == 0:
         return {k: v[:0] for k, v in r.items()}
     return r
+
+
+def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
+                        fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
+                        order: str = "crossing", bracket=None, precision: str = "fp64"):
+    """hnt_shell_analyse_tiles: every tile in one persistent launch.
+
+    tiles: list of dicts with keys fields, pi, pj, pk, Rf, ir2min and optionally mask,
+    ZZon_pp, fcrit_pp.  Returns one result dict (as shell_analyse) per tile."""
+    torch = _require_gpu()
+    ctx = context()
+    dev = torch.device("cuda", ctx.device)
+    nt = len(tiles)
+    FA = (abi.Fields * nt)()
+    BA = (abi.Batch * nt)()
+    RA = (abi.Results * nt)()
+    MA = (C.POINTER(C.c_int8) * nt)()
+    keep, out = [], []
+    for q, t in enumerate(tiles):
+        fs, kf = _fields_struct(t["fields"])
+        b, kb = _batch_struct(t["pi"], t["pj"], t["pk"], t["Rf"], t["ir2min"], ZZon, fcrit,
+                              t.get("ZZon_pp"), t.get("fcrit_pp"), dev)
+        r = alloc_results(int(len(t["pi"])), dev)
+        FA[q], BA[q], RA[q] = fs, b, _results_struct(r)
+        MA[q] = abi.as_ptr(_ptr(t.get("mask")), C.c_int8)
+        keep += [kf, kb]
+        out.append(r)
+    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
+    cs, keep_t = _ctab_struct(ct)
+    ts = tables.struct()
+    st = ctx.lib.hnt_shell_analyse_tiles(ctx.handle, nt, FA, BA, C.byref(ts), C.byref(cs),
+                                         C.byref(prm), RA, MA)
+    abi.check(ctx.lib, st)
+    del keep, keep_t
+    return [{k: v[:len(t["pi"])] for k, v in r.items()} for r, t in zip(out, tiles)]
 
 
 def _host_results(r, npk):

@@ -235,6 +235,20 @@ hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields, const hnt_b
                              const hnt_shell_tables* tables, const hnt_ctab* ctab,
                              const hnt_params* params, hnt_results* results, int8_t* mask);
 
+/*
+ * hnt_shell_analyse_tiles -- hnt_shell_analyse over many tiles in one persistent launch (no
+ * counterpart in the reference, whose drivers call analyse_peaks_gpu_cuda_multirf once per
+ * tile, src/MultiResolution.jl:532-644).  fields, batches, results: arrays of ntiles
+ * structs; masks: host array of ntiles device pointers (entries may be NULL) or NULL.  Tiles
+ * may differ in size; peaks of all tiles share the work queues, so tiles of uneven
+ * candidate count and shell depth keep every CU busy.  Results equal those of one
+ * hnt_shell_analyse call per tile.
+ */
+hnt_status hnt_shell_analyse_tiles(hnt_ctx* ctx, int32_t ntiles, const hnt_fields* fields,
+                                   const hnt_batch* batches, const hnt_shell_tables* tables,
+                                   const hnt_ctab* ctab, const hnt_params* params,
+                                   hnt_results* results, int8_t* const* masks);
+
 /* ---------------------------------------------------------------- primitive entries */
 /* Single-primitive entry points; the reference exposes them as test harnesses
  * (src/Huntian.jl:98-217). */

@@ -1,0 +1,63 @@
+"""hnt_shell_analyse_tiles: tiles of uneven size and candidate count in one persistent launch
+give exactly the results (and masks) of one hnt_shell_analyse call per tile, which the
+parity tests tie to the oracle; one tile is also checked against the oracle directly."""
+import numpy as np
+import pytest
+
+from huntian_hip import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _tile(n, seed, nbuff, oracle):
+    box = 2.0 * n
+    delta, dk = synth.gaussian_field(n, box, seed)
+    f = {"delta": delta}
+    f.update(synth.lpt_fields(dk, n, box))
+    f["lapd"] = synth.laplacian_field(dk, n, box)
+    radii = synth.filter_radii(1.5, 12.0, 5)
+    sm = [synth.smooth_tophat(dk, n, box, R) for R in radii]
+    pk, _ = oracle.peak_find_multi(sm, [1.5] * len(radii), nbuff,
+                                   np.zeros((n, n, n), np.int8, order="F"))
+    Rf = np.array([radii[q] for q in pk["filt"]])
+    ir2 = np.array([api.ir2min_of(r, 2.0) for r in Rf], np.int32)
+    return dict(fields=f, pi=pk["i"], pj=pk["j"], pk=pk["k"], Rf=Rf, ir2min=ir2)
+
+
+@pytest.mark.parametrize("order", ["crossing", "plain"])
+def test_tiles_equal_one_call_per_tile(oracle, websky, order):
+    import torch
+
+    nbuff, nhunt = 11, 10
+    specs = [(32, 101), (48, 102), (24, 103), (40, 104)]
+    tiles = [_tile(n, s, nbuff, oracle) for n, s in specs]
+    # an empty tile in the middle
+    e = tiles[2]
+    tiles.insert(2, dict(fields=e["fields"], pi=e["pi"][:0], pj=e["pj"][:0], pk=e["pk"][:0],
+                         Rf=e["Rf"][:0], ir2min=e["ir2min"][:0]))
+    stab = api.build_shell_tables(nhunt)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    kw = dict(alatt=2.0, ZZon=1.0, fcrit=fcrit, nbuff=nbuff, order=order)
+    for t in tiles:
+        n = t["fields"]["delta"].shape[0]
+        t["mask"] = api.to_device_field(np.zeros((n, n, n), np.int8), dtype=torch.int8)
+    got = api.shell_analyse_tiles(tiles, stab, websky, **kw)
+    assert sum(len(t["pi"]) for t in tiles) > 200
+    for t, g in zip(tiles, got):
+        n = t["fields"]["delta"].shape[0]
+        m1 = api.to_device_field(np.zeros((n, n, n), np.int8), dtype=torch.int8)
+        one = api.shell_analyse(t["fields"], t["pi"], t["pj"], t["pk"], t["Rf"], t["ir2min"],
+                                stab, websky, mask=m1, **kw)
+        for k in one:
+            assert torch.equal(torch.nan_to_num(one[k], nan=123.0),
+                               torch.nan_to_num(g[k], nan=123.0)), k
+        assert torch.equal(m1, t["mask"])
+    # first tile against the oracle directly
+    t = tiles[0]
+    ref = oracle.analyse_peaks(t["fields"], t["pi"], t["pj"], t["pk"], t["Rf"], t["ir2min"],
+                               oracle.shell_tables(nhunt), websky, alatt=2.0, ZZon=1.0,
+                               fcrit=fcrit, variant="C", nbuff=nbuff, early_exit=True)
+    g0 = {k: v.cpu().numpy() for k, v in got[0].items()}
+    assert np.array_equal(g0["RTHL"] > 0, ref["RTHL"] > 0)
+    assert np.array_equal(g0["nshells_used"], ref["nshells_used"])
+    np.testing.assert_allclose(g0["RTHL"], ref["RTHL"], rtol=1e-10)
