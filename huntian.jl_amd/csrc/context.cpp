@@ -307,6 +307,21 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
     return HNT_OK;
 }
 
+hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value) {
+    clear_error();
+    HNT_REQUIRE(ctx, "hnt_ctx_set_tuning: NULL context");
+    if (what == HNT_TUNE_BRICK_LOG2) {
+        HNT_REQUIRE(value >= 0 && value <= 13, "hnt_ctx_set_tuning: brick_log2 %d", value);
+        ctx->brick_log2 = value;
+    } else if (what == HNT_TUNE_WAVES_PER_CU) {
+        HNT_REQUIRE(value >= 0 && value <= 64, "hnt_ctx_set_tuning: waves_per_cu %d", value);
+        ctx->waves_per_cu = value;
+    } else {
+        HNT_REQUIRE(false, "hnt_ctx_set_tuning: unknown knob %d", what);
+    }
+    return HNT_OK;
+}
+
 hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on) {
     HNT_TRY(begin_call(ctx));
     if (on && !ctx->ev[0])

@@ -85,6 +85,8 @@ struct hnt_ctx {
     hnt::MlpScratch mlp;
     hnt::DevBuf tiles, tilemeta, slots, gtiles;  // multi-tile shell analysis
     std::vector<hnt::TileDev> tile_host;
+    int brick_log2 = 6;  // peak-order bricks of 2^brick_log2 cells a side (0..13)
+    int waves_per_cu = 0;  // cap on resident shell-analysis waves per CU (0 = occupancy)
     std::vector<int64_t> meta_host;
     // stage timers (hnt_ctx_set_profiling)
     bool profiling = false;

@@ -112,20 +112,26 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
     return x;
 }
 
-// Sort key of a tile's peaks: (tile, primary, Morton code of the peak cell), value = global
-// peak id base + q.  primary is ir2min for the first order and the stop shell for the
-// second; it is clamped to 11 bits (it only orders).
+// Sort key of a tile's peaks, value = global peak id base + q.  Bits, high to low: tile (14),
+// Morton code of the peak's brick of 2^L cells a side (39 - 3L), primary (11), Morton code
+// of the cell within its brick (3L).  Consecutive peaks therefore share a brick (and the
+// waves resident on one XCD a neighbourhood of bricks, whose records stay in its L2), and
+// within a brick they are ordered by the primary: ir2min for the first order, the stop
+// shell for the second, so that a wavefront's lanes have similar gather depths.
 __global__ void tile_key_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
                                 const int32_t* __restrict__ pk, const int32_t* __restrict__ prim,
-                                int64_t np, uint64_t tile, int64_t base,
+                                int64_t np, uint64_t tile, int64_t base, int L,
                                 uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
-    uint64_t mort = spread3((uint32_t)(pi[q] - 1) & 0x1fff) |
-                    (spread3((uint32_t)(pj[q] - 1) & 0x1fff) << 1) |
-                    (spread3((uint32_t)(pk[q] - 1) & 0x1fff) << 2);
-    uint64_t g = (uint64_t)min(max(prim[q], 0), 2047);
-    keys[base + q] = (tile << 50) | (g << 39) | mort;
+    const uint64_t mort = spread3((uint32_t)(pi[q] - 1) & 0x1fff) |
+                          (spread3((uint32_t)(pj[q] - 1) & 0x1fff) << 1) |
+                          (spread3((uint32_t)(pk[q] - 1) & 0x1fff) << 2);
+    const uint64_t g = (uint64_t)min(max(prim[q], 0), 2047);
+    const int lb = 3 * L;
+    const uint64_t local = mort & ((1ull << lb) - 1ull);
+    const uint64_t brick = mort >> lb;
+    keys[base + q] = (tile << 50) | (brick << (lb + 11)) | (g << lb) | local;
     vals[base + q] = (int32_t)(base + q);
 }
 
@@ -1164,8 +1170,8 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
             if (b.npeaks == 0) continue;
             hipLaunchKernelGGL(tile_key_kernel, dim3((unsigned)((b.npeaks + 255) / 256)),
                                dim3(256), 0, st, b.pi, b.pj, b.pk, prim[t], b.npeaks,
-                               (uint64_t)t, meta[t], (uint64_t*)ctx->keys_a.p,
-                               (int32_t*)ctx->vals_a.p);
+                               (uint64_t)t, meta[t], ctx->brick_log2,
+                               (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);
         }
         hipError_t e;
         sort_pairs(ctx, tmp_bytes, ntot, st, &e);
@@ -1219,6 +1225,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
         HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, shell_analyse_kernel<SumF64>, 64, 0));
     if (per_cu < 1) per_cu = 1;
+    if (ctx->waves_per_cu > 0 && ctx->waves_per_cu < per_cu) per_cu = ctx->waves_per_cu;
     A.slot_doubles = (int64_t)(A.NS + 1) * PW;
     const size_t slot_bytes = (size_t)A.slot_doubles * 8;
     const size_t budget = (size_t)24 << 30;  // profile scratch cap

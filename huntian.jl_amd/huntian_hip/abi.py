@@ -112,6 +112,7 @@ HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
 ORDERS = {"crossing": HNT_ORDER_CROSSING, "plain": HNT_ORDER_PLAIN,
           "learned": HNT_ORDER_LEARNED}
 PRECISIONS = {"fp64": 0, "fp32_kahan": 1}
+TUNING = {"brick_log2": 0, "waves_per_cu": 1}
 
 
 class Mlp(C.Structure):
@@ -177,6 +178,7 @@ EXPORTED = (
     "hnt_ctx_set_stream",
     "hnt_last_error",
     "hnt_abi_version",
+    "hnt_ctx_set_tuning",
     "hnt_ctx_set_profiling",
     "hnt_ctx_get_profile",
     "hnt_shell_table_size",
@@ -225,6 +227,7 @@ def load(path: str | None = None):
     lib.hnt_last_error.argtypes = [C.c_char_p, C.c_size_t]
     lib.hnt_last_error.restype = C.c_size_t
     lib.hnt_abi_version.restype = C.c_int
+    lib.hnt_ctx_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
     lib.hnt_ctx_set_profiling.argtypes = [vp, C.c_int]
     lib.hnt_ctx_get_profile.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                         C.c_int32]

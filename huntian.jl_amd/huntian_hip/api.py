@@ -57,6 +57,11 @@ class Context:
             abi.check(self.lib, self.lib.hnt_ctx_set_stream(self.handle, C.c_void_p(ptr)))
             self._stream_ptr = ptr
 
+    def set_tuning(self, **knobs):
+        """Launch-shape knobs (brick_log2, waves_per_cu); speed only, never results."""
+        for k, v in knobs.items():
+            abi.check(self.lib, self.lib.hnt_ctx_set_tuning(self.handle, abi.TUNING[k], int(v)))
+
     def set_profiling(self, on: bool = True):
         abi.check(self.lib, self.lib.hnt_ctx_set_profiling(self.handle, int(bool(on))))
 

@@ -78,6 +78,14 @@ enum {
 hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on);
 hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t nstages);
 
+/* Launch-shape knobs of a context (no counterpart in the reference; set by the autotuner,
+ * huntian_hip/autotune.py).  They change speed, never results.
+ *  BRICK_LOG2:   shell-analysis peak order groups peaks in bricks of 2^v cells a side
+ *                (0..13; 13 = no bricks), default 6
+ *  WAVES_PER_CU: cap on resident shell-analysis wavefronts per CU (0 = as many as fit) */
+enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1 };
+hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value);
+
 /* ---------------------------------------------------------------- shells and tables */
 
 /* Lattice offsets within radius rmax, sorted by r^2 with a stable sort, grouped into shells
