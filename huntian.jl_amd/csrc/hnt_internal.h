@@ -87,6 +87,7 @@ struct hnt_ctx {
     std::vector<hnt::TileDev> tile_host;
     int brick_log2 = 6;  // peak-order bricks of 2^brick_log2 cells a side (0..13)
     int waves_per_cu = 0;  // cap on resident shell-analysis waves per CU (0 = occupancy)
+    int gang = 1;          // lanes per peak in the shell gather: 1 (exact order) or 16
     std::vector<int64_t> meta_host;
     // stage timers (hnt_ctx_set_profiling)
     bool profiling = false;

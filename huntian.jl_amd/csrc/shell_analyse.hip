@@ -866,6 +866,281 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
 }
 
+// ------------------------------------------------------------------ ganged gather
+// Sixteen lanes (one DPP row) per peak, four peaks per wavefront at a time.  Lane k of a
+// gang sums cells k, k + 16, ... of each shell; the sixteen partial sums of each quantity
+// are then added in lane order through LDS, one or two quantities per lane.  Compared with
+// one lane per peak this keeps a quarter as many peaks in flight per XCD, so the cell
+// records they read stay in L2.  The sums are the same FP64 sums of the same Float32 terms
+// in another order (results agree with the oracle to ~1e-13 instead of bit for bit).
+// A wavefront still owns a group of 64 peaks: it gathers them four at a time (16 rounds,
+// in schedule order, so the four have similar depths), then post-processes all 64 with
+// one lane per peak.
+constexpr int GANG = 16;
+constexpr int QP = 26;  // LDS row of one lane's partial sums (25 used)
+// row layout: F, S0..2, T0..2, G0..2, H0..2, R00 R01 R02 R10 R11 R12 R20 R21 R22, nin, lap, lapn
+constexpr int Q_F = 0, Q_S = 1, Q_T = 4, Q_G = 7, Q_H = 10, Q_R = 13, Q_NIN = 22, Q_LAP = 23,
+              Q_LAPN = 24;
+
+template <class S, bool CHECK>
+__device__ void gang_gather(const SAArgs& a, const TileDev& T, int p, double* __restrict__ P,
+                            double* __restrict__ lds) {
+    const int lane = threadIdx.x, gang = lane >> 4, k = lane & (GANG - 1), row0 = gang * GANG;
+    const bool valid = p >= 0;
+    int ci = 1, cj = 1, ck = 1, ir2min = 0;
+    double fcrit = 0.0, Rf = 0.0;
+    if (valid) {
+        ci = T.pi[p];
+        cj = T.pj[p];
+        ck = T.pk[p];
+        ir2min = T.ir2min[p];
+        Rf = T.Rf[p];
+        fcrit = T.fcrit ? T.fcrit[p] : T.fcrit_s;
+    }
+    const double* __restrict__ rad = a.rad;
+    const int m = a.NS - 1;
+    int mrf = 1, mlow_rf = 1, mupp_rf = 1;
+    if (valid) rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
+    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
+    const bool varA = a.variant == HNT_VARIANT_A;
+    const bool eta2 = T.has_eta2, haslap = T.has_lapd;
+
+    // gang-uniform: Fprev, dFbarp (every lane of the gang computes them identically);
+    // lane-held running values: k = 0 cumulative n, 1..3 cumulative S, 4..6 cumulative S2,
+    // 10 running lapd sum, 11 running lapd count
+    double Fprev = 0.0, dFbarp = 0.0, run = 0.0;
+    if (valid) {
+        const float4 v0 = T.recs[2 * c0], v1 = T.recs[2 * c0 + 1];
+        Fprev = (double)v0.x;
+        dFbarp = Fprev;
+        const float e3[3] = {v0.z, v0.w, v1.x}, t3[3] = {v1.y, v1.z, v1.w};
+        if (k == 0) run = 1.0;
+        else if (k <= 3) run = 0.0 + (double)e3[k - 1];
+        else if (k <= 6) run = eta2 ? 0.0 + (double)t3[k - 4] : 0.0;
+        else if (k == 10) run = (haslap && !varA) ? 0.0 + (double)T.lapd[c0] : 0.0;
+        else if (k == 11) run = (haslap && !varA) ? 1.0 : 0.0;
+        double* X = P + PW;
+        if (k < 12) X[k] = 0.0;
+        if (k == 0) {
+            X[K_F] = Fprev;
+            X[K_N] = run;
+        } else if (k <= 3) {
+            X[K_S + k - 1] = run;
+        } else if (k <= 6) {
+            X[K_S2 + k - 4] = run;
+        } else if (k == 10) {
+            X[K_LAP] = run;
+        } else if (k == 11) {
+            X[K_LAPN] = run;
+        }
+    }
+
+    bool active = valid, past = false, crossed = false, stop_set = false, oobA = false;
+    double rstop = 0.0;
+    int stop = m, scross = 0;
+    double* __restrict__ row = lds + lane * QP;
+    for (int s = 2; s <= m; ++s) {
+        if (__ballot(active) == 0ull) break;
+        const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
+        const double radS = rad[s], radP = rad[s - 1];
+        // variant A stops adding lapd at the first out-of-bounds cell in cell order
+        int cutA = nc;
+        if (CHECK && varA && haslap) {
+            int first = nc;
+            if (active)
+                for (int c = k; c < nc; c += GANG) {
+                    const int4 o = a.cells[st + c];
+                    const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
+                    if (i1 < 1 || i1 > T.n || i2 < 1 || i2 > T.n || i3 < 1 || i3 > T.n) {
+                        first = c;
+                        break;
+                    }
+                }
+#pragma unroll
+            for (int w = 1; w < GANG; w <<= 1) first = min(first, __shfl_xor(first, w));
+            cutA = oobA ? 0 : first;
+        }
+        Acc<S> A;
+        A.zero();
+        A.lap = 0.0;
+        A.lapn = 0;
+        A.nin = 0;
+        if (active) {
+            if (!CHECK) {
+                int c = k;
+                for (; c + GANG < nc; c += 2 * GANG) {
+                    const int4 o0 = a.cells[st + c], o1 = a.cells[st + c + GANG];
+                    const int64_t f0 = c0 + o0.x + o0.y * T.s1 + o0.z * T.s2;
+                    const int64_t f1 = c0 + o1.x + o1.y * T.s1 + o1.z * T.s2;
+                    const float4 a0 = T.recs[2 * f0], b0 = T.recs[2 * f0 + 1];
+                    const float4 a1 = T.recs[2 * f1], b1 = T.recs[2 * f1 + 1];
+                    const float l0 = haslap ? T.lapd[f0] : 0.0f;
+                    const float l1 = haslap ? T.lapd[f1] : 0.0f;
+                    acc_cell(A, a0, b0, (float)o0.x, (float)o0.y, (float)o0.z, eta2);
+                    acc_cell(A, a1, b1, (float)o1.x, (float)o1.y, (float)o1.z, eta2);
+                    A.lap += (double)l0;
+                    A.lap += (double)l1;
+                }
+                if (c < nc) {
+                    const int4 o0 = a.cells[st + c];
+                    const int64_t f0 = c0 + o0.x + o0.y * T.s1 + o0.z * T.s2;
+                    const float4 a0 = T.recs[2 * f0], b0 = T.recs[2 * f0 + 1];
+                    acc_cell(A, a0, b0, (float)o0.x, (float)o0.y, (float)o0.z, eta2);
+                    if (haslap) A.lap += (double)T.lapd[f0];
+                }
+                // cells of this lane: ceil((nc - k) / 16)
+                A.nin = k < nc ? (nc - k + GANG - 1) / GANG : 0;
+                A.lapn = haslap ? A.nin : 0;
+            } else {
+                for (int c = k; c < nc; c += GANG) {
+                    const int4 o = a.cells[st + c];
+                    const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
+                    if (i1 < 1 || i1 > T.n || i2 < 1 || i2 > T.n || i3 < 1 || i3 > T.n) continue;
+                    const int64_t f = c0 + o.x + o.y * T.s1 + o.z * T.s2;
+                    const float4 v0 = T.recs[2 * f], v1 = T.recs[2 * f + 1];
+                    acc_cell(A, v0, v1, (float)o.x, (float)o.y, (float)o.z, eta2);
+                    A.nin += 1;
+                    if (haslap && (!varA || c < cutA)) {
+                        A.lap += (double)T.lapd[f];
+                        A.lapn += 1;
+                    }
+                }
+            }
+        }
+        if (CHECK && varA && haslap && cutA < nc) oobA = true;
+        // partial sums to LDS, then lane (gang, k) adds rows row0..row0+15 of its quantities
+        row[Q_F] = A.F.val();
+        row[Q_S] = A.S0.val(); row[Q_S + 1] = A.S1.val(); row[Q_S + 2] = A.S2.val();
+        row[Q_T] = A.T0.val(); row[Q_T + 1] = A.T1.val(); row[Q_T + 2] = A.T2.val();
+        row[Q_G] = A.G0.val(); row[Q_G + 1] = A.G1.val(); row[Q_G + 2] = A.G2.val();
+        row[Q_H] = A.H0.val(); row[Q_H + 1] = A.H1.val(); row[Q_H + 2] = A.H2.val();
+        row[Q_R] = A.R00.val(); row[Q_R + 1] = A.R01.val(); row[Q_R + 2] = A.R02.val();
+        row[Q_R + 3] = A.R10.val(); row[Q_R + 4] = A.R11.val(); row[Q_R + 5] = A.R12.val();
+        row[Q_R + 6] = A.R20.val(); row[Q_R + 7] = A.R21.val(); row[Q_R + 8] = A.R22.val();
+        row[Q_NIN] = (double)A.nin;
+        row[Q_LAP] = A.lap;
+        row[Q_LAPN] = (double)A.lapn;
+        __syncthreads();
+        // quantity of lane k (and a second one for some lanes)
+        int q1, q2 = -1;
+        if (k <= 6) q1 = k;                                   // F, S, S2
+        else if (k <= 9) { q1 = Q_G + (k - 7); q2 = Q_R + 4 * (k - 7); }  // G and R00/R11/R22
+        else if (k == 10) { q1 = Q_H; q2 = Q_LAP; }
+        else if (k == 11) { q1 = Q_H + 1; q2 = Q_LAPN; }
+        else if (k == 12) { q1 = Q_H + 2; q2 = Q_NIN; }
+        else if (k == 13) { q1 = Q_R + 1; q2 = Q_R + 3; }    // R01, R10
+        else if (k == 14) { q1 = Q_R + 2; q2 = Q_R + 6; }    // R02, R20
+        else { q1 = Q_R + 5; q2 = Q_R + 7; }                  // R12, R21
+        double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < GANG; ++r) {
+            const double* rr = lds + (row0 + r) * QP;
+            v1 += rr[q1];
+            if (q2 >= 0) v2 += rr[q2];
+        }
+        __syncthreads();
+        // Fbar recurrence, identical in every lane of the gang
+        const double Ftot = __shfl(v1, row0);
+        const double nin = __shfl(v2, row0 + 12);
+        const double dFbar = nin > 0.0 ? Ftot / nin : 0.0;
+        const double rad3p = cube(radP), rad3 = cube(radS);
+        const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
+        Fprev = Fb;
+        dFbarp = dFbar;
+        if (active) {
+            double* X = P + (int64_t)s * PW;
+            if (k == 0) {
+                if (nin > 0.0) run += nin;
+                X[K_F] = Fb;
+                X[K_N] = run;
+            } else if (k <= 3) {
+                if (nin > 0.0) run += v1;
+                X[K_S + k - 1] = run;
+            } else if (k <= 6) {
+                if (nin > 0.0) run += v1;
+                X[K_S2 + k - 4] = run;
+            } else if (k <= 9) {
+                X[k - 7] = v1 / radS;
+                const double rd = v2 / radS;
+                X[k == 7 ? 6 : (k == 8 ? 9 : 11)] = rd + rd;
+            } else if (k <= 12) {
+                X[3 + (k - 10)] = v1 / radS;
+                if (k == 10) {
+                    run += v2;
+                    X[K_LAP] = run;
+                } else if (k == 11) {
+                    run += v2;
+                    X[K_LAPN] = run;
+                }
+            } else {
+                const double ra = v1 / radS, rb = v2 / radS;
+                X[k == 13 ? 7 : (k == 14 ? 8 : 10)] = ra + rb;
+            }
+            if (!past && a.shell_r2[s - 1] > ir2min) past = true;
+            if (past && !crossed && Fb < fcrit) {
+                crossed = true;
+                scross = s;
+                rstop = rad[s + 1] + 2.0;
+            }
+            if (!stop_set && crossed && rad[s + 1] >= rstop && s >= mupp_rf) {
+                stop = s;
+                stop_set = true;
+                if (a.early) active = false;
+            }
+        }
+    }
+    if (valid && k == 0) {
+        if (T.res.nshells_used) T.res.nshells_used[p] = stop;
+        if (T.res.crossing_shell) T.res.crossing_shell[p] = scross;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void shell_gang_kernel(SAArgs a) {
+    __shared__ double lds[64 * QP];
+    double* Pbase = a.prof + (int64_t)blockIdx.x * 64 * a.slot_doubles;
+    int q = blockIdx.x & 7;
+    for (int tried = 0; tried < 8;) {
+        unsigned long long g = 0;
+        if (threadIdx.x == 0) g = atomicAdd(&a.qctr[q], 1ull);
+        g = __shfl(g, 0);
+        const int64_t grp = a.qbeg[q] + (int64_t)g;
+        if (grp >= a.qend[q]) {
+            ++tried;
+            q = (q + 1) & 7;
+            continue;
+        }
+        const TileDev& T = a.tiles[a.gtile[grp]];
+        const int R = a.rmax;
+        for (int r = 0; r < 64 / 4; ++r) {
+            const int idx = r * 4 + (threadIdx.x >> 4);
+            const int p = a.slot[grp * 64 + idx];
+            if (__ballot(p >= 0) == 0ull) continue;
+            bool inside = true;
+            if (p >= 0) {
+                const int ci = T.pi[p], cj = T.pj[p], ck = T.pk[p];
+                inside = ci - R >= 1 && ci + R <= T.n && cj - R >= 1 && cj + R <= T.n &&
+                         ck - R >= 1 && ck + R <= T.n;
+            }
+            double* P = Pbase + (int64_t)idx * a.slot_doubles;
+            if (__ballot(!inside) == 0ull) gang_gather<S, false>(a, T, p, P, lds);
+            else gang_gather<S, true>(a, T, p, P, lds);
+        }
+        // radial logic of the 64 peaks, one lane each
+        const int lane = threadIdx.x;
+        const int p = a.slot[grp * 64 + lane];
+        if (p >= 0) {
+            const int ci = T.pi[p], cj = T.pj[p], ck = T.pk[p];
+            const double ZZon = T.ZZon ? T.ZZon[p] : T.ZZon_s;
+            const double fcrit = T.fcrit ? T.fcrit[p] : T.fcrit_s;
+            int mrf, mlow_rf, mupp_rf;
+            rf_window(a, T.Rf[p], mrf, mlow_rf, mupp_rf);
+            post_process(a, T, p, Pbase + (int64_t)lane * a.slot_doubles, ci, cj, ck, ZZon,
+                         fcrit, T.ir2min[p], mrf, mlow_rf, mupp_rf);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ crossing pass
 // Work of a peak = shells gathered, fixed by its crossing shell s_c (first shell past ir2min
 // with Fbar < fcrit) and its filter-scale window.  This pass finds s_c from delta alone,
@@ -1217,13 +1492,13 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
     A.early = params->early_exit ? 1 : 0;
 
     const bool kahan = params->precision == HNT_PREC_FP32_KAHAN;
+    const bool gang = ctx->gang == 16;
+    const void* kern = gang ? (kahan ? (const void*)shell_gang_kernel<SumKahan>
+                                     : (const void*)shell_gang_kernel<SumF64>)
+                            : (kahan ? (const void*)shell_analyse_kernel<SumKahan>
+                                     : (const void*)shell_analyse_kernel<SumF64>);
     int per_cu = 0;
-    if (kahan)
-        HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, shell_analyse_kernel<SumKahan>, 64, 0));
-    else
-        HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, shell_analyse_kernel<SumF64>, 64, 0));
+    HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
     if (per_cu < 1) per_cu = 1;
     if (ctx->waves_per_cu > 0 && ctx->waves_per_cu < per_cu) per_cu = ctx->waves_per_cu;
     A.slot_doubles = (int64_t)(A.NS + 1) * PW;
@@ -1274,7 +1549,12 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
     }
 
     stage_begin(ctx, HNT_STAGE_SA_KERNEL);
-    if (kahan)
+    if (gang && kahan)
+        hipLaunchKernelGGL(shell_gang_kernel<SumKahan>, dim3((unsigned)nblocks), dim3(64), 0, st,
+                           A);
+    else if (gang)
+        hipLaunchKernelGGL(shell_gang_kernel<SumF64>, dim3((unsigned)nblocks), dim3(64), 0, st, A);
+    else if (kahan)
         hipLaunchKernelGGL(shell_analyse_kernel<SumKahan>, dim3((unsigned)nblocks), dim3(64), 0,
                            st, A);
     else

@@ -82,8 +82,12 @@ hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t
  * huntian_hip/autotune.py).  They change speed, never results.
  *  BRICK_LOG2:   shell-analysis peak order groups peaks in bricks of 2^v cells a side
  *                (0..13; 13 = no bricks), default 6
- *  WAVES_PER_CU: cap on resident shell-analysis wavefronts per CU (0 = as many as fit) */
-enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1 };
+ *  WAVES_PER_CU: cap on resident shell-analysis wavefronts per CU (0 = as many as fit)
+ *  GANG:         lanes per peak in the shell gather.  1: each lane sums its peak's cells in
+ *                the reference's order (results equal the FP64 CPU path bit for bit);
+ *                16: sixteen lanes share a peak and their partial sums are added in lane
+ *                order (the same FP64 sums in another order, ~1e-13 relative) */
+enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1, HNT_TUNE_GANG = 2 };
 hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value);
 
 /* ---------------------------------------------------------------- shells and tables */

@@ -19,6 +19,16 @@ FLOAT_KEYS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half", "eigs", 
 RTOL = 1e-10
 
 
+@pytest.fixture(params=[1, 16], ids=["lane", "gang16"])
+def gang(request):
+    """Run a test with one lane per peak (the reference's summation order) and with sixteen
+    lanes per peak (the same sums in lane order); the bar is the same."""
+    ctx = api.context()
+    ctx.set_tuning(gang=request.param)
+    yield request.param
+    ctx.set_tuning(gang=1)
+
+
 def to_host(r):
     out = {k: v.cpu().numpy() for k, v in r.items() if v is not None}
     out["strain"] = np.swapaxes(out["strain"], 1, 2)  # [p, K, L] storage -> [p, L, K]
@@ -122,7 +132,7 @@ def _bump_peaks(oracle, f, nbuff=1, fcrit=0.12):
 
 @pytest.mark.parametrize("variant", ["A", "B", "C"])
 @pytest.mark.parametrize("early", [True, False])
-def test_shell_analyse_bumps(oracle, bumps, variant, early):
+def test_shell_analyse_bumps(oracle, bumps, variant, early, gang):
     import torch
 
     ct = nontrivial_ctab(make_ctab)
@@ -148,7 +158,7 @@ def test_shell_analyse_bumps(oracle, bumps, variant, early):
     assert (ref["RTHL"] > 0).sum() >= 3
 
 
-def test_shell_analyse_boundary_and_optional_fields(oracle, bumps):
+def test_shell_analyse_boundary_and_optional_fields(oracle, bumps, gang):
     """Peaks whose shells leave the grid (bounds-checked path), no psi_2, no lapd,
     per-peak ZZon / fcrit."""
     ct = nontrivial_ctab(make_ctab)
@@ -175,7 +185,7 @@ def test_shell_analyse_boundary_and_optional_fields(oracle, bumps):
         assert np.all(g["Sbar2"] == 0) and np.all(g["d2F"] == 0)
 
 
-def test_fortran_golden_vector_on_gpu(oracle, fortran_table):
+def test_fortran_golden_vector_on_gpu(oracle, fortran_table, gang):
     gold = FORTRAN_GOLDEN
     stab = api.build_shell_tables(8)
     for variant in ("A", "B", "C"):
@@ -221,7 +231,7 @@ def cosmo64():
     return f, radii, sm
 
 
-def test_end_to_end_realistic_field(oracle, websky, cosmo64):
+def test_end_to_end_realistic_field(oracle, websky, cosmo64, gang):
     """Peak finding over the filter bank, then every candidate's collapse, GPU vs oracle."""
     import torch
 
@@ -309,3 +319,25 @@ def test_fp32_kahan_sums_within_tolerance(oracle, websky, cosmo64):
         worst[k] = float(err.max())
         assert worst[k] <= tol, (k, worst[k])
     print(f"fp32_kahan vs FP64 oracle over {halo.sum()} halos: worst relative {worst}")
+
+
+def test_boundary_peaks_with_lapd(oracle, bumps, gang):
+    """d2F of peaks whose shells leave the grid: variant A stops adding the Laplacian at the
+    first out-of-bounds cell (RadialShell.jl:784-799), B and C skip such cells."""
+    ct = nontrivial_ctab(make_ctab)
+    assert bumps.get("lapd") is not None
+    rmax = 9
+    t = oracle.shell_tables(rmax)
+    stab = api.build_shell_tables(rmax)
+    pi, pj, pk, _ = _bump_peaks(oracle, bumps)
+    pi = np.concatenate([pi, [1, 56, 3, 55, 5, 52]]).astype(np.int32)
+    pj = np.concatenate([pj, [1, 20, 54, 2, 28, 30]]).astype(np.int32)
+    pk = np.concatenate([pk, [24, 56, 2, 30, 6, 50]]).astype(np.int32)
+    for variant in ("A", "B", "C"):
+        ref = oracle_run(oracle, bumps, pi, pj, pk, 3.0, 9, t, ct, alatt=1.0, ZZon=0.0,
+                         fcrit=1.0, variant=variant, early_exit=True)
+        r = api.shell_analyse(bumps, pi, pj, pk, 3.0, 9, stab, ct, alatt=1.0, ZZon=0.0,
+                              fcrit=1.0, variant=variant, early_exit=True)
+        g = to_host(r)
+        compare(g, ref, f"{variant} boundary+lapd")
+        assert np.any(g["d2F"] != 0)
