@@ -54,6 +54,10 @@ def main():
     ap.add_argument("--variant", default="C", choices=["A", "B", "C"])
     ap.add_argument("--order", default="crossing", choices=["crossing", "plain", "learned"],
                     help="wavefront schedule of the shell analysis (never changes a result)")
+    ap.add_argument("--gang", type=int, default=None, choices=[1, 16],
+                    help="lanes per peak in the shell gather (library default if omitted)")
+    ap.add_argument("--brick-log2", type=int, default=None,
+                    help="peak-order brick size 2^v cells (library default if omitted)")
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip timing the shell analysis under the other schedules")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -106,6 +110,10 @@ def main():
     mask = api.fortran_empty((n, n, n), torch.int8, dev)
     cap = 48_000_000
     ctx = api.context(0)
+    knobs = {k: v for k, v in (("gang", args.gang), ("brick_log2", args.brick_log2))
+             if v is not None}
+    if knobs:
+        ctx.set_tuning(**knobs)
     from huntian_hip import bracket
 
     net = None

@@ -11,6 +11,7 @@
 // Matrix-core tile: v_mfma_f32_16x16x32_bf16 (gfx950).  Lane l holds A[l % 16][8 (l / 16)
 // + e] and B[8 (l / 16) + e][l % 16], e = 0..7, and receives C[4 (l / 16) + r][l % 16],
 // r = 0..3.
+#include "hnt_features.h"
 #include "hnt_internal.h"
 
 namespace {
@@ -232,6 +233,114 @@ __global__ void adam_step(float* __restrict__ p, const float* __restrict__ g,
     p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
 }
 
+// ------------------------------------------------------------------ fused inference
+// All four layers in one kernel: the BF16 weights (80 KB) are staged in LDS once per
+// workgroup; each wavefront then pushes tiles of 16 samples through the network, keeping
+// the 16 x 128 activations in its own LDS tile.  Inputs are normalised and rounded to BF16
+// on the fly (the A operand of layer 1 comes straight from registers).
+constexpr int FW = 4;  // wavefronts per workgroup
+
+// the wave's LDS writes are complete and visible to its other lanes before later reads
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(64 * FW) void mlp_forward_fused(
+    const float* __restrict__ x, int64_t B, const float* __restrict__ norm,
+    const __bf16* __restrict__ w1t, const __bf16* __restrict__ w2t,
+    const __bf16* __restrict__ w3t, const __bf16* __restrict__ w4t,
+    const float* __restrict__ P, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) __bf16 sw1[H * KI], sw2[H * H], sw3[H * H],
+        sw4[KO * H];
+    __shared__ __attribute__((aligned(16))) __bf16 act[FW][16 * H];
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    // stage the weights (16-byte chunks)
+    for (int q = tid; q < H * KI / 8; q += 64 * FW)
+        ((uint4*)sw1)[q] = ((const uint4*)w1t)[q];
+    for (int q = tid; q < H * H / 8; q += 64 * FW) {
+        ((uint4*)sw2)[q] = ((const uint4*)w2t)[q];
+        ((uint4*)sw3)[q] = ((const uint4*)w3t)[q];
+    }
+    for (int q = tid; q < KO * H / 8; q += 64 * FW)
+        ((uint4*)sw4)[q] = ((const uint4*)w4t)[q];
+    __syncthreads();
+    const float* b1 = P + OFF_B1;
+    const float* b2 = P + OFF_B2;
+    const float* b3 = P + OFF_B3;
+    const float* b4 = P + OFF_B4;
+    __bf16* A = act[wave];
+    const int rl = l & 15, kq = 8 * (l >> 4);
+    const int64_t ntiles = (B + 15) / 16;
+    for (int64_t t = (int64_t)blockIdx.x * FW + wave; t < ntiles; t += (int64_t)gridDim.x * FW) {
+        const int64_t row = t * 16 + rl;
+        // layer 1: A operand from x (k = kq..kq+7 of this lane's row)
+        bf16x8 a0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = kq + e;
+            float v = 0.0f;
+            if (row < B && k < NI) v = (x[row * NI + k] - norm[k]) / norm[NI + k];
+            a0[e] = to_bf16(v);
+        }
+#pragma unroll
+        for (int nt = 0; nt < H / 16; ++nt) {
+            const bf16x8 bv = *(const bf16x8*)&sw1[(nt * 16 + rl) * KI + kq];
+            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = 4 * (l >> 4) + r, n = nt * 16 + rl;
+                A[m * H + n] = to_bf16(silu(c[r] + b1[n]));
+            }
+        }
+        wave_lds_fence();
+        // layers 2 and 3: 128 -> 128
+#pragma unroll
+        for (int layer = 0; layer < 2; ++layer) {
+            const __bf16* W = layer == 0 ? sw2 : sw3;
+            const float* bb = layer == 0 ? b2 : b3;
+            bf16x8 av[H / 32];
+#pragma unroll
+            for (int ks = 0; ks < H / 32; ++ks) av[ks] = *(const bf16x8*)&A[rl * H + ks * 32 + kq];
+            f32x4 c[H / 16];
+#pragma unroll
+            for (int nt = 0; nt < H / 16; ++nt) {
+                c[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < H / 32; ++ks) {
+                    const bf16x8 bv = *(const bf16x8*)&W[(nt * 16 + rl) * H + ks * 32 + kq];
+                    c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[ks], bv, c[nt], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int nt = 0; nt < H / 16; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = 4 * (l >> 4) + r, n = nt * 16 + rl;
+                    A[m * H + n] = to_bf16(silu(c[nt][r] + bb[n]));
+                }
+            wave_lds_fence();
+        }
+        // layer 4: 128 -> 16 (3 used)
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < H / 32; ++ks) {
+            const bf16x8 av = *(const bf16x8*)&A[rl * H + ks * 32 + kq];
+            const bf16x8 bv = *(const bf16x8*)&sw4[rl * H + ks * 32 + kq];
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+        }
+        if (rl < NO) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t m = t * 16 + 4 * (l >> 4) + r;
+                if (m < B) out[m * NO + rl] = c[r] + b4[rl];
+            }
+        }
+        wave_lds_fence();
+    }
+}
+
 // ------------------------------------------------------------------ features
 // Per peak, from the cell records {delta, delta^T, psi1 xyz, psi2 xyz} of shells 1..12:
 //  f[0..11]  mean delta of shell s (in-bounds cells; 0 if none)
@@ -259,51 +368,21 @@ struct FeatArgs {
     float* out;
 };
 
-__device__ void ep_of(const double T[9], float* e, float* p) {
-    double lam[3];
-    eig3_sym(T[0], 0.5 * (T[1] + T[3]), 0.5 * (T[2] + T[6]), T[4], 0.5 * (T[5] + T[7]), T[8],
-             lam);
-    double F, ev, pv;
-    ev_from_lam(lam, &F, &ev, &pv);
-    *e = (float)ev;
-    *p = (float)pv;
-}
+
 
 __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= a.npeaks) return;
     const int ci = a.pi[q], cj = a.pj[q], ck = a.pk[q];
     const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
-    float* f = a.out + q * NI;
-    double T[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double tot = 0.0;
-    int64_t ntot = 0;
-    for (int s = 1; s <= NFEAT_SHELLS; ++s) {
-        const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
-        double F = 0.0;
-        int nin = 0;
-        for (int c = 0; c < nc; ++c) {
-            const int4 o = a.cells[st + c];
-            const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
-            if (i1 < 1 || i1 > a.n || i2 < 1 || i2 > a.n || i3 < 1 || i3 > a.n) continue;
-            const int64_t off = c0 + o.x + o.y * a.s1 + o.z * a.s2;
-            F += (double)a.delta[off];
-            ++nin;
-            const double e3[3] = {(double)a.ex[off], (double)a.ey[off], (double)a.ez[off]};
-            const double d3[3] = {(double)o.x, (double)o.y, (double)o.z};
-#pragma unroll
-            for (int L = 0; L < 3; ++L)
-#pragma unroll
-                for (int K = 0; K < 3; ++K) T[L * 3 + K] += e3[L] * d3[K];
-        }
-        f[s - 1] = nin > 0 ? (float)(F / (double)nin) : 0.0f;
-        tot += F;
-        ntot += nin;
-        if (s == 6) ep_of(T, &f[12], &f[13]);
-    }
-    ep_of(T, &f[14], &f[15]);
-    f[16] = ntot > 0 ? (float)(tot / (double)ntot) : 0.0f;
-    f[17] = (float)(a.Rf[q] / a.alatt);
+    auto load = [&](int64_t off, float& d, float& ex, float& ey, float& ez) {
+        d = a.delta[off];
+        ex = a.ex[off];
+        ey = a.ey[off];
+        ez = a.ez[off];
+    };
+    peak_features(a.cells, a.shell_start, a.shell_count, ci, cj, ck, c0, a.n, a.s1, a.s2,
+                  a.Rf[q] / a.alatt, load, a.out + q * NI);
 }
 
 }  // namespace
@@ -416,15 +495,23 @@ namespace hnt {
 
 // Network outputs for B rows of features, in chunks of 1M rows (bf16 activations only).
 hnt_status mlp_predict(hnt_ctx* ctx, const hnt_mlp* net, const float* x, int64_t B, float* out) {
-    const int64_t CH = 1 << 20;
-    for (int64_t b0 = 0; b0 < B; b0 += CH) {
-        const int64_t nb = B - b0 < CH ? B - b0 : CH;
-        Views v;
-        HNT_TRY(prepare(ctx, nb, v, false));
-        HNT_TRY(forward(ctx, net, x + b0 * NI, nb, v, false));
-        hipLaunchKernelGGL(bias_out, dim3((unsigned)((nb * NO + 255) / 256)), dim3(256), 0,
-                           ctx->stream, v.y, nb, net->params + OFF_B4, out + b0 * NO);
-    }
+    if (B <= 0) return HNT_OK;
+    MlpScratch& S = ctx->mlp;
+    const size_t wcount = (size_t)H * KI * 2 + (size_t)H * H * 4 + (size_t)KO * H * 2;
+    HNT_TRY(reserve(ctx, S.wt, wcount * 2));
+    __bf16* w = (__bf16*)S.wt.p;
+    __bf16 *w1t = w, *w2t = w1t + H * KI, *w3t = w2t + H * H, *w4t = w3t + H * H;
+    __bf16 *w1 = w4t + KO * H, *w2 = w1 + KI * H, *w3 = w2 + H * H, *w4 = w3 + H * H;
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(pack_weights, dim3((H * H + 255) / 256), dim3(256), 0, st, net->params,
+                       w1t, w2t, w3t, w4t, w1, w2, w3, w4);
+    const int64_t ntiles = (B + 15) / 16;
+    int64_t nblocks = (ntiles + FW - 1) / FW;
+    const int64_t cap = (int64_t)ctx->num_cus * 2;
+    if (nblocks > cap) nblocks = cap;
+    hipLaunchKernelGGL(mlp_forward_fused, dim3((unsigned)nblocks), dim3(64 * FW), 0, st, x, B,
+                       net->norm, w1t, w2t, w3t, w4t, net->params, out);
+    HNT_HIP(hipGetLastError());
     return HNT_OK;
 }
 

@@ -31,6 +31,7 @@
 //    empty.
 #include <hipcub/hipcub.hpp>
 
+#include "hnt_features.h"
 #include "hnt_internal.h"
 
 namespace {
@@ -1241,6 +1242,29 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, int64_t G8,
     stop_out[T.base + p] = stop;
 }
 
+// Bracket-network features of every peak in schedule order (same grouping and XCD mapping
+// as the crossing pass), read from the cell records; feats[(base + p) * 18 + j].
+__global__ __launch_bounds__(64) void features_sched_kernel(SAArgs a, int64_t G8,
+                                                            float* __restrict__ feats) {
+    const int64_t g = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
+    if (g >= a.ngroups) return;
+    const TileDev& T = a.tiles[a.gtile[g]];
+    const int p = a.slot[g * 64 + threadIdx.x];
+    if (p < 0) return;
+    const int ci = T.pi[p], cj = T.pj[p], ck = T.pk[p];
+    const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
+    const float4* __restrict__ recs = T.recs;
+    auto load = [&](int64_t off, float& d, float& ex, float& ey, float& ez) {
+        const float4 v0 = recs[2 * off];
+        d = v0.x;
+        ex = v0.z;
+        ey = v0.w;
+        ez = recs[2 * off + 1].x;
+    };
+    peak_features(a.cells, a.shell_start, a.shell_count, ci, cj, ck, c0, T.n, T.s1, T.s2,
+                  T.Rf[p] / a.alatt, load, feats + (T.base + p) * HNT_MLP_INPUTS);
+}
+
 // Predicted gather extent (network output 1 = log(1 + shells)), clamped to [1, m].
 __global__ void pred_stop_kernel(const float* __restrict__ pred, int64_t np, int m,
                                  int32_t* __restrict__ stop) {
@@ -1536,9 +1560,9 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
                             (size_t)ntot * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4));
             float* feats = (float*)ctx->feats.p;
             float* pred = feats + ntot * HNT_MLP_INPUTS;
-            for (int t = 0; t < ntiles; ++t)
-                HNT_TRY(bracket_features(ctx, &fields[t], &batches[t], params->alatt,
-                                         feats + meta[t] * HNT_MLP_INPUTS));
+            const int64_t G8 = (ngroups + 7) / 8;
+            hipLaunchKernelGGL(features_sched_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st,
+                               A, G8, feats);
             HNT_TRY(mlp_predict(ctx, params->bracket, feats, ntot, pred));
             hipLaunchKernelGGL(pred_stop_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256),
                                0, st, pred, ntot, A.NS - 1, stops);
