@@ -38,10 +38,17 @@ hnt_status reserve(hnt_ctx* ctx, DevBuf& b, size_t bytes) {
     return HNT_OK;
 }
 
-static void release(DevBuf& b) {
+void release(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+}
+
+void release_work(SAWork& w) {
+    DevBuf* bufs[] = {&w.recs, &w.keys_a, &w.keys_b, &w.vals_a, &w.vals_b, &w.cub_tmp, &w.slots,
+                      &w.gtiles, &w.tiles, &w.tilemeta, &w.stops, &w.feats, &w.prof, &w.queue,
+                      &w.flag};
+    for (DevBuf* b : bufs) release(*b);
 }
 
 hnt_status begin_call(hnt_ctx* ctx) {
@@ -290,10 +297,9 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
     if (!ctx) return HNT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->recs, &ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp,
-                      &ctx->keys_a, &ctx->keys_b, &ctx->vals_a, &ctx->vals_b, &ctx->prof,
-                      &ctx->queue, &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small, &ctx->stops, &ctx->feats,
-                      &ctx->tiles, &ctx->tilemeta, &ctx->slots, &ctx->gtiles,
+    release_work(ctx->sa);
+    DevBuf* bufs[] = {&ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp,
+                      &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small,
                       &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
                       &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
                       &ctx->tables.kw_idx, &ctx->ctab.table, &ctx->mlp.wt, &ctx->mlp.x0,

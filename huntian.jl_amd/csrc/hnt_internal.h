@@ -58,6 +58,15 @@ struct TileDev {
     int64_t base;  // global id of the tile's first peak
 };
 
+// Device buffers of one shell analysis (the context holds one for direct calls, each plan
+// its own): cell records, sort keys, lane slots, tile descriptors, profile slots, queues.
+struct SAWork {
+    DevBuf recs, keys_a, keys_b, vals_a, vals_b, cub_tmp, slots, gtiles, tiles, tilemeta, stops,
+        feats, prof, queue, flag;
+    std::vector<TileDev> tile_host;
+    std::vector<int64_t> meta_host;
+};
+
 // scratch of the bracket MLP (bf16 weights and activations, gradients)
 struct MlpScratch {
     DevBuf wt, x0, z, h, y, dy, dyf, dz, gw, part, gradv;
@@ -78,8 +87,8 @@ struct hnt_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     // buffers reused across calls (grow only)
-    hnt::DevBuf recs, claim, counts, offsets, cub_tmp, keys_a, keys_b, vals_a, vals_b, prof,
-        queue, fieldptrs, fcrit, akk, small, stops, feats;
+    hnt::DevBuf claim, counts, offsets, cub_tmp, fieldptrs, fcrit, akk, small;
+    hnt::SAWork sa;  // shell analysis buffers of direct calls
     hnt::TablesDev tables;
     hnt::CTabDev ctab;
     hnt::MlpScratch mlp;
@@ -100,6 +109,8 @@ struct hnt_ctx {
 namespace hnt {
 
 hnt_status reserve(hnt_ctx* ctx, DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+void release_work(SAWork& w);
 hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t);
 hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);

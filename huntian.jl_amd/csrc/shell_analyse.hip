@@ -1331,18 +1331,52 @@ hnt_status check_tile(const hnt_fields* f, const hnt_batch* b, const hnt_results
     return HNT_OK;
 }
 
-void sort_pairs(hnt_ctx* ctx, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipError_t* e) {
-    *e = hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp.p, tmp_bytes, (uint64_t*)ctx->keys_a.p,
-                                            (uint64_t*)ctx->keys_b.p, (int32_t*)ctx->vals_a.p,
-                                            (int32_t*)ctx->vals_b.p, (int)ntot, 0, 64, st);
+void sort_pairs(SAWork& W, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipError_t* e) {
+    *e = hipcub::DeviceRadixSort::SortPairs(W.cub_tmp.p, tmp_bytes, (uint64_t*)W.keys_a.p,
+                                            (uint64_t*)W.keys_b.p, (int32_t*)W.vals_a.p,
+                                            (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st);
+}
+
+// What a replay of an analysis launches: the record repacks and the main kernel.
+struct RepackJob {
+    const float *d, *ex, *ey, *ez, *e2x, *e2y, *e2z;
+    int n;
+    float4* recs;
+};
+struct LaunchDesc {
+    SAArgs A;
+    int64_t nblocks = 0;
+    bool gang = false, kahan = false;
+    std::vector<RepackJob> repacks;
+};
+
+void launch_repacks(hipStream_t st, const std::vector<RepackJob>& jobs) {
+    for (const RepackJob& j : jobs) {
+        dim3 g((j.n + 63) / 64, (j.n + 63) / 64, j.n);
+        hipLaunchKernelGGL(repack_kernel, g, dim3(256), 0, st, j.d, j.ex, j.ey, j.ez, j.e2x,
+                           j.e2y, j.e2z, j.n, j.recs);
+    }
+}
+
+void launch_main(hipStream_t st, const LaunchDesc& d) {
+    const dim3 grid((unsigned)d.nblocks), blk(64);
+    if (d.gang && d.kahan)
+        hipLaunchKernelGGL(shell_gang_kernel<SumKahan>, grid, blk, 0, st, d.A);
+    else if (d.gang)
+        hipLaunchKernelGGL(shell_gang_kernel<SumF64>, grid, blk, 0, st, d.A);
+    else if (d.kahan)
+        hipLaunchKernelGGL(shell_analyse_kernel<SumKahan>, grid, blk, 0, st, d.A);
+    else
+        hipLaunchKernelGGL(shell_analyse_kernel<SumF64>, grid, blk, 0, st, d.A);
 }
 
 // All tiles of a call in one persistent launch: peaks of every tile share the eight queues;
-// a wavefront's 64 lanes belong to one tile.
-hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
+// a wavefront's 64 lanes belong to one tile.  W holds the call's device buffers; desc (if
+// not NULL) receives what a replay launches.
+hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* fields,
                          const hnt_batch* batches, const hnt_shell_tables* tables,
                          const hnt_ctab* ctab, const hnt_params* params, hnt_results* results,
-                         int8_t* const* masks) {
+                         int8_t* const* masks, LaunchDesc* desc) {
     HNT_REQUIRE(ntiles >= 1 && ntiles <= 8192, "hnt_shell_analyse: %d tiles", ntiles);
     HNT_REQUIRE(fields && batches && tables && ctab && params && results,
                 "hnt_shell_analyse: NULL argument");
@@ -1381,39 +1415,38 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
 
     // ---- validate the peak cells and ZZon on the device before any gather ----
     stage_begin(ctx, HNT_STAGE_SA_PREP);
-    HNT_TRY(reserve(ctx, ctx->small, 64));
-    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ctx->small.p, 1,
+    HNT_TRY(reserve(ctx, W.flag, 64));
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)W.flag.p, 1,
                        0ull);
     for (int t = 0; t < ntiles; ++t) {
         const hnt_batch& b = batches[t];
         if (b.npeaks == 0) continue;
         hipLaunchKernelGGL(validate_kernel, dim3((unsigned)((b.npeaks + 255) / 256)), dim3(256),
                            0, st, b.pi, b.pj, b.pk, b.ZZon, b.npeaks, fields[t].n1,
-                           (unsigned int*)ctx->small.p);
+                           (unsigned int*)W.flag.p);
     }
     {
         unsigned long long bad = 0;
-        HNT_HIP(hipMemcpyAsync(&bad, ctx->small.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+        HNT_HIP(hipMemcpyAsync(&bad, W.flag.p, sizeof(bad), hipMemcpyDeviceToHost, st));
         HNT_HIP(hipStreamSynchronize(st));
         HNT_REQUIRE(!(bad & 1ull), "hnt_shell_analyse: a peak lies outside its grid");
         HNT_REQUIRE(!(bad & 2ull), "hnt_shell_analyse: per-peak ZZon must be > -1");
     }
 
     // ---- repack every tile's fields into 32-byte cell records ----
-    HNT_TRY(reserve(ctx, ctx->recs, (size_t)ncell * 32));
-    std::vector<TileDev>& TD = ctx->tile_host;
+    HNT_TRY(reserve(ctx, W.recs, (size_t)ncell * 32));
+    std::vector<TileDev>& TD = W.tile_host;
     TD.assign((size_t)ntiles, TileDev{});
+    std::vector<RepackJob> jobs;
     {
         int64_t off = 0;
         for (int t = 0; t < ntiles; ++t) {
             const hnt_fields& f = fields[t];
             const int n = f.n1;
-            float4* recs = (float4*)ctx->recs.p + 2 * off;
-            if (batches[t].npeaks > 0) {
-                dim3 g((n + 63) / 64, (n + 63) / 64, n);
-                hipLaunchKernelGGL(repack_kernel, g, dim3(256), 0, st, f.delta, f.etax, f.etay,
-                                   f.etaz, f.eta2x, f.eta2y, f.eta2z, n, recs);
-            }
+            float4* recs = (float4*)W.recs.p + 2 * off;
+            if (batches[t].npeaks > 0)
+                jobs.push_back(RepackJob{f.delta, f.etax, f.etay, f.etaz, f.eta2x, f.eta2y,
+                                         f.eta2z, n, recs});
             TileDev& T = TD[t];
             T.recs = recs;
             T.delta = f.delta;
@@ -1440,29 +1473,30 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
             off += (int64_t)n * n * n;
         }
     }
-    HNT_TRY(reserve(ctx, ctx->tiles, TD.size() * sizeof(TileDev)));
-    HNT_TRY(reserve(ctx, ctx->tilemeta, meta.size() * sizeof(int64_t)));
-    ctx->meta_host = meta;
-    HNT_HIP(hipMemcpyAsync(ctx->tiles.p, TD.data(), TD.size() * sizeof(TileDev),
+    launch_repacks(st, jobs);
+    HNT_TRY(reserve(ctx, W.tiles, TD.size() * sizeof(TileDev)));
+    HNT_TRY(reserve(ctx, W.tilemeta, meta.size() * sizeof(int64_t)));
+    W.meta_host = meta;
+    HNT_HIP(hipMemcpyAsync(W.tiles.p, TD.data(), TD.size() * sizeof(TileDev),
                            hipMemcpyHostToDevice, st));
-    HNT_HIP(hipMemcpyAsync(ctx->tilemeta.p, ctx->meta_host.data(), meta.size() * sizeof(int64_t),
+    HNT_HIP(hipMemcpyAsync(W.tilemeta.p, W.meta_host.data(), meta.size() * sizeof(int64_t),
                            hipMemcpyHostToDevice, st));
-    const int64_t* tile_base = (const int64_t*)ctx->tilemeta.p;
+    const int64_t* tile_base = (const int64_t*)W.tilemeta.p;
     const int64_t* tile_pad = tile_base + ntiles;
 
     // ---- first order: (tile, ir2min, Morton) ----
-    HNT_TRY(reserve(ctx, ctx->keys_a, (size_t)ntot * 8));
-    HNT_TRY(reserve(ctx, ctx->keys_b, (size_t)ntot * 8));
-    HNT_TRY(reserve(ctx, ctx->vals_a, (size_t)ntot * 4));
-    HNT_TRY(reserve(ctx, ctx->vals_b, (size_t)ntot * 4));
-    HNT_TRY(reserve(ctx, ctx->slots, (size_t)ngroups * 64 * 4));
-    HNT_TRY(reserve(ctx, ctx->gtiles, (size_t)ngroups * 4));
-    int32_t* slot = (int32_t*)ctx->slots.p;
+    HNT_TRY(reserve(ctx, W.keys_a, (size_t)ntot * 8));
+    HNT_TRY(reserve(ctx, W.keys_b, (size_t)ntot * 8));
+    HNT_TRY(reserve(ctx, W.vals_a, (size_t)ntot * 4));
+    HNT_TRY(reserve(ctx, W.vals_b, (size_t)ntot * 4));
+    HNT_TRY(reserve(ctx, W.slots, (size_t)ngroups * 64 * 4));
+    HNT_TRY(reserve(ctx, W.gtiles, (size_t)ngroups * 4));
+    int32_t* slot = (int32_t*)W.slots.p;
     size_t tmp_bytes = 0;
-    HNT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint64_t*)ctx->keys_a.p,
-                                               (uint64_t*)ctx->keys_b.p, (int32_t*)ctx->vals_a.p,
-                                               (int32_t*)ctx->vals_b.p, (int)ntot, 0, 64, st));
-    HNT_TRY(reserve(ctx, ctx->cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
+    HNT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint64_t*)W.keys_a.p,
+                                               (uint64_t*)W.keys_b.p, (int32_t*)W.vals_a.p,
+                                               (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st));
+    HNT_TRY(reserve(ctx, W.cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
     auto order_by = [&](const int32_t* const* prim) -> hnt_status {
         for (int t = 0; t < ntiles; ++t) {
             const hnt_batch& b = batches[t];
@@ -1470,15 +1504,15 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
             hipLaunchKernelGGL(tile_key_kernel, dim3((unsigned)((b.npeaks + 255) / 256)),
                                dim3(256), 0, st, b.pi, b.pj, b.pk, prim[t], b.npeaks,
                                (uint64_t)t, meta[t], ctx->brick_log2,
-                               (uint64_t*)ctx->keys_a.p, (int32_t*)ctx->vals_a.p);
+                               (uint64_t*)W.keys_a.p, (int32_t*)W.vals_a.p);
         }
         hipError_t e;
-        sort_pairs(ctx, tmp_bytes, ntot, st, &e);
+        sort_pairs(W, tmp_bytes, ntot, st, &e);
         HNT_HIP(e);
         hipLaunchKernelGGL(fill_i32, dim3((unsigned)((ngroups * 64 + 255) / 256)), dim3(256), 0,
                            st, slot, ngroups * 64, -1);
         hipLaunchKernelGGL(slot_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st,
-                           (const uint64_t*)ctx->keys_b.p, (const int32_t*)ctx->vals_b.p, ntot,
+                           (const uint64_t*)W.keys_b.p, (const int32_t*)W.vals_b.p, ntot,
                            tile_base, tile_pad, slot);
         return HNT_OK;
     };
@@ -1486,14 +1520,14 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
     for (int t = 0; t < ntiles; ++t) prim[t] = batches[t].ir2min;
     HNT_TRY(order_by(prim.data()));
     hipLaunchKernelGGL(group_tile_kernel, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0,
-                       st, tile_pad, ntiles, ngroups, (int32_t*)ctx->gtiles.p);
+                       st, tile_pad, ntiles, ngroups, (int32_t*)W.gtiles.p);
 
     // ---- persistent grid and profile scratch ----
     SAArgs A;
     memset(&A, 0, sizeof(A));
-    A.tiles = (const TileDev*)ctx->tiles.p;
+    A.tiles = (const TileDev*)W.tiles.p;
     A.slot = slot;
-    A.gtile = (const int32_t*)ctx->gtiles.p;
+    A.gtile = (const int32_t*)W.gtiles.p;
     A.ngroups = ngroups;
     A.cells = (const int4*)ctx->tables.cells.p;
     A.shell_start = (const int32_t*)ctx->tables.start.p;
@@ -1535,10 +1569,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
     if (nblocks > want) nblocks = want;
     nblocks = (nblocks / 8) * 8;
     if (nblocks < 8) nblocks = 8;
-    HNT_TRY(reserve(ctx, ctx->prof, (size_t)nblocks * 64 * slot_bytes));
-    A.prof = (double*)ctx->prof.p;
-    HNT_TRY(reserve(ctx, ctx->queue, 8 * sizeof(unsigned long long)));
-    A.qctr = (unsigned long long*)ctx->queue.p;
+    HNT_TRY(reserve(ctx, W.prof, (size_t)nblocks * 64 * slot_bytes));
+    A.prof = (double*)W.prof.p;
+    HNT_TRY(reserve(ctx, W.queue, 8 * sizeof(unsigned long long)));
+    A.qctr = (unsigned long long*)W.queue.p;
     for (int x = 0; x < 8; ++x) {
         A.qbeg[x] = ngroups * x / 8;
         A.qend[x] = ngroups * (x + 1) / 8;
@@ -1549,16 +1583,16 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
     // ---- schedule: order the peaks by their (exact or predicted) gather extent ----
     if (A.early && params->order != HNT_ORDER_PLAIN) {
         stage_begin(ctx, HNT_STAGE_SA_CROSS);
-        HNT_TRY(reserve(ctx, ctx->stops, (size_t)ntot * 4));
-        int32_t* stops = (int32_t*)ctx->stops.p;
+        HNT_TRY(reserve(ctx, W.stops, (size_t)ntot * 4));
+        int32_t* stops = (int32_t*)W.stops.p;
         if (params->order == HNT_ORDER_CROSSING) {
             const int64_t G8 = (ngroups + 7) / 8;
             hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A, G8,
                                stops);
         } else {
-            HNT_TRY(reserve(ctx, ctx->feats,
+            HNT_TRY(reserve(ctx, W.feats,
                             (size_t)ntot * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4));
-            float* feats = (float*)ctx->feats.p;
+            float* feats = (float*)W.feats.p;
             float* pred = feats + ntot * HNT_MLP_INPUTS;
             const int64_t G8 = (ngroups + 7) / 8;
             hipLaunchKernelGGL(features_sched_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st,
@@ -1573,17 +1607,14 @@ hnt_status analyse_tiles(hnt_ctx* ctx, int ntiles, const hnt_fields* fields,
     }
 
     stage_begin(ctx, HNT_STAGE_SA_KERNEL);
-    if (gang && kahan)
-        hipLaunchKernelGGL(shell_gang_kernel<SumKahan>, dim3((unsigned)nblocks), dim3(64), 0, st,
-                           A);
-    else if (gang)
-        hipLaunchKernelGGL(shell_gang_kernel<SumF64>, dim3((unsigned)nblocks), dim3(64), 0, st, A);
-    else if (kahan)
-        hipLaunchKernelGGL(shell_analyse_kernel<SumKahan>, dim3((unsigned)nblocks), dim3(64), 0,
-                           st, A);
-    else
-        hipLaunchKernelGGL(shell_analyse_kernel<SumF64>, dim3((unsigned)nblocks), dim3(64), 0, st,
-                           A);
+    LaunchDesc local;
+    LaunchDesc& D = desc ? *desc : local;
+    D.A = A;
+    D.nblocks = nblocks;
+    D.gang = gang;
+    D.kahan = kahan;
+    D.repacks = std::move(jobs);
+    launch_main(st, D);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);
     return HNT_OK;
 }
@@ -1595,7 +1626,8 @@ extern "C" hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields,
                                         const hnt_ctab* ctab, const hnt_params* params,
                                         hnt_results* results, int8_t* mask) {
     HNT_TRY(begin_call(ctx));
-    HNT_TRY(analyse_tiles(ctx, 1, fields, batch, tables, ctab, params, results, &mask));
+    HNT_TRY(analyse_tiles(ctx, ctx->sa, 1, fields, batch, tables, ctab, params, results, &mask,
+                          nullptr));
     return finish_call(ctx, "hnt_shell_analyse");
 }
 
@@ -1606,6 +1638,91 @@ extern "C" hnt_status hnt_shell_analyse_tiles(hnt_ctx* ctx, int32_t ntiles,
                                               const hnt_ctab* ctab, const hnt_params* params,
                                               hnt_results* results, int8_t* const* masks) {
     HNT_TRY(begin_call(ctx));
-    HNT_TRY(analyse_tiles(ctx, ntiles, fields, batches, tables, ctab, params, results, masks));
+    HNT_TRY(analyse_tiles(ctx, ctx->sa, ntiles, fields, batches, tables, ctab, params, results,
+                          masks, nullptr));
     return finish_call(ctx, "hnt_shell_analyse_tiles");
+}
+
+// ------------------------------------------------------------------ plans (hipGraph replay)
+
+struct hnt_plan {
+    hnt_ctx* ctx = nullptr;
+    SAWork W;
+    LaunchDesc D;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void plan_free(hnt_plan* P) {
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->cap) (void)hipStreamDestroy(P->cap);
+    release_work(P->W);
+    delete P;
+}
+
+static hnt_status plan_build(hnt_plan* P, int32_t ntiles, const hnt_fields* fields,
+                             const hnt_batch* batches, const hnt_shell_tables* tables,
+                             const hnt_ctab* ctab, const hnt_params* params,
+                             hnt_results* results, int8_t* const* masks) {
+    hnt_ctx* ctx = P->ctx;
+    HNT_TRY(analyse_tiles(ctx, P->W, ntiles, fields, batches, tables, ctab, params, results,
+                          masks, &P->D));
+    HNT_HIP(hipStreamSynchronize(ctx->stream));
+    if (P->D.nblocks == 0) return HNT_OK;  // no peaks: launching the plan does nothing
+    HNT_HIP(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
+    HNT_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, P->cap, P->D.A.qctr, 8, 0ull);
+    launch_repacks(P->cap, P->D.repacks);
+    launch_main(P->cap, P->D);
+    HNT_HIP(hipStreamEndCapture(P->cap, &P->graph));
+    // the graph must hold kernels only (no memset or copy nodes)
+    size_t nn = 0;
+    HNT_HIP(hipGraphGetNodes(P->graph, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    HNT_HIP(hipGraphGetNodes(P->graph, nodes.data(), &nn));
+    for (hipGraphNode_t n : nodes) {
+        hipGraphNodeType ty;
+        HNT_HIP(hipGraphNodeGetType(n, &ty));
+        HNT_REQUIRE(ty == hipGraphNodeTypeKernel, "hnt_plan_create: graph holds a non-kernel node");
+    }
+    HNT_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    return HNT_OK;
+}
+
+extern "C" hnt_status hnt_plan_create(hnt_ctx* ctx, int32_t ntiles, const hnt_fields* fields,
+                                      const hnt_batch* batches, const hnt_shell_tables* tables,
+                                      const hnt_ctab* ctab, const hnt_params* params,
+                                      hnt_results* results, int8_t* const* masks,
+                                      hnt_plan** out) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(out, "hnt_plan_create: NULL output");
+    *out = nullptr;
+    hnt_plan* P = new hnt_plan;
+    P->ctx = ctx;
+    const hnt_status st = plan_build(P, ntiles, fields, batches, tables, ctab, params, results,
+                                     masks);
+    if (st != HNT_OK) {
+        plan_free(P);
+        return st;
+    }
+    *out = P;
+    return finish_call(ctx, "hnt_plan_create");
+}
+
+extern "C" hnt_status hnt_plan_launch(hnt_plan* plan) {
+    HNT_REQUIRE(plan, "hnt_plan_launch: NULL plan");
+    HNT_TRY(begin_call(plan->ctx));
+    if (plan->exec) HNT_HIP(hipGraphLaunch(plan->exec, plan->ctx->stream));
+    return finish_call(plan->ctx, "hnt_plan_launch");
+}
+
+extern "C" hnt_status hnt_plan_destroy(hnt_plan* plan) {
+    clear_error();
+    if (!plan) return HNT_OK;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    plan_free(plan);
+    return HNT_OK;
 }

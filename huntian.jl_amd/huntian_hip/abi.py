@@ -187,6 +187,9 @@ EXPORTED = (
     "hnt_peak_find",
     "hnt_shell_analyse",
     "hnt_shell_analyse_tiles",
+    "hnt_plan_create",
+    "hnt_plan_launch",
+    "hnt_plan_destroy",
     "hnt_shell_gather_full",
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
@@ -253,6 +256,12 @@ def load(path: str | None = None):
                                             C.POINTER(ShellTables), C.POINTER(CTab),
                                             C.POINTER(Params), C.POINTER(Results),
                                             C.POINTER(C.POINTER(C.c_int8))]
+    lib.hnt_plan_create.argtypes = [vp, C.c_int32, C.POINTER(Fields), C.POINTER(Batch),
+                                    C.POINTER(ShellTables), C.POINTER(CTab), C.POINTER(Params),
+                                    C.POINTER(Results), C.POINTER(C.POINTER(C.c_int8)),
+                                    C.POINTER(vp)]
+    lib.hnt_plan_launch.argtypes = [vp]
+    lib.hnt_plan_destroy.argtypes = [vp]
     lib.hnt_bracket_features.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                          C.POINTER(ShellTables), C.c_double, _f32p]
     lib.hnt_mlp_forward.argtypes = [vp, C.POINTER(Mlp), _f32p, C.c_int64, _f32p]

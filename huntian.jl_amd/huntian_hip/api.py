@@ -409,16 +409,7 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     return r
 
 
-def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
-                        fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
-                        order: str = "crossing", bracket=None, precision: str = "fp64"):
-    """hnt_shell_analyse_tiles: every tile in one persistent launch.
-
-    tiles: list of dicts with keys fields, pi, pj, pk, Rf, ir2min and optionally mask,
-    ZZon_pp, fcrit_pp.  Returns one result dict (as shell_analyse) per tile."""
-    torch = _require_gpu()
-    ctx = context()
-    dev = torch.device("cuda", ctx.device)
+def _tiles_structs(tiles, ZZon, fcrit, dev):
     nt = len(tiles)
     FA = (abi.Fields * nt)()
     BA = (abi.Batch * nt)()
@@ -434,14 +425,70 @@ def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, Z
         MA[q] = abi.as_ptr(_ptr(t.get("mask")), C.c_int8)
         keep += [kf, kb]
         out.append(r)
+    return FA, BA, RA, MA, keep, out
+
+
+def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
+                        fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
+                        order: str = "crossing", bracket=None, precision: str = "fp64"):
+    """hnt_shell_analyse_tiles: every tile in one persistent launch.
+
+    tiles: list of dicts with keys fields, pi, pj, pk, Rf, ir2min and optionally mask,
+    ZZon_pp, fcrit_pp.  Returns one result dict (as shell_analyse) per tile."""
+    torch = _require_gpu()
+    ctx = context()
+    dev = torch.device("cuda", ctx.device)
+    FA, BA, RA, MA, keep, out = _tiles_structs(tiles, ZZon, fcrit, dev)
     prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
     cs, keep_t = _ctab_struct(ct)
     ts = tables.struct()
-    st = ctx.lib.hnt_shell_analyse_tiles(ctx.handle, nt, FA, BA, C.byref(ts), C.byref(cs),
-                                         C.byref(prm), RA, MA)
+    st = ctx.lib.hnt_shell_analyse_tiles(ctx.handle, len(tiles), FA, BA, C.byref(ts),
+                                         C.byref(cs), C.byref(prm), RA, MA)
     abi.check(ctx.lib, st)
     del keep, keep_t
     return [{k: v[:len(t["pi"])] for k, v in r.items()} for r, t in zip(out, tiles)]
+
+
+class ShellPlan:
+    """hnt_plan_create / hnt_plan_launch: the tiles' analysis recorded once as a hipGraph.
+
+    Construction runs the analysis (results are valid afterwards); launch() replays it over
+    the current contents of the same field and peak arrays.  `results` holds one result
+    dict per tile; `inputs` the device arrays the plan reads (update them in place)."""
+
+    def __init__(self, tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
+                 fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
+                 order: str = "crossing", bracket=None, precision: str = "fp64"):
+        torch = _require_gpu()
+        self.ctx = context()
+        dev = torch.device("cuda", self.ctx.device)
+        FA, BA, RA, MA, keep, out = _tiles_structs(tiles, ZZon, fcrit, dev)
+        prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
+        cs, keep_t = _ctab_struct(ct)
+        ts = tables.struct()
+        h = C.c_void_p()
+        abi.check(self.ctx.lib, self.ctx.lib.hnt_plan_create(
+            self.ctx.handle, len(tiles), FA, BA, C.byref(ts), C.byref(cs), C.byref(prm), RA, MA,
+            C.byref(h)))
+        self.handle = h
+        self._keep = (FA, BA, RA, MA, keep, keep_t, out, tables, bracket, tiles)
+        self.inputs = keep
+        self.results = [{k: v[:len(t["pi"])] for k, v in r.items()} for r, t in zip(out, tiles)]
+
+    def launch(self):
+        context(self.ctx.device)  # run on torch's current stream
+        abi.check(self.ctx.lib, self.ctx.lib.hnt_plan_launch(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.hnt_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _host_results(r, npk):

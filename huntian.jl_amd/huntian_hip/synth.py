@@ -193,7 +193,7 @@ def gaussian_field(n, boxsize, seed, pk: PowerSpectrum | None = None):
         amp = np.sqrt(pk.file(np.where(k2 > 0, k, 1.0)) * dk ** 3 * float(n) ** 3)
     amp = np.where(k2 > 0, amp, 0.0)
     fk = fk * amp
-    d = np.fft.irfftn(fk, s=b.shape).astype(np.float32)
+    d = np.fft.irfftn(fk, s=b.shape, axes=(0, 1, 2)).astype(np.float32)
     dk_ = np.fft.rfftn(d.astype(np.float64))
     return np.asfortranarray(d.T), dk_
 
@@ -212,27 +212,27 @@ def lpt_fields(delta_k, n, boxsize, second_order=True):
     for name, kc in (("etax", KX), ("etay", KY), ("etaz", KZ)):
         pk_ = 1j * kc * inv * delta_k
         pk_ = np.where(zero_nyq | (k2 == 0), 0.0, pk_)
-        out[name] = np.asfortranarray(np.fft.irfftn(pk_, s=(n, n, n)).astype(np.float32).T)
+        out[name] = np.asfortranarray(np.fft.irfftn(pk_, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32).T)
     if second_order:
         comps = {}
         for a, ka in (("x", KX), ("y", KY), ("z", KZ)):
             for b_, kb in (("x", KX), ("y", KY), ("z", KZ)):
                 if a + b_ in ("xx", "yy", "zz", "xy", "xz", "yz"):
-                    comps[a + b_] = np.fft.irfftn(-ka * kb * inv * delta_k, s=(n, n, n))
+                    comps[a + b_] = np.fft.irfftn(-ka * kb * inv * delta_k, s=(n, n, n), axes=(0, 1, 2))
         src = (comps["xx"] * comps["yy"] - comps["xy"] ** 2 + comps["xx"] * comps["zz"]
                - comps["xz"] ** 2 + comps["yy"] * comps["zz"] - comps["yz"] ** 2)
         sk = np.fft.rfftn(src)
         for name, kc in (("eta2x", KX), ("eta2y", KY), ("eta2z", KZ)):
             pk_ = -1j * kc * inv * sk
             pk_ = np.where(zero_nyq | (k2 == 0), 0.0, pk_)
-            out[name] = np.asfortranarray(np.fft.irfftn(pk_, s=(n, n, n)).astype(np.float32).T)
+            out[name] = np.asfortranarray(np.fft.irfftn(pk_, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32).T)
     return out
 
 
 def laplacian_field(delta_k, n, boxsize):
     KZ, KY, KX, _ = _kgrid(n, boxsize)
     k2 = KX ** 2 + KY ** 2 + KZ ** 2
-    return np.asfortranarray(np.fft.irfftn(-k2 * delta_k, s=(n, n, n)).astype(np.float32).T)
+    return np.asfortranarray(np.fft.irfftn(-k2 * delta_k, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32).T)
 
 
 def smooth_tophat(delta_k, n, boxsize, R):
@@ -241,7 +241,7 @@ def smooth_tophat(delta_k, n, boxsize, R):
     k = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
     w = tophat_w(k * R).astype(np.float32).astype(np.float64)
     w = np.where(k == 0, 1.0, w)
-    return np.asfortranarray(np.fft.irfftn(delta_k * w, s=(n, n, n)).astype(np.float32).T)
+    return np.asfortranarray(np.fft.irfftn(delta_k * w, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32).T)
 
 
 def filter_radii(rmin, rmax, nfilt):

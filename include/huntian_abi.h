@@ -261,6 +261,24 @@ hnt_status hnt_shell_analyse_tiles(hnt_ctx* ctx, int32_t ntiles, const hnt_field
                                    const hnt_ctab* ctab, const hnt_params* params,
                                    hnt_results* results, int8_t* const* masks);
 
+/*
+ * Plans: hnt_shell_analyse_tiles recorded once and replayed as one hipGraph (no counterpart
+ * in the reference).  hnt_plan_create runs the analysis eagerly (results are written) and
+ * records its device work -- the record repacks and the persistent analysis kernel -- in a
+ * graph of kernel nodes only.  hnt_plan_launch replays it on the context's stream: it
+ * re-reads the fields, peak arrays, ZZon and fcrit at the same addresses (contents may
+ * change between launches, sizes and pointers may not) and rewrites results and masks.  The
+ * peak schedule of the creating call is kept, so a replay over changed fields is exact but
+ * may be slower.  A plan owns its device buffers; it must not outlive its context.
+ */
+typedef struct hnt_plan hnt_plan;
+hnt_status hnt_plan_create(hnt_ctx* ctx, int32_t ntiles, const hnt_fields* fields,
+                           const hnt_batch* batches, const hnt_shell_tables* tables,
+                           const hnt_ctab* ctab, const hnt_params* params, hnt_results* results,
+                           int8_t* const* masks, hnt_plan** out);
+hnt_status hnt_plan_launch(hnt_plan* plan);
+hnt_status hnt_plan_destroy(hnt_plan* plan);
+
 /* ---------------------------------------------------------------- primitive entries */
 /* Single-primitive entry points; the reference exposes them as test harnesses
  * (src/Huntian.jl:98-217). */

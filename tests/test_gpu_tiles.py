@@ -61,3 +61,49 @@ def test_tiles_equal_one_call_per_tile(oracle, websky, order):
     assert np.array_equal(g0["RTHL"] > 0, ref["RTHL"] > 0)
     assert np.array_equal(g0["nshells_used"], ref["nshells_used"])
     np.testing.assert_allclose(g0["RTHL"], ref["RTHL"], rtol=1e-10)
+
+
+def test_plan_replays_match_eager_calls(oracle, websky):
+    """A plan (hipGraph of the repacks and the persistent kernel) replayed several times,
+    with other GPU work in between, reproduces the eager results; after the fields change in
+    place, a replay equals a fresh eager call on the changed fields."""
+    import torch
+
+    nbuff, nhunt = 11, 10
+    tiles = [_tile(n, s, nbuff, oracle) for n, s in [(32, 201), (40, 202), (24, 203)]]
+    stab = api.build_shell_tables(nhunt)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    kw = dict(alatt=2.0, ZZon=1.0, fcrit=fcrit, nbuff=nbuff)
+    plan = api.ShellPlan(tiles, stab, websky, **kw)
+    eager = api.shell_analyse_tiles(tiles, stab, websky, **kw)
+
+    def same(a, b):
+        for k in a:
+            assert torch.equal(torch.nan_to_num(a[k], nan=7.0), torch.nan_to_num(b[k], nan=7.0)), k
+
+    for a, b in zip(plan.results, eager):
+        same(a, b)
+    for rep in range(3):
+        for r in plan.results:
+            r["RTHL"].fill_(-5.0)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        for a, b in zip(plan.results, eager):
+            same(a, b)
+    # change the fields in place (the plan's own device copies) and in a fresh set of inputs
+    changed = []
+    for t, (kf, kb) in zip(tiles, zip(plan.inputs[0::2], plan.inputs[1::2])):
+        kf["delta"].mul_(1.07)
+        kf["etax"].mul_(0.9)
+        f2 = {k: v.clone() for k, v in kf.items()}
+        changed.append(dict(t, fields=f2))
+    plan.launch()
+    torch.cuda.synchronize()
+    fresh = api.shell_analyse_tiles(changed, stab, websky, **kw)
+    for a, b in zip(plan.results, fresh):
+        same(a, b)
+    assert not torch.equal(plan.results[0]["RTHL"], eager[0]["RTHL"])
+    plan.close()
