@@ -36,6 +36,9 @@ CONFIGS = {
                     nhunt=20),
     "tile128": dict(n=128, box=256.0, seed=12345, rmin=1.5, rmax=20.0, nfilt=16, nbuff=16,
                     nhunt=15),
+    # 64 tiles, sides drawn from {96, 128, 192, 256}, buffers 16..48, seeds 100..163
+    "batched64": dict(ntiles=64, sides=(96, 128, 192, 256), nbuff=(16, 48), seeds=(100, 163),
+                      rmin=1.5, rmax=20.0, nfilt=16, nhunt=15, alatt=2.0),
 }
 
 
@@ -52,8 +55,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0 = skip)")
     ap.add_argument("--variant", default="C", choices=["A", "B", "C"])
-    ap.add_argument("--order", default="crossing", choices=["crossing", "plain", "learned"],
-                    help="wavefront schedule of the shell analysis (never changes a result)")
+    ap.add_argument("--order", default=None, choices=["crossing", "plain", "learned"],
+                    help="wavefront schedule of the shell analysis (never changes a result); "
+                         "default: the autotuner's choice")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (library defaults, schedule 'crossing')")
     ap.add_argument("--gang", type=int, default=None, choices=[1, 16],
                     help="lanes per peak in the shell gather (library default if omitted)")
     ap.add_argument("--brick-log2", type=int, default=None,
@@ -86,6 +92,8 @@ def main():
         return
 
     cfg = CONFIGS[args.config]
+    if args.config == "batched64":
+        return run_batched(args, cfg)
     n, box, nbuff, nhunt = cfg["n"], cfg["box"], cfg["nbuff"], cfg["nhunt"]
     alatt = box / n
     dev = torch.device("cuda", 0)
@@ -123,6 +131,31 @@ def main():
         raise SystemExit(f"--order learned needs trained weights at {bracket.DEFAULT_WEIGHTS}")
 
     results = None
+    tuning = None
+    if not args.no_autotune and args.order is None and not knobs:
+        from huntian_hip import autotune
+
+        mask.zero_()
+        pf0 = api.peak_find(smoothed, fc_peak, nbuff, mask, capacity=cap)
+        n0 = len(pf0["i"])
+        res0 = api.alloc_results(n0, dev)
+        Rf0 = Rf_tab[pf0["filt"].long()]
+        ir0 = ir2_tab[pf0["filt"].long()]
+
+        def run(order):
+            mask.zero_()
+            api.shell_analyse(fields, pf0["i"], pf0["j"], pf0["k"], Rf0, ir0, stab, ct,
+                              alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff,
+                              variant=args.variant, early_exit=True, mask=mask, results=res0,
+                              order=order, bracket=net)
+
+        t1 = time.time()
+        tuning = autotune.tune(run, ctx, has_bracket=net is not None, log=log)
+        log(f"autotune: best {tuning['best']} after {time.time() - t1:.1f} s")
+        del res0, pf0
+        args.order = tuning["best"]["order"]
+    if args.order is None:
+        args.order = "crossing"
 
     def step():
         nonlocal results
@@ -286,6 +319,7 @@ def main():
                                "z = 0, psi1 + psi2, no lapd",
                    "method": f"shell scan with exact early exit, schedule '{args.order}'",
                    "schedule_ablation_shell_ms": ablation,
+                   "autotune": tuning,
                    "variant": args.variant, "candidates": npk, "halos": nhalo,
                    "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
@@ -297,6 +331,99 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args, cfg):
+    """Uneven tiles: shell analysis of all candidates of 64 tiles, (a) one library call per
+    tile as the reference drivers do, (b) one persistent launch over all tiles, (c) the same
+    recorded as a hipGraph plan and replayed.  Peak finding runs per tile, untimed."""
+    import numpy as np
+    import torch
+
+    from huntian_hip import api, synth
+    from huntian_hip.homeltab import WEBSKY_TABLE, read_homeltab
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(cfg["seeds"][0])
+    nt = cfg["ntiles"]
+    sides = rng.choice(cfg["sides"], size=nt)
+    nbuffs = rng.integers(cfg["nbuff"][0], cfg["nbuff"][1] + 1, size=nt)
+    radii = synth.filter_radii(cfg["rmin"], cfg["rmax"], cfg["nfilt"])
+    alatt, nhunt = cfg["alatt"], cfg["nhunt"]
+    ct = read_homeltab(WEBSKY_TABLE)
+    fcrit = api.fsc_of_z(ct, 0.0)
+    stab = api.build_shell_tables(nhunt)
+    Rf_tab = torch.tensor(radii, dtype=torch.float64, device=dev)
+    ir2_tab = torch.tensor([api.ir2min_of(R, alatt) for R in radii], dtype=torch.int32,
+                           device=dev)
+    ctx = api.context(0)
+    tiles, counts = [], []
+    t0 = time.time()
+    for q in range(nt):
+        n, nb, seed = int(sides[q]), int(nbuffs[q]), cfg["seeds"][0] + q
+        box = alatt * n
+        fields, dk = synth.fields_torch(n, box, seed, dev)
+        sm = [synth.smooth_tophat_torch(dk, n, box, R) for R in radii]
+        del dk
+        mask = api.fortran_empty((n, n, n), torch.int8, dev)
+        mask.zero_()
+        pf = api.peak_find(sm, [1.5] * len(radii), nb, mask, capacity=(n - 2 * nb) ** 3)
+        del sm
+        f = pf["filt"].long()
+        tiles.append(dict(fields=fields, pi=pf["i"], pj=pf["j"], pk=pf["k"], Rf=Rf_tab[f],
+                          ir2min=ir2_tab[f], mask=mask, nbuff=nb))
+        counts.append(len(pf["i"]))
+    torch.cuda.synchronize()
+    total = int(sum(counts))
+    log(f"{nt} tiles, sides {sorted(set(int(s) for s in sides))}, {total} candidates "
+        f"(per tile {min(counts)}..{max(counts)}), setup {time.time() - t0:.1f} s")
+    kw = dict(alatt=alatt, ZZon=1.0, fcrit=fcrit, variant=args.variant)
+    nbuff_all = int(min(nbuffs))  # one mask buffer width for the batched call
+
+    def per_tile():
+        for t in tiles:
+            api.shell_analyse(t["fields"], t["pi"], t["pj"], t["pk"], t["Rf"], t["ir2min"],
+                              stab, ct, nbuff=t["nbuff"], mask=t["mask"], **kw)
+
+    def batched():
+        api.shell_analyse_tiles(tiles, stab, ct, nbuff=nbuff_all, **kw)
+
+    plan = api.ShellPlan(tiles, stab, ct, nbuff=nbuff_all, **kw)
+    modes = {"per_tile_calls": per_tile, "one_launch": batched, "hipgraph_plan": plan.launch}
+    ms = {}
+    for name, fn in modes.items():
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms[name] = e0.elapsed_time(e1) / args.steps
+        log(f"{name}: {ms[name]:.2f} ms per pass")
+    plan.close()
+    best = min(ms, key=ms.get)
+    out = {
+        "metric": "peaks resolved/s, 64 uneven tiles, shell analysis (batched)",
+        "value": round(total / (ms[best] * 1e-3), 1), "unit": "peaks/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms[best], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (sums of f32 inputs and f32 products, as the reference CPU path)",
+        "data": "synthetic",
+        "config": {"workload": f"{nt} tiles, sides from {list(cfg['sides'])}, nbuff "
+                               f"{cfg['nbuff'][0]}-{cfg['nbuff'][1]}, seeds "
+                               f"{cfg['seeds'][0]}-{cfg['seeds'][1]}, 16 top-hat filters "
+                               f"{cfg['rmin']}-{cfg['rmax']} Mpc/h, hunt radius {nhunt}",
+                   "candidates": total, "candidates_per_tile": [min(counts), max(counts)],
+                   "ms_per_pass": {k: round(v, 3) for k, v in ms.items()},
+                   "speedup_vs_per_tile_calls": round(ms["per_tile_calls"] / ms[best], 3),
+                   "best_mode": best, "parallelism": "1 GPU"},
     }
     print(json.dumps(out), flush=True)
 
