@@ -1497,14 +1497,14 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
                                                (uint64_t*)W.keys_b.p, (int32_t*)W.vals_a.p,
                                                (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st));
     HNT_TRY(reserve(ctx, W.cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
-    auto order_by = [&](const int32_t* const* prim) -> hnt_status {
+    auto order_by = [&](const int32_t* const* prim, int L) -> hnt_status {
         for (int t = 0; t < ntiles; ++t) {
             const hnt_batch& b = batches[t];
             if (b.npeaks == 0) continue;
             hipLaunchKernelGGL(tile_key_kernel, dim3((unsigned)((b.npeaks + 255) / 256)),
                                dim3(256), 0, st, b.pi, b.pj, b.pk, prim[t], b.npeaks,
-                               (uint64_t)t, meta[t], ctx->brick_log2,
-                               (uint64_t*)W.keys_a.p, (int32_t*)W.vals_a.p);
+                               (uint64_t)t, meta[t], L, (uint64_t*)W.keys_a.p,
+                               (int32_t*)W.vals_a.p);
         }
         hipError_t e;
         sort_pairs(W, tmp_bytes, ntot, st, &e);
@@ -1518,7 +1518,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     };
     std::vector<const int32_t*> prim((size_t)ntiles);
     for (int t = 0; t < ntiles; ++t) prim[t] = batches[t].ir2min;
-    HNT_TRY(order_by(prim.data()));
+    // first order: 16^3 bricks when it only feeds the crossing pass (its delta-only reads
+    // favour small bricks), else the schedule's brick size
+    const bool two_orders = params->early_exit && params->order != HNT_ORDER_PLAIN;
+    HNT_TRY(order_by(prim.data(), two_orders ? 4 : ctx->brick_log2));
     hipLaunchKernelGGL(group_tile_kernel, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0,
                        st, tile_pad, ntiles, ngroups, (int32_t*)W.gtiles.p);
 
@@ -1602,7 +1605,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
                                0, st, pred, ntot, A.NS - 1, stops);
         }
         for (int t = 0; t < ntiles; ++t) prim[t] = stops + meta[t];
-        HNT_TRY(order_by(prim.data()));
+        HNT_TRY(order_by(prim.data(), ctx->brick_log2));
         stage_end(ctx, HNT_STAGE_SA_CROSS);
     }
 
