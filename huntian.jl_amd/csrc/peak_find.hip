@@ -10,11 +10,17 @@
 // makes every cell independent, so one pass decides all filters.
 //
 // Kernels
-//  pf_mark    one workgroup per 64 x 8 x 16 brick of the inner cube.  For each filter the
-//             workgroup sweeps the brick along k with a ring of three LDS planes (66 x 10
-//             floats each: brick + 1-cell halo), so every field value is read from HBM about
-//             once (x1.13 for the k halo, halo rows/columns mostly hit L2).  The claim (first
-//             filter index, 255 = none) of each cell is kept in registers until the end.
+//  pf_mark    a wavefront owns 62 i x 8 j x 32 k cells of the inner cube; a workgroup is
+//             4 x 2 wavefronts along i and j.  For each filter a wavefront streams the 10
+//             rows (its 8 + a 1-cell halo) of each plane with one coalesced 256-byte load
+//             per row, one plane ahead of the arithmetic; the plane is reduced to 3x3 maxima
+//             (DPP lane shifts along i, then three rows), and the k direction keeps the last
+//             planes' maxima in registers (no LDS).  One barrier per plane keeps the
+//             workgroup's wavefronts on the same plane, so the cache lines they share (halo
+//             columns and rows) come from HBM once: 1.45x the field bytes at 1024^3, against
+//             2.1x for free-running wavefronts.  A peak is written to the claim array
+//             (filter index; 255 = none) by the one lane owning the cell; a per-lane bitset
+//             marks cells masked or already taken by an earlier filter.
 //  pf_count   per 4096-cell chunk of the claim array (inner-cube scan order k, j, i): peaks
 //             per filter.
 //  (hipcub exclusive scan over the filter-major count table)
@@ -26,87 +32,151 @@
 
 namespace {
 
-constexpr int TI = 64, TJ = 8, TK = 16;
-constexpr int HI = TI + 2, HJ = TJ + 2;
+// pf_mark geometry: a wavefront owns OI = 62 consecutive i (its 64 lanes load i-1 .. i+62),
+// TJ rows of j and TK planes of k; a workgroup is WI x WJ wavefronts along i and j.
+constexpr int OI = 62, TJ = 8, TK = 32, WI = 4, WJ = 2;
+constexpr int PF_THREADS = 64 * WI * WJ;
 constexpr int CHUNK = 4096;
-constexpr uint8_t NONE = 255, BLOCKED = 254;
+constexpr uint8_t NONE = 255;
+// Word 3 of a raw (stride 0) buffer resource on CDNA: 32-bit data format, no swizzle.
+constexpr int BUF_DWORD3 = 0x00020000;
+static_assert(TJ * 4 == 32 && TK % 4 == 0, "claim bits: one 32-bit word per 4 planes of TJ=8");
 
-__global__ __launch_bounds__(256) void pf_mark(const float* const* __restrict__ fields,
+// Left and right neighbour of a lane's value within the wavefront (lanes 0 and 63 receive
+// values that are never used: they are halo columns).
+// DPP wave_shr:1 / wave_shl:1 (whole-wavefront shifts by one lane, a VALU operand modifier).
+__device__ __forceinline__ float from_lane_below(float v) {
+    const int x = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_lane_above(float v) {
+    const int x = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, 0x130, 0xF, 0xF, false));
+}
+
+// The 26-neighbour test of the reference, !(ff < x) for every neighbour x and for fcrit, is
+// the same as !(ff < max(neighbours)) with IEEE maxNum (fmaxf): a NaN neighbour drops out of
+// the max exactly as !(ff < NaN) drops out of the conjunction, and an all-NaN max passes.
+// So each plane is reduced once to its 3x3 maxima (row max through lane shuffles, then over
+// three rows) and the k direction keeps the last three planes' maxima in registers.
+__global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __restrict__ fields,
                                                const float* __restrict__ fcrit, int nfilt,
                                                int n, int nbuff, const int8_t* __restrict__ mask,
                                                uint8_t* __restrict__ claim) {
-    __shared__ float pl[3][HJ][HI];
     const int m = n - 2 * nbuff;
-    const int ti = threadIdx.x & 63, tj = threadIdx.x >> 6;  // tj in 0..3, rows tj and tj+4
-    const int i0 = blockIdx.x * TI, j0 = blockIdx.y * TJ, k0 = blockIdx.z * TK;
-    const int64_t s1 = n, s2 = (int64_t)n * n;
+    // the wave index is uniform across the wavefront: say so, so row offsets stay scalar
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i0 = (blockIdx.x * WI + w % WI) * OI;
+    const int j0 = (blockIdx.y * WJ + w / WI) * TJ;
+    const int k0 = blockIdx.z * TK;
+    // A wavefront past the inner cube still streams (clamped) planes and meets every barrier;
+    // all its cells are outside and so marked taken below.
+    const int io = i0 - 1 + lane;              // inner-cube i of this lane's column, >= -1
+    const bool out_i = lane >= 1 && lane <= OI && io < m;
+    // column offset inside a plane; io is clamped to m (a halo column: values unused)
+    const int col_bytes = (min(io, m) + nbuff) * 4;
+    const int64_t s2 = (int64_t)n * n;
+    const int row_bytes = 4 * n, plane_bytes = (int)(4 * s2);  // n <= 4096: below 2^31
+    const int64_t mm = m, mm2 = (int64_t)m * m;
 
-    uint8_t cl[2][TK];
+    // Cells already taken (masked, or a peak of an earlier filter): bit 8*(kk%4)+jr of
+    // word kk/4.  The words rotate by one per group of 4 planes, so word 0 is always current.
+    uint32_t taken[TK / 4];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-#pragma unroll
-        for (int kk = 0; kk < TK; ++kk) {
-            int ii = i0 + ti, jj = j0 + tj + 4 * c, kq = k0 + kk;
-            uint8_t v = BLOCKED;
-            if (ii < m && jj < m && kq < m) {
-                int64_t a = (int64_t)(ii + nbuff) + (jj + nbuff) * s1 + (kq + nbuff) * s2;
-                v = (mask[a] == 1) ? BLOCKED : NONE;
-            }
-            cl[c][kk] = v;
+    for (int q = 0; q < TK / 4; ++q) {
+        uint32_t bits = 0;
+#pragma unroll 4
+        for (int e = 0; e < 32; ++e) {
+            const int kk = 4 * q + (e >> 3), jr = e & 7;
+            const int jj = j0 + jr, kq = k0 + kk;
+            bool t = true;  // cells outside the inner cube count as taken
+            if (out_i && jj < m && kq < m)
+                t = mask[(int64_t)(io + nbuff) + (int64_t)(jj + nbuff) * n + (kq + nbuff) * s2] == 1;
+            bits |= (uint32_t)t << e;
         }
+        taken[q] = bits;
     }
 
     for (int f = 0; f < nfilt; ++f) {
         const float* __restrict__ F = fields[f];
         const float fc = fcrit[f];
+        // Rows j0-1 .. j0+TJ of inner-cube plane kp (clamped to m: beyond it nothing is
+        // tested), as buffer loads off a per-plane resource: the lane's column offset is the
+        // one vector register, the row offset is scalar.
+        auto load = [&](int kp, float (&v)[TJ + 2]) {
+            const float* P = F + (int64_t)(min(kp, m) + nbuff) * s2;
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc((void*)P, 0, plane_bytes, BUF_DWORD3);
 #pragma unroll
-        for (int step = 0; step < TK + 2; ++step) {
-            const int p = k0 - 1 + step;  // inner-cube plane index, -1 .. k0+TK
-            const int ka = min(p + nbuff, n - 1);
-            const int slot = step % 3;
-            for (int e = threadIdx.x; e < HJ * HI; e += 256) {
-                int jh = e / HI, ih = e - jh * HI;
-                int ia = min(i0 - 1 + ih + nbuff, n - 1);
-                int ja = min(j0 - 1 + jh + nbuff, n - 1);
-                pl[slot][jh][ih] = F[(int64_t)ia + ja * s1 + ka * s2];
+            for (int r = 0; r < TJ + 2; ++r)
+                v[r] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_raw_buffer_load_b32(
+                               rs, col_bytes, (min(j0 - 1 + r, m) + nbuff) * row_bytes, 0));
+        };
+        auto reduce = [&](const float (&v)[TJ + 2], float (&mx)[TJ]) {
+            float h[TJ + 2];
+#pragma unroll
+            for (int r = 0; r < TJ + 2; ++r)
+                h[r] = fmaxf(v[r], fmaxf(from_lane_below(v[r]), from_lane_above(v[r])));
+#pragma unroll
+            for (int r = 0; r < TJ; ++r) mx[r] = fmaxf(h[r], fmaxf(h[r + 1], h[r + 2]));
+        };
+        // State entering the test of plane kk: mx = 3x3 maxima of plane kk, cc = its cell
+        // values, and bit r of out = cell (r, kk) already fails against fcrit or against
+        // planes kk-1 and kk.  !(ff < max(a, b, c)) splits exactly into !(ff < max(a, b))
+        // and !(ff < c) under maxNum, so plane kk+1 completes the test.
+        float va[TJ + 2], mx[TJ], cc[TJ];
+        uint32_t out = 0;
+        {
+            float mprev[TJ];
+            load(k0 - 1, va);
+            reduce(va, mprev);
+            load(k0, va);
+            reduce(va, mx);
+#pragma unroll
+            for (int r = 0; r < TJ; ++r) {
+                cc[r] = va[r + 1];
+                // the reference skips `ff < fcrit`; a NaN is therefore not skipped
+                out |= (uint32_t)((cc[r] < fc) || (cc[r] < fmaxf(mprev[r], mx[r]))) << r;
             }
-            __syncthreads();
-            if (step >= 2) {
-                const int kk = step - 2;
-                const int sb = (step - 2) % 3, sc = (step - 1) % 3, sa = step % 3;
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    if (cl[c][kk] == NONE) {
-                        const int jh = tj + 4 * c + 1, ih = ti + 1;
-                        const float ff = pl[sc][jh][ih];
-                        // the reference skips `ff < fcrit`; a NaN is therefore not skipped
-                        if (!(ff < fc)) {
-                            bool ismax = true;
-#pragma unroll
-                            for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-                                for (int dx = -1; dx <= 1; ++dx) {
-                                    ismax &= !(ff < pl[sb][jh + dy][ih + dx]);
-                                    ismax &= !(ff < pl[sc][jh + dy][ih + dx]);
-                                    ismax &= !(ff < pl[sa][jh + dy][ih + dx]);
-                                }
-                            if (ismax) cl[c][kk] = (uint8_t)f;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
         }
-    }
-
-    const int64_t mm = m, mm2 = (int64_t)m * m;
+        load(k0 + 1, va);  // plane kk+1, in flight
+#pragma unroll 1
+        for (int g = 0; g < TK / 4; ++g) {
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+            for (int u = 0; u < 4; ++u) {
+                const int kk = 4 * g + u;
+                float cur[TJ + 2], mn[TJ];
 #pragma unroll
-        for (int kk = 0; kk < TK; ++kk) {
-            int ii = i0 + ti, jj = j0 + tj + 4 * c, kq = k0 + kk;
-            if (ii < m && jj < m && kq < m)
-                claim[ii + jj * mm + kq * mm2] = (cl[c][kk] == BLOCKED) ? NONE : cl[c][kk];
+                for (int r = 0; r < TJ + 2; ++r) cur[r] = va[r];
+                load(k0 + kk + 2, va);  // next plane in flight while this one is reduced
+                // Keep the workgroup's wavefronts on the same plane: the cache lines they
+                // share (halo rows and columns) are then fetched from HBM once.
+                __syncthreads();
+                reduce(cur, mn);
+                uint32_t hit = 0, nout = 0;
+#pragma unroll
+                for (int r = 0; r < TJ; ++r) {
+                    hit |= (uint32_t)!(cc[r] < mn[r]) << (8 * u + r);
+                    const float c = cur[r + 1];
+                    nout |= (uint32_t)((c < fc) || (c < fmaxf(mx[r], mn[r]))) << r;
+                    cc[r] = c;
+                    mx[r] = mn[r];
+                }
+                hit &= ~(taken[0] | (out << (8 * u)));
+                out = nout;
+                if (hit & (0xFFu << (8 * u))) {
+#pragma unroll
+                    for (int r = 0; r < TJ; ++r)
+                        if (hit & (1u << (8 * u + r)))
+                            claim[io + (j0 + r) * mm + (k0 + kk) * mm2] = (uint8_t)f;
+                }
+                taken[0] |= hit;
+            }
+            const uint32_t t0 = taken[0];
+#pragma unroll
+            for (int q = 0; q + 1 < TK / 4; ++q) taken[q] = taken[q + 1];
+            taken[TK / 4 - 1] = t0;
         }
     }
 }
@@ -239,9 +309,10 @@ extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
                            hipMemcpyHostToDevice, st));
     const float* const* dfields = (const float* const*)ctx->fieldptrs.p;
 
-    dim3 g1((m + TI - 1) / TI, (m + TJ - 1) / TJ, (m + TK - 1) / TK);
+    dim3 g1((m + OI * WI - 1) / (OI * WI), (m + TJ * WJ - 1) / (TJ * WJ), (m + TK - 1) / TK);
     stage_begin(ctx, HNT_STAGE_PF_MARK);
-    hipLaunchKernelGGL(pf_mark, g1, dim3(256), 0, st, dfields, (const float*)ctx->fcrit.p, nfilt,
+    HNT_HIP(hipMemsetAsync(ctx->claim.p, NONE, (size_t)ncell, st));
+    hipLaunchKernelGGL(pf_mark, g1, dim3(PF_THREADS), 0, st, dfields, (const float*)ctx->fcrit.p, nfilt,
                        n, nbuff, (const int8_t*)mask, (uint8_t*)ctx->claim.p);
     stage_end(ctx, HNT_STAGE_PF_MARK);
     stage_begin(ctx, HNT_STAGE_PF_COMPACT);

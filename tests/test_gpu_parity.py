@@ -69,14 +69,23 @@ def _mask(n):
     return np.zeros((n, n, n), np.int8, order="F")
 
 
-@pytest.mark.parametrize("n,nbuff,seed", [(24, 2, 1), (37, 3, 2), (64, 16, 3), (70, 1, 4)])
-def test_peak_find_matches_oracle(oracle, n, nbuff, seed):
+@pytest.mark.parametrize("n,nbuff,seed,nan", [(24, 2, 1, False), (37, 3, 2, False),
+                                               (64, 16, 3, False), (70, 1, 4, False),
+                                               (100, 2, 5, False), (45, 4, 6, True)])
+def test_peak_find_matches_oracle(oracle, n, nbuff, seed, nan):
+    """Random (rough) fields: many maxima, ties on a plateau, a 3% mask; inner cubes that are
+    not multiples of the kernel's 62 x 8 x 32 cells per wavefront, several of them along
+    every axis at n=100; with nan, 0.5% NaN cells, whose comparisons are all false (a NaN
+    neighbour never blocks a peak, a NaN cell passes every test)."""
     import torch
 
     rng = np.random.default_rng(seed)
     fields = [np.asfortranarray((rng.standard_normal((n, n, n)) * s).astype(np.float32))
               for s in (1.0, 1.4, 0.7, 1.1)]
     fields[2][nbuff + 2:nbuff + 5, nbuff + 2:nbuff + 5, nbuff + 2:nbuff + 5] = 5.0  # plateau
+    if nan:
+        for a in fields:
+            a[rng.random((n, n, n)) < 0.005] = np.nan
     fc = [0.6, 1.0, 0.3, 1.7]
     mask0 = _mask(n)
     mask0[rng.random((n, n, n)) < 0.03] = 1
@@ -86,8 +95,11 @@ def test_peak_find_matches_oracle(oracle, n, nbuff, seed):
     md = api.to_device_field(mask0, dtype=torch.int8)
     got = api.peak_find(dev, fc, nbuff, md, capacity=tot + 10)
     assert got["total"] == tot > 0
-    for k in ("i", "j", "k", "filt", "delta"):
+    for k in ("i", "j", "k", "filt"):
         assert np.array_equal(got[k].cpu().numpy(), ref[k]), k
+    assert np.array_equal(got["delta"].cpu().numpy(), ref["delta"], equal_nan=True)
+    if nan:
+        assert np.isnan(ref["delta"]).sum() > 10
     assert np.array_equal(md.cpu().numpy(), mref)
 
 
