@@ -320,7 +320,8 @@ hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value) {
         HNT_REQUIRE(value >= 0 && value <= 13, "hnt_ctx_set_tuning: brick_log2 %d", value);
         ctx->brick_log2 = value;
     } else if (what == HNT_TUNE_GANG) {
-        HNT_REQUIRE(value == 1 || value == 16, "hnt_ctx_set_tuning: gang must be 1 or 16");
+        HNT_REQUIRE(value == 1 || value == 16 || value == 64,
+                    "hnt_ctx_set_tuning: gang must be 1, 16 or 64");
         ctx->gang = value;
     } else if (what == HNT_TUNE_WAVES_PER_CU) {
         HNT_REQUIRE(value >= 0 && value <= 64, "hnt_ctx_set_tuning: waves_per_cu %d", value);

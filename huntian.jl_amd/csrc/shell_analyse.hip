@@ -63,6 +63,7 @@ struct SAArgs {
     int nbuff, variant, early;
     double* prof;
     int64_t slot_doubles;
+    const int32_t* stop_hint;  // exact stop shell per global peak id (crossing pass) or null
     unsigned long long* qctr;
     int64_t qbeg[8], qend[8];
 };
@@ -330,6 +331,9 @@ struct SumF64 {
     double s;
     __device__ __forceinline__ void zero() { s = 0.0; }
     __device__ __forceinline__ void add(float v) { s += (double)v; }
+    // add v if `on`, branch-free: a sum that starts at +0.0 is never -0.0, so adding +0.0
+    // leaves it bit for bit as it was
+    __device__ __forceinline__ void add_if(bool on, float v) { s += on ? (double)v : 0.0; }
     __device__ __forceinline__ double val() const { return s; }
 };
 struct SumKahan {
@@ -343,6 +347,13 @@ struct SumKahan {
         const float t = s + y;
         c = (t - s) - y;
         s = t;
+    }
+    __device__ __forceinline__ void add_if(bool on, float v) {
+        const float y = v - c;
+        const float t = s + y;
+        const float cn = (t - s) - y;
+        c = on ? cn : c;
+        s = on ? t : s;
     }
     __device__ __forceinline__ double val() const { return (double)s - (double)c; }
 };
@@ -1142,6 +1153,368 @@ __global__ __launch_bounds__(64) void shell_gang_kernel(SAArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ wavefront gather
+// One wavefront per peak.  The wavefront walks its peak's cell table in chunks of up to
+// WCAP cells (ending at most WMAXSH shells later): first all 64 lanes load the chunk's cell
+// records into LDS at once (the records of a peak's ball share cache lines, and lines
+// requested together are fetched from HBM once), then 22 lanes each run one of the
+// reference's shell sums over the chunk cell by cell in table order -- the same FP64 (or
+// Kahan) additions of the same Float32 terms as the one-lane-per-peak gather, so the
+// results are bit-identical to it -- and lane 22 the running lapd sum.  At every shell end
+// the sums go to an LDS table; after the chunk, lane e of the wavefront turns each finished
+// shell into profile entry e, and every lane evaluates the Fbar recurrence and the early-exit
+// rule.  With the crossing pass's exact stop shell a chunk never reads past the stop; without
+// it, at most one chunk is read beyond.  The 64 peaks of a group are gathered one after the
+// other, then post-processed one lane each.
+constexpr int WCAP = 256, WMAXSH = 32;
+static_assert(WCAP % 8 == 0 && WMAXSH + 2 <= 64, "wave gather chunk shape");
+
+// lane l's double, read by every lane (l uniform)
+__device__ __forceinline__ double rl_double(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+struct WaveLds {
+    float4 rec[2 * WCAP];      // cell records {delta, delta^T, psi1 xyz, psi2 xyz}
+    float mult[3][WCAP];       // the cell's offsets as Float32 (x, y, z)
+    float lap[WCAP];
+    uint8_t in[WCAP];          // cell inside the grid
+    double shs[WMAXSH][24];    // per finished shell: the 22 sums, running lap, lapn
+    int shn[WMAXSH];           // cells inside the grid, per finished shell
+};
+
+// What the gather of one peak needs besides the tables, computed by the peak's own lane
+// for all 64 peaks of a group at once and handed to the wavefront with readlane.
+struct WavePeak {
+    int ci, cj, ck, ir2min, mupp_rf, s_lim;
+    double fcrit;
+    float4 v0c, v1c;  // the peak cell's record
+    float lapc;
+};
+
+__device__ __forceinline__ float rl_float(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ float4 rl_float4(float4 v, int l) {
+    return make_float4(rl_float(v.x, l), rl_float(v.y, l), rl_float(v.z, l), rl_float(v.w, l));
+}
+
+// Registers of one chunk in flight: lane l holds cells cA + l + 64u.
+struct WaveFetch {
+    int4 o[WCAP / 64];
+    float4 r0[WCAP / 64], r1[WCAP / 64];
+    float lp[WCAP / 64];
+    bool ok[WCAP / 64];
+};
+
+template <bool CHECK>
+__device__ __forceinline__ void wave_fetch(const SAArgs& a, const TileDev& T, const WavePeak& K,
+                                           int64_t c0, int cA, int cB, WaveFetch& F) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < WCAP / 64; ++u) {
+        const int c = cA + lane + 64 * u;
+        F.o[u] = c < cB ? a.cells[c] : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < WCAP / 64; ++u) {
+        const int c = cA + lane + 64 * u;
+        const int4 o = F.o[u];
+        bool ok = c < cB;
+        if (CHECK) {
+            const int i1 = K.ci + o.x, i2 = K.cj + o.y, i3 = K.ck + o.z;
+            ok = ok && i1 >= 1 && i1 <= T.n && i2 >= 1 && i2 <= T.n && i3 >= 1 && i3 <= T.n;
+        }
+        F.ok[u] = ok;
+        if (ok) {
+            const int64_t off = c0 + o.x + o.y * T.s1 + o.z * T.s2;
+            F.r0[u] = T.recs[2 * off];
+            F.r1[u] = T.recs[2 * off + 1];
+            F.lp[u] = T.has_lapd ? T.lapd[off] : 0.0f;
+        }
+    }
+}
+
+template <class S, bool CHECK>
+__device__ void wave_gather(const SAArgs& a, const TileDev& T, int p, const WavePeak& K,
+                            double* __restrict__ P, WaveLds& L) {
+    const int lane = threadIdx.x;
+    const double* __restrict__ rad = a.rad;
+    const int m = a.NS - 1;
+    const int64_t c0 = (int64_t)(K.ci - 1) + (K.cj - 1) * T.s1 + (K.ck - 1) * T.s2;
+    const bool varA = a.variant == HNT_VARIANT_A;
+    const bool eta2 = T.has_eta2, haslap = T.has_lapd;
+    const double fcrit = K.fcrit;
+
+    // sum lanes: 0 F, 1-3 S, 4-6 S2 (psi2), 7-9 G, 10-12 H, 13-21 R00 R01 R02 R10 .. R22;
+    // record float fidx, times the cell's offset component msel (0 = no product)
+    int fidx = 0, msel = 0;
+    if (lane <= 6) fidx = lane == 0 ? 0 : lane + 1;
+    else if (lane <= 9) msel = lane - 6;
+    else if (lane <= 12) { fidx = 1; msel = lane - 9; }
+    else if (lane <= 21) { fidx = 2 + (lane - 13) / 3; msel = 1 + (lane - 13) % 3; }
+    const bool summer = lane < 22 && (eta2 || lane < 4 || lane > 6);
+    const bool lapper = lane == 22 && haslap;
+    const float* __restrict__ recf = reinterpret_cast<const float*>(L.rec);
+    const float* __restrict__ multf = L.mult[msel > 0 ? msel - 1 : 0];
+
+    // shell 1: the peak cell
+    const float4 v0c = K.v0c, v1c = K.v1c;
+    double Fprev = (double)v0c.x, dFbarp = Fprev, cumN = 1.0;
+    double run = 0.0;  // entry lanes 14-16 cumulative psi1, 17-19 cumulative psi2
+    if (lane == 14) run = 0.0 + (double)v0c.z;
+    else if (lane == 15) run = 0.0 + (double)v0c.w;
+    else if (lane == 16) run = 0.0 + (double)v1c.x;
+    else if (lane == 17) run = eta2 ? 0.0 + (double)v1c.y : 0.0;
+    else if (lane == 18) run = eta2 ? 0.0 + (double)v1c.z : 0.0;
+    else if (lane == 19) run = eta2 ? 0.0 + (double)v1c.w : 0.0;
+    double lap = 0.0;  // lane 22
+    int64_t lapn = 0;
+    bool oobA = false;
+    if (haslap && !varA) {
+        lap = 0.0 + (double)K.lapc;
+        lapn = 1;
+    }
+    if (lane < 22) {
+        double x = 0.0;
+        if (lane == K_F) x = Fprev;
+        else if (lane == K_N) x = cumN;
+        else if (lane >= K_S && lane < K_LAP) x = run;
+        else if (lane == K_LAP) x = lap;
+        else if (lane == K_LAPN) x = (double)lapn;
+        P[PW + lane] = x;
+    }
+
+    bool past = false, crossed = false, stop_set = false, done = false;
+    double rstop = 0.0;
+    int stop = m, scross = 0;
+    int s_lim = K.s_lim;
+    int s_cur = 2;  // first shell not yet finished (summed and turned into profile entries)
+    // lane q: table end of shell s_cur + q
+    auto shell_ends = [&](int s0) {
+        const int sq = s0 + lane;
+        return sq <= m ? a.shell_start[sq - 1] + a.shell_count[sq - 1] : 0x7fffffff;
+    };
+    int ends_v = shell_ends(s_cur);
+    int cA = a.shell_start[1];
+    int cB = min(cA + WCAP,
+                 __builtin_amdgcn_readlane(ends_v, min(s_cur + WMAXSH - 1, s_lim) - s_cur));
+    WaveFetch F;
+    wave_fetch<CHECK>(a, T, K, c0, cA, cB, F);
+    S acc;
+    acc.zero();
+    int nin_sh = 0;
+    while (!done) {
+        // ---- stage the chunk [cA, cB) ----
+#pragma unroll
+        for (int u = 0; u < WCAP / 64; ++u) {
+            const int cl = lane + 64 * u;
+            if (cA + cl < cB) {
+                if (CHECK) L.in[cl] = F.ok[u];
+                L.rec[2 * cl] = F.r0[u];
+                L.rec[2 * cl + 1] = F.r1[u];
+                L.lap[cl] = F.lp[u];
+                L.mult[0][cl] = (float)F.o[u].x;
+                L.mult[1][cl] = (float)F.o[u].y;
+                L.mult[2][cl] = (float)F.o[u].z;
+            }
+        }
+        __syncthreads();
+        // ---- the next chunk follows from the table alone: fetch it during the sums ----
+        const int j = __popcll(__ballot(ends_v <= cB));  // shells ending in this chunk
+        const int s_next = s_cur + j;
+        const bool more = s_next <= s_lim;
+        int cB2 = cB;
+        if (more) {
+            cB2 = min(cB + WCAP, __builtin_amdgcn_readlane(
+                                     ends_v, min(s_next + WMAXSH - 1, s_lim) - s_cur));
+            wave_fetch<CHECK>(a, T, K, c0, cB, cB2, F);
+        }
+        // for the finishing pass, lane q: rad[s_cur - 1 + q] and the r^2 of shell s_cur + q
+        const double rad_v = rad[min(s_cur - 1 + lane, m + 1)];
+        const int r2_v = a.shell_r2[min(s_cur + lane, m) - 1];
+        // ---- sums in table order ----
+        int jj = 0;
+        int e = __builtin_amdgcn_readlane(ends_v, 0);
+        for (int cc = cA; cc < cB; cc += 8) {
+            float t[8], lv[8];
+            bool ok[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int cl = cc - cA + u;  // < WCAP: WCAP is a multiple of 8
+                ok[u] = CHECK ? L.in[cl] != 0 : true;
+                const float v = recf[8 * cl + fidx];
+                const float prod = v * multf[cl];
+                t[u] = msel ? prod : v;
+                lv[u] = L.lap[cl];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (cc + u < cB) {
+                    acc.add_if(summer && ok[u], t[u]);
+                    // lane 22: running lapd sum; variant A stops at the first cell outside
+                    oobA = oobA || !ok[u];
+                    const bool addlap = lapper && ok[u] && (!varA || !oobA);
+                    lap += addlap ? (double)lv[u] : 0.0;
+                    lapn += addlap ? 1 : 0;
+                    nin_sh += ok[u] ? 1 : 0;
+                    if (cc + u + 1 == e) {
+                        if (lane < 22) L.shs[jj][lane] = acc.val();
+                        if (lane == 22) {
+                            L.shs[jj][22] = lap;
+                            L.shs[jj][23] = (double)lapn;
+                        }
+                        if (lane == 0) L.shn[jj] = nin_sh;
+                        ++jj;
+                        acc.zero();
+                        nin_sh = 0;
+                        e = __builtin_amdgcn_readlane(ends_v, jj);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- finish the shells that ended in this chunk ----
+        for (int q = 0; q < j && !done; ++q) {
+            const int s = s_cur + q;
+            const double radP = rl_double(rad_v, q), radS = rl_double(rad_v, q + 1),
+                         radN = rl_double(rad_v, q + 2);
+            const int r2s = __builtin_amdgcn_readlane(r2_v, q);
+            const double* sh = L.shs[q];
+            const int nin = L.shn[q];
+            const double dFbar = nin > 0 ? sh[0] / (double)nin : 0.0;
+            const double rad3p = cube(radP), rad3 = cube(radS);
+            const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
+            Fprev = Fb;
+            dFbarp = dFbar;
+            if (nin > 0) cumN += (double)nin;
+            if (lane < 22) {
+                double x;
+                if (lane < 3) {
+                    x = sh[7 + lane] / radS;
+                } else if (lane < 6) {
+                    x = sh[10 + lane - 3] / radS;
+                } else if (lane < 12) {
+                    // SR + SR^T: r00+r00, r01+r10, r02+r20, r11+r11, r12+r21, r22+r22
+                    const int ia = lane == 6 ? 13 : lane == 7 ? 14 : lane == 8 ? 15
+                                 : lane == 9 ? 17 : lane == 10 ? 18 : 21;
+                    const int ib = lane == 6 ? 13 : lane == 7 ? 16 : lane == 8 ? 19
+                                 : lane == 9 ? 17 : lane == 10 ? 20 : 21;
+                    x = sh[ia] / radS + sh[ib] / radS;
+                } else if (lane == K_F) {
+                    x = Fb;
+                } else if (lane == K_N) {
+                    x = cumN;
+                } else if (lane < K_LAP) {
+                    if (nin > 0) run += sh[lane < K_S2 ? 1 + lane - K_S : 4 + lane - K_S2];
+                    x = run;
+                } else {
+                    x = sh[lane == K_LAP ? 22 : 23];
+                }
+                P[(int64_t)s * PW + lane] = x;
+            }
+            if (!past && r2s > K.ir2min) past = true;
+            if (past && !crossed && Fb < fcrit) {
+                crossed = true;
+                scross = s;
+                rstop = radN + 2.0;
+            }
+            if (!stop_set && crossed && radN >= rstop && s >= K.mupp_rf) {
+                stop = s;
+                stop_set = true;
+                if (a.early) done = true;
+            }
+        }
+        s_cur = s_next;
+        if (s_cur > m) done = true;
+        if (done) break;
+        ends_v = shell_ends(s_cur);
+        if (!more) {
+            // a stop hint that fell short only bounded the chunks: go on to the last shell
+            s_lim = m;
+            cB2 = min(cB + WCAP,
+                      __builtin_amdgcn_readlane(ends_v, min(s_cur + WMAXSH - 1, s_lim) - s_cur));
+            wave_fetch<CHECK>(a, T, K, c0, cB, cB2, F);
+        }
+        cA = cB;
+        cB = cB2;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        if (T.res.nshells_used) T.res.nshells_used[p] = stop;
+        if (T.res.crossing_shell) T.res.crossing_shell[p] = scross;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void shell_wave_kernel(SAArgs a) {
+    __shared__ WaveLds L;
+    double* Pbase = a.prof + (int64_t)blockIdx.x * 64 * a.slot_doubles;
+    const int lane = threadIdx.x;
+    int q = blockIdx.x & 7;
+    for (int tried = 0; tried < 8;) {
+        unsigned long long g = 0;
+        if (lane == 0) g = atomicAdd(&a.qctr[q], 1ull);
+        g = __shfl(g, 0);
+        const int64_t grp = a.qbeg[q] + (int64_t)g;
+        if (grp >= a.qend[q]) {
+            ++tried;
+            q = (q + 1) & 7;
+            continue;
+        }
+        const TileDev& T = a.tiles[a.gtile[grp]];
+        const int R = a.rmax, m = a.NS - 1;
+        // every lane prepares its own peak
+        const int pl = a.slot[grp * 64 + lane];
+        WavePeak mine{};
+        int mrf = 1, mlow_rf = 1, inside = 1;
+        double ZZon = 0.0;
+        if (pl >= 0) {
+            mine.ci = T.pi[pl];
+            mine.cj = T.pj[pl];
+            mine.ck = T.pk[pl];
+            mine.ir2min = T.ir2min[pl];
+            mine.fcrit = T.fcrit ? T.fcrit[pl] : T.fcrit_s;
+            ZZon = T.ZZon ? T.ZZon[pl] : T.ZZon_s;
+            rf_window(a, T.Rf[pl], mrf, mlow_rf, mine.mupp_rf);
+            mine.s_lim = m;
+            if (a.early && a.stop_hint) mine.s_lim = min(max(a.stop_hint[T.base + pl], 2), m);
+            const int64_t c0 = (int64_t)(mine.ci - 1) + (mine.cj - 1) * T.s1 + (mine.ck - 1) * T.s2;
+            mine.v0c = T.recs[2 * c0];
+            mine.v1c = T.recs[2 * c0 + 1];
+            mine.lapc = T.has_lapd ? T.lapd[c0] : 0.0f;
+            inside = mine.ci - R >= 1 && mine.ci + R <= T.n && mine.cj - R >= 1 &&
+                     mine.cj + R <= T.n && mine.ck - R >= 1 && mine.ck + R <= T.n;
+        }
+        for (int idx = 0; idx < 64; ++idx) {
+            const int p = __builtin_amdgcn_readlane(pl, idx);
+            if (p < 0) continue;
+            WavePeak K;
+            K.ci = __builtin_amdgcn_readlane(mine.ci, idx);
+            K.cj = __builtin_amdgcn_readlane(mine.cj, idx);
+            K.ck = __builtin_amdgcn_readlane(mine.ck, idx);
+            K.ir2min = __builtin_amdgcn_readlane(mine.ir2min, idx);
+            K.mupp_rf = __builtin_amdgcn_readlane(mine.mupp_rf, idx);
+            K.s_lim = __builtin_amdgcn_readlane(mine.s_lim, idx);
+            K.fcrit = rl_double(mine.fcrit, idx);
+            K.v0c = rl_float4(mine.v0c, idx);
+            K.v1c = rl_float4(mine.v1c, idx);
+            K.lapc = rl_float(mine.lapc, idx);
+            double* P = Pbase + (int64_t)idx * a.slot_doubles;
+            if (__builtin_amdgcn_readlane(inside, idx)) wave_gather<S, false>(a, T, p, K, P, L);
+            else wave_gather<S, true>(a, T, p, K, P, L);
+        }
+        __syncthreads();  // profiles written by the whole wavefront, read per lane below
+        if (pl >= 0)
+            post_process(a, T, pl, Pbase + (int64_t)lane * a.slot_doubles, mine.ci, mine.cj,
+                         mine.ck, ZZon, mine.fcrit, mine.ir2min, mrf, mlow_rf, mine.mupp_rf);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ crossing pass
 // Work of a peak = shells gathered, fixed by its crossing shell s_c (first shell past ir2min
 // with Fbar < fcrit) and its filter-scale window.  This pass finds s_c from delta alone,
@@ -1346,7 +1719,8 @@ struct RepackJob {
 struct LaunchDesc {
     SAArgs A;
     int64_t nblocks = 0;
-    bool gang = false, kahan = false;
+    int gang = 1;  // lanes per peak: 1, 16 or 64 (one wavefront)
+    bool kahan = false;
     std::vector<RepackJob> repacks;
 };
 
@@ -1360,9 +1734,13 @@ void launch_repacks(hipStream_t st, const std::vector<RepackJob>& jobs) {
 
 void launch_main(hipStream_t st, const LaunchDesc& d) {
     const dim3 grid((unsigned)d.nblocks), blk(64);
-    if (d.gang && d.kahan)
+    if (d.gang == 64 && d.kahan)
+        hipLaunchKernelGGL(shell_wave_kernel<SumKahan>, grid, blk, 0, st, d.A);
+    else if (d.gang == 64)
+        hipLaunchKernelGGL(shell_wave_kernel<SumF64>, grid, blk, 0, st, d.A);
+    else if (d.gang == 16 && d.kahan)
         hipLaunchKernelGGL(shell_gang_kernel<SumKahan>, grid, blk, 0, st, d.A);
-    else if (d.gang)
+    else if (d.gang == 16)
         hipLaunchKernelGGL(shell_gang_kernel<SumF64>, grid, blk, 0, st, d.A);
     else if (d.kahan)
         hipLaunchKernelGGL(shell_analyse_kernel<SumKahan>, grid, blk, 0, st, d.A);
@@ -1553,11 +1931,13 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     A.early = params->early_exit ? 1 : 0;
 
     const bool kahan = params->precision == HNT_PREC_FP32_KAHAN;
-    const bool gang = ctx->gang == 16;
-    const void* kern = gang ? (kahan ? (const void*)shell_gang_kernel<SumKahan>
-                                     : (const void*)shell_gang_kernel<SumF64>)
-                            : (kahan ? (const void*)shell_analyse_kernel<SumKahan>
-                                     : (const void*)shell_analyse_kernel<SumF64>);
+    const int gang = ctx->gang;
+    const void* kern = gang == 64 ? (kahan ? (const void*)shell_wave_kernel<SumKahan>
+                                           : (const void*)shell_wave_kernel<SumF64>)
+                     : gang == 16 ? (kahan ? (const void*)shell_gang_kernel<SumKahan>
+                                           : (const void*)shell_gang_kernel<SumF64>)
+                                  : (kahan ? (const void*)shell_analyse_kernel<SumKahan>
+                                           : (const void*)shell_analyse_kernel<SumF64>);
     int per_cu = 0;
     HNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
     if (per_cu < 1) per_cu = 1;
@@ -1605,6 +1985,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
                                0, st, pred, ntot, A.NS - 1, stops);
         }
         for (int t = 0; t < ntiles; ++t) prim[t] = stops + meta[t];
+        if (params->order == HNT_ORDER_CROSSING) A.stop_hint = stops;
         HNT_TRY(order_by(prim.data(), ctx->brick_log2));
         stage_end(ctx, HNT_STAGE_SA_CROSS);
     }

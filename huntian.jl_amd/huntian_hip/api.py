@@ -58,7 +58,8 @@ class Context:
             self._stream_ptr = ptr
 
     def set_tuning(self, **knobs):
-        """Launch-shape knobs (brick_log2, waves_per_cu); speed only, never results."""
+        """Launch-shape knobs (brick_log2, waves_per_cu, gang = lanes per peak: 1, 16 or
+        64); speed only, never results."""
         for k, v in knobs.items():
             abi.check(self.lib, self.lib.hnt_ctx_set_tuning(self.handle, abi.TUNING[k], int(v)))
 

@@ -1,8 +1,10 @@
 """Autotuner for the shell analysis launch shape.
 
-It times the real call over a small search space -- lanes per peak (1 or 16, "gang"), the
-peak-order brick size (2^brick_log2 cells a side) and the wavefront schedule (exact crossing
-pass, plain order, learned bracket) -- and keeps the fastest.  None of these knobs changes a
+It times the real call over a small search space -- lanes per peak (1 or 16, "gang"; the
+library also has 64, one wavefront per peak, which is left out of the default space: at the
+1024^3 tile it reads 1.5x fewer bytes but runs 3x longer), the peak-order brick size
+(2^brick_log2 cells a side) and the wavefront schedule (exact crossing pass, plain order,
+learned bracket) -- and keeps the fastest.  None of these knobs changes a
 result; only the time.  Results are cached per workload signature in a JSON file so a
 production run tunes once per shape.
 """

@@ -86,7 +86,9 @@ hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t
  *  GANG:         lanes per peak in the shell gather.  1: each lane sums its peak's cells in
  *                the reference's order (results equal the FP64 CPU path bit for bit);
  *                16: sixteen lanes share a peak and their partial sums are added in lane
- *                order (the same FP64 sums in another order, ~1e-13 relative) */
+ *                order (the same FP64 sums in another order, ~1e-13 relative);
+ *                64: a wavefront loads a peak's cells together into LDS and 22 of its
+ *                lanes each run one sum in the reference's order (bit for bit, like 1) */
 enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1, HNT_TUNE_GANG = 2 };
 hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value);
 

@@ -19,10 +19,11 @@ FLOAT_KEYS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half", "eigs", 
 RTOL = 1e-10
 
 
-@pytest.fixture(params=[1, 16], ids=["lane", "gang16"])
+@pytest.fixture(params=[1, 16, 64], ids=["lane", "gang16", "wave"])
 def gang(request):
-    """Run a test with one lane per peak (the reference's summation order) and with sixteen
-    lanes per peak (the same sums in lane order); the bar is the same."""
+    """Run a test with one lane per peak (the reference's summation order), with sixteen
+    lanes per peak (the same sums in lane order) and with one wavefront per peak (cells
+    staged in LDS, each sum in the reference's order by its own lane); the bar is the same."""
     ctx = api.context()
     ctx.set_tuning(gang=request.param)
     yield request.param
@@ -353,3 +354,35 @@ def test_boundary_peaks_with_lapd(oracle, bumps, gang):
         g = to_host(r)
         compare(g, ref, f"{variant} boundary+lapd")
         assert np.any(g["d2F"] != 0)
+
+
+@pytest.mark.parametrize("order", ["crossing", "plain"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_kahan"])
+def test_wave_gather_bit_identical_to_lane_gather(websky, cosmo64, order, precision):
+    """The wavefront gather runs every sum in the reference's cell order, like the one-lane
+    gather, so every output is bit-identical -- with the crossing pass's stop shells bounding
+    its chunks and without them (plain order), in FP64 and in FP32-Kahan."""
+    import torch
+
+    f, radii, sm = cosmo64
+    n, alatt, nbuff, nhunt = 64, 2.0, 11, 10
+    md = api.to_device_field(_mask(n), dtype=torch.int8)
+    pk = api.peak_find([api.to_device_field(a) for a in sm], [1.5] * len(radii), nbuff, md,
+                       capacity=n ** 3)
+    Rf = np.array([radii[q] for q in pk["filt"].cpu().numpy()])
+    ir2 = np.array([api.ir2min_of(r, alatt) for r in Rf], np.int32)
+    stab = api.build_shell_tables(nhunt)
+    kw = dict(alatt=alatt, ZZon=1.0, fcrit=api.fsc_of_z(websky, 0.0), variant="C",
+              nbuff=nbuff, order=order, precision=precision)
+    args = (f, pk["i"], pk["j"], pk["k"], Rf, ir2, stab, websky)
+    ctx = api.context()
+    try:
+        ctx.set_tuning(gang=1)
+        one = api.shell_analyse(*args, **kw)
+        ctx.set_tuning(gang=64)
+        wave = api.shell_analyse(*args, **kw)
+    finally:
+        ctx.set_tuning(gang=1)
+    assert len(pk["i"]) > 30
+    for k in one:
+        assert torch.equal(torch.nan_to_num(one[k], nan=7.0), torch.nan_to_num(wave[k], nan=7.0)), k
