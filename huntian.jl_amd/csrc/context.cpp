@@ -58,15 +58,25 @@ hnt_status begin_call(hnt_ctx* ctx) {
     return HNT_OK;
 }
 
+// A stage may run several times in one call (the per-filter steps of the tile peak finder):
+// each run records its own pair of events, and finish_call adds all of them up.
 void stage_begin(hnt_ctx* ctx, int stage) {
     if (!ctx->profiling) return;
-    if (hipEventRecord(ctx->ev[2 * stage], ctx->stream) == hipSuccess) ctx->ev_used[stage] = true;
+    std::vector<hipEvent_t>& ev = ctx->ev[stage];
+    const size_t q = 2 * (size_t)ctx->ev_n[stage];
+    while (ev.size() < q + 2) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ev.push_back(e);
+    }
+    ctx->ev_open[stage] = hipEventRecord(ev[q], ctx->stream) == hipSuccess;
 }
 
 void stage_end(hnt_ctx* ctx, int stage) {
-    if (!ctx->profiling || !ctx->ev_used[stage]) return;
-    if (hipEventRecord(ctx->ev[2 * stage + 1], ctx->stream) != hipSuccess)
-        ctx->ev_used[stage] = false;
+    if (!ctx->profiling || !ctx->ev_open[stage]) return;
+    ctx->ev_open[stage] = false;
+    const size_t q = 2 * (size_t)ctx->ev_n[stage];
+    if (hipEventRecord(ctx->ev[stage][q + 1], ctx->stream) == hipSuccess) ctx->ev_n[stage] += 1;
 }
 
 hnt_status finish_call(hnt_ctx* ctx, const char* what) {
@@ -82,13 +92,16 @@ hnt_status finish_call(hnt_ctx* ctx, const char* what) {
     }
     if (ctx->profiling) {
         for (int s = 0; s < HNT_NSTAGES; ++s) {
-            if (!ctx->ev_used[s]) continue;
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ctx->ev[2 * s], ctx->ev[2 * s + 1]) == hipSuccess) {
-                ctx->stage_ms[s] += ms;
-                ctx->stage_calls[s] += 1;
+            if (ctx->ev_n[s] == 0) continue;
+            for (int q = 0; q < ctx->ev_n[s]; ++q) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, ctx->ev[s][2 * q], ctx->ev[s][2 * q + 1]) ==
+                    hipSuccess)
+                    ctx->stage_ms[s] += ms;
             }
-            ctx->ev_used[s] = false;
+            ctx->stage_calls[s] += 1;
+            ctx->ev_n[s] = 0;
+            ctx->ev_open[s] = false;
         }
     }
     return HNT_OK;
@@ -304,10 +317,13 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
                       &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
                       &ctx->tables.kw_idx, &ctx->ctab.table, &ctx->mlp.wt, &ctx->mlp.x0,
                       &ctx->mlp.z, &ctx->mlp.h, &ctx->mlp.y, &ctx->mlp.dy, &ctx->mlp.dyf,
-                      &ctx->mlp.dz, &ctx->mlp.gw, &ctx->mlp.part, &ctx->mlp.gradv};
+                      &ctx->mlp.dz, &ctx->mlp.gw, &ctx->mlp.part, &ctx->mlp.gradv,
+                      &ctx->fb_spec, &ctx->fb_work, &ctx->fb_field, &ctx->fb_lap,
+                      &ctx->fb_val, &ctx->fb_d2};
     for (DevBuf* b : bufs) release(*b);
-    for (hipEvent_t& e : ctx->ev)
-        if (e) (void)hipEventDestroy(e);
+    release_fft(ctx);
+    for (std::vector<hipEvent_t>& v : ctx->ev)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return HNT_OK;
@@ -334,13 +350,12 @@ hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value) {
 
 hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on) {
     HNT_TRY(begin_call(ctx));
-    if (on && !ctx->ev[0])
-        for (hipEvent_t& e : ctx->ev) HNT_HIP(hipEventCreate(&e));
     ctx->profiling = on != 0;
     for (int s = 0; s < HNT_NSTAGES; ++s) {
         ctx->stage_ms[s] = 0.0;
         ctx->stage_calls[s] = 0;
-        ctx->ev_used[s] = false;
+        ctx->ev_n[s] = 0;
+        ctx->ev_open[s] = false;
     }
     return HNT_OK;
 }

@@ -93,6 +93,11 @@ struct hnt_ctx {
     hnt::CTabDev ctab;
     hnt::MlpScratch mlp;
     hnt::DevBuf tiles, tilemeta, slots, gtiles;  // multi-tile shell analysis
+    // tile peak finding: spectrum, windowed spectrum, smoothed fields, per-cell peak values
+    hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2;
+    void* fft_r2c = nullptr;  // hipfftHandle of an n^3 real-to-complex plan
+    void* fft_c2r = nullptr;
+    int fft_n = 0;
     std::vector<hnt::TileDev> tile_host;
     int brick_log2 = 6;  // peak-order bricks of 2^brick_log2 cells a side (0..13)
     int waves_per_cu = 0;  // cap on resident shell-analysis waves per CU (0 = occupancy)
@@ -100,8 +105,9 @@ struct hnt_ctx {
     std::vector<int64_t> meta_host;
     // stage timers (hnt_ctx_set_profiling)
     bool profiling = false;
-    hipEvent_t ev[2 * HNT_NSTAGES] = {};
-    bool ev_used[HNT_NSTAGES] = {};
+    std::vector<hipEvent_t> ev[HNT_NSTAGES];  // begin/end pairs of the current call
+    int ev_n[HNT_NSTAGES] = {};                // pairs recorded in the current call
+    bool ev_open[HNT_NSTAGES] = {};
     double stage_ms[HNT_NSTAGES] = {};
     int64_t stage_calls[HNT_NSTAGES] = {};
 };
@@ -111,6 +117,7 @@ namespace hnt {
 hnt_status reserve(hnt_ctx* ctx, DevBuf& b, size_t bytes);
 void release(DevBuf& b);
 void release_work(SAWork& w);
+void release_fft(hnt_ctx* ctx);  // FFT plans of hnt_peak_find_tile
 hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t);
 hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);

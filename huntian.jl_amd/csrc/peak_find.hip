@@ -27,6 +27,7 @@
 //  pf_scatter writes the peaks of each chunk at their scan offsets: output order is
 //             filter, then k, j, i -- the order of the reference CPU loop.
 #include <hipcub/hipcub.hpp>
+#include <hipfft/hipfft.h>
 
 #include "hnt_internal.h"
 
@@ -37,7 +38,7 @@ namespace {
 constexpr int OI = 62, TJ = 8, TK = 32, WI = 4, WJ = 2;
 constexpr int PF_THREADS = 64 * WI * WJ;
 constexpr int CHUNK = 4096;
-constexpr uint8_t NONE = 255;
+constexpr uint8_t NONE = 255, BLOCKED = 254;  // claim of a cell: filter index, or these
 // Word 3 of a raw (stride 0) buffer resource on CDNA: 32-bit data format, no swizzle.
 constexpr int BUF_DWORD3 = 0x00020000;
 static_assert(TJ * 4 == 32 && TK % 4 == 0, "claim bits: one 32-bit word per 4 planes of TJ=8");
@@ -59,10 +60,21 @@ __device__ __forceinline__ float from_lane_above(float v) {
 // the max exactly as !(ff < NaN) drops out of the conjunction, and an all-NaN max passes.
 // So each plane is reduced once to its 3x3 maxima (row max through lane shuffles, then over
 // three rows) and the k direction keeps the last three planes' maxima in registers.
+// One launch runs filters fbase .. fbase + nfilt - 1 (fields[f] is filter fbase + f).  The
+// cells not to claim come from the mask (the caller cleared the claim array), or, in the
+// TILE instance with from_claim = 1, from the claim array itself, read only where a peak is
+// found (masked cells hold BLOCKED, cells of earlier launches their filter); there a claimed
+// cell also records its field value in cellval (and, with celld2, the value of
+// lapfields[f]), indexed like the claim array.
+template <bool TILE>
 __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __restrict__ fields,
                                                const float* __restrict__ fcrit, int nfilt,
-                                               int n, int nbuff, const int8_t* __restrict__ mask,
-                                               uint8_t* __restrict__ claim) {
+                                               int fbase, int n, int nbuff,
+                                               const int8_t* __restrict__ mask, int from_claim,
+                                               uint8_t* __restrict__ claim,
+                                               float* __restrict__ cellval,
+                                               const float* const* __restrict__ lapfields,
+                                               float* __restrict__ celld2) {
     const int m = n - 2 * nbuff;
     // the wave index is uniform across the wavefront: say so, so row offsets stay scalar
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -91,7 +103,10 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
             const int jj = j0 + jr, kq = k0 + kk;
             bool t = true;  // cells outside the inner cube count as taken
             if (out_i && jj < m && kq < m)
-                t = mask[(int64_t)(io + nbuff) + (int64_t)(jj + nbuff) * n + (kq + nbuff) * s2] == 1;
+                t = (TILE && from_claim)
+                        ? false  // the claim array is checked at a hit instead
+                        : mask[(int64_t)(io + nbuff) + (int64_t)(jj + nbuff) * n +
+                               (kq + nbuff) * s2] == 1;
             bits |= (uint32_t)t << e;
         }
         taken[q] = bits;
@@ -168,8 +183,19 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
                 if (hit & (0xFFu << (8 * u))) {
 #pragma unroll
                     for (int r = 0; r < TJ; ++r)
-                        if (hit & (1u << (8 * u + r)))
-                            claim[io + (j0 + r) * mm + (k0 + kk) * mm2] = (uint8_t)f;
+                        if (hit & (1u << (8 * u + r))) {
+                            const int64_t ci = io + (j0 + r) * mm + (k0 + kk) * mm2;
+                            // masked (BLOCKED) or taken by an earlier launch's filter
+                            if (TILE && from_claim && claim[ci] != NONE) continue;
+                            claim[ci] = (uint8_t)(fbase + f);
+                            if (TILE && cellval) {
+                                const int64_t a = (int64_t)(io + nbuff) +
+                                                  (int64_t)(j0 + r + nbuff) * n +
+                                                  (k0 + kk + nbuff) * s2;
+                                cellval[ci] = F[a];
+                                if (celld2) celld2[ci] = lapfields[f][a];
+                            }
+                        }
                 }
                 taken[0] |= hit;
             }
@@ -192,7 +218,7 @@ __global__ __launch_bounds__(256) void pf_count(const uint8_t* __restrict__ clai
         int64_t idx = base + e;
         if (idx < ncell) {
             uint8_t v = claim[idx];
-            if (v != NONE) atomicAdd(&c[v], 1u);
+            if (v < BLOCKED) atomicAdd(&c[v], 1u);
         }
     }
     __syncthreads();
@@ -202,9 +228,10 @@ __global__ __launch_bounds__(256) void pf_count(const uint8_t* __restrict__ clai
 __global__ __launch_bounds__(256) void pf_scatter(
     const uint8_t* __restrict__ claim, int64_t ncell, int64_t nchunks,
     const unsigned long long* __restrict__ offsets, int n, int nbuff,
-    const float* const* __restrict__ fields, int8_t* __restrict__ mask, int32_t* __restrict__ oi,
+    const float* const* __restrict__ fields, const float* __restrict__ cellval,
+    const float* __restrict__ celld2, int8_t* __restrict__ mask, int32_t* __restrict__ oi,
     int32_t* __restrict__ oj, int32_t* __restrict__ ok, int32_t* __restrict__ of,
-    float* __restrict__ od, int64_t capacity) {
+    float* __restrict__ od, float* __restrict__ od2, int64_t capacity) {
     __shared__ int s_scan[256];
     __shared__ int s_cell[CHUNK];
     __shared__ uint8_t s_f[CHUNK];
@@ -217,7 +244,7 @@ __global__ __launch_bounds__(256) void pf_scatter(
     for (int q = 0; q < 16; ++q) {
         int64_t idx = base + t * 16 + q;
         v[q] = idx < ncell ? claim[idx] : NONE;
-        mine += v[q] != NONE;
+        mine += v[q] < BLOCKED;
     }
     // block exclusive scan of the per-thread counts (Hillis-Steele in LDS)
     s_scan[t] = mine;
@@ -232,7 +259,7 @@ __global__ __launch_bounds__(256) void pf_scatter(
     int pos = s_scan[t] - mine;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        if (v[q] != NONE) {
+        if (v[q] < BLOCKED) {
             s_cell[pos] = t * 16 + q;
             s_f[pos] = v[q];
             ++pos;
@@ -258,15 +285,173 @@ __global__ __launch_bounds__(256) void pf_scatter(
             oj[dst] = jj + nbuff + 1;
             ok[dst] = kq + nbuff + 1;
             of[dst] = f;
-            od[dst] = fields[f][a];
+            od[dst] = cellval ? cellval[idx] : fields[f][a];
+            if (od2) od2[dst] = celld2 ? celld2[idx] : 0.0f;
             mask[a] = 1;
         }
+    }
+}
+
+// Claim array of the tile path before its first filter: BLOCKED where the mask is 1.
+__global__ __launch_bounds__(256) void pf_claim_init(const int8_t* __restrict__ mask, int n,
+                                                     int nbuff, uint8_t* __restrict__ claim) {
+    const int m = n - 2 * nbuff;
+    const int64_t mm = m, ncell = mm * mm * mm;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int kq = (int)(c / (mm * mm));
+        const int64_t rem = c - kq * mm * mm;
+        const int jj = (int)(rem / mm), ii = (int)(rem - jj * mm);
+        const int64_t a = (int64_t)(ii + nbuff) + (int64_t)(jj + nbuff) * n +
+                          (int64_t)(kq + nbuff) * n * n;
+        claim[c] = mask[a] == 1 ? BLOCKED : NONE;
+    }
+}
+
+// Window product of _smooth_window_kernel! (ext/CUDAExt.jl:2831-2871) on the rfft spectrum
+// [k][j][0..n/2] of a column-major tile: kx from the halved index, ky and kz signed; W in FP64
+// (top-hat 3(sin x - x cos x)/x^3, Gaussian exp(-x^2/8), or k^2 times it), rounded to Float32
+// and applied to both parts; k = 0 is left unchanged.  k^2 = kx^2 + ky^2 + kz^2 takes the
+// same FP64 value for +-ky and +-kz (negation is exact), so one thread evaluates W once for
+// (kx, |ky|, |kz|) and applies it to the up to four elements with those magnitudes.  The
+// inverse transform's 1/n^3 is applied here too (for a power-of-two n it is exact, so it
+// equals scaling afterwards).
+__global__ __launch_bounds__(256) void fb_window(const float2* __restrict__ spec,
+                                                 float2* __restrict__ out, int n, double dk,
+                                                 double Rf, int wsmooth, float inv_n3) {
+    const int nk = n / 2 + 1, nyq = n / 2;
+    const int64_t total = (int64_t)nk * (nyq + 1) * (nyq + 1);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(e % nk);
+        const int64_t rem = e / nk;
+        const int ay = (int)(rem % (nyq + 1)), az = (int)(rem / (nyq + 1));  // |ky|, |kz| / dk
+        const double kx = (double)ix * dk, ky = (double)ay * dk, kz = (double)az * dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        float w32 = 1.0f;
+        bool apply = false;
+        if (k2 != 0.0) {
+            const double k = sqrt(k2), kR = k * Rf;
+            double w;
+            if (wsmooth == 0) {
+                w = exp(-kR * kR * 0.125);
+            } else if (wsmooth == 3) {
+                w = exp(-kR * kR * 0.125) * k2;
+            } else {
+                double sn, cs;
+                sincos(kR, &sn, &cs);
+                w = 3.0 * (sn - kR * cs) / (kR * kR * kR);
+            }
+            w32 = (float)w;
+            apply = true;
+        }
+        // rows iy = ay and n - ay (one row when ay is 0, or n/2 for an even n), likewise iz
+        const bool even = (n & 1) == 0;
+        const int ys[2] = {ay, n - ay}, zs[2] = {az, n - az};
+        const int ny = (ay == 0 || (even && ay == nyq)) ? 1 : 2;
+        const int nz = (az == 0 || (even && az == nyq)) ? 1 : 2;
+        for (int b = 0; b < nz; ++b)
+            for (int a = 0; a < ny; ++a) {
+                const int64_t q = ix + (int64_t)nk * (ys[a] + (int64_t)n * zs[b]);
+                float2 v = spec[q];
+                if (apply) {
+                    v.x *= w32;
+                    v.y *= w32;
+                }
+                v.x *= inv_n3;
+                v.y *= inv_n3;
+                out[q] = v;
+            }
     }
 }
 
 }  // namespace
 
 using namespace hnt;
+
+namespace hnt {
+void release_fft(hnt_ctx* ctx) {
+    if (ctx->fft_r2c) (void)hipfftDestroy((hipfftHandle)ctx->fft_r2c);
+    if (ctx->fft_c2r) (void)hipfftDestroy((hipfftHandle)ctx->fft_c2r);
+    ctx->fft_r2c = ctx->fft_c2r = nullptr;
+    ctx->fft_n = 0;
+}
+}  // namespace hnt
+
+#define HNT_FFT(expr)                                                                   \
+    do {                                                                                \
+        const hipfftResult r_ = (expr);                                                 \
+        if (r_ != HIPFFT_SUCCESS) {                                                     \
+            hnt::set_error("%s: hipFFT status %d (%s:%d)", #expr, (int)r_, __FILE__,    \
+                           __LINE__);                                                   \
+            return HNT_ERR_DEVICE;                                                      \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+// Buffers of the claim array and the filter-major count table for an inner cube of ncell
+// cells and nfilt filters.
+hnt_status reserve_claims(hnt_ctx* ctx, int64_t ncell, int nfilt, size_t& tmp_bytes) {
+    const int64_t nchunks = (ncell + CHUNK - 1) / CHUNK;
+    const int64_t ntab = nchunks * nfilt;
+    HNT_TRY(reserve(ctx, ctx->claim, (size_t)ncell));
+    HNT_TRY(reserve(ctx, ctx->counts, sizeof(unsigned long long) * (size_t)ntab));
+    HNT_TRY(reserve(ctx, ctx->offsets, sizeof(unsigned long long) * (size_t)ntab));
+    tmp_bytes = 0;
+    HNT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes,
+                                             (unsigned long long*)ctx->counts.p,
+                                             (unsigned long long*)ctx->offsets.p, (int)ntab,
+                                             ctx->stream));
+    HNT_TRY(reserve(ctx, ctx->cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
+    return HNT_OK;
+}
+
+// Count, scan and scatter the claimed cells into `out` (filter, then k, j, i), mask them,
+// wait for the stream and fill out->count / out->total.
+hnt_status compact_claims(hnt_ctx* ctx, const char* what, int n, int nbuff, int nfilt,
+                          size_t tmp_bytes, const float* const* dfields, const float* cellval,
+                          const float* celld2, int8_t* mask, hnt_peaks* out, float* od2) {
+    hipStream_t st = ctx->stream;
+    const int m = n - 2 * nbuff;
+    const int64_t ncell = (int64_t)m * m * m;
+    const int64_t nchunks = (ncell + CHUNK - 1) / CHUNK;
+    const int64_t ntab = nchunks * nfilt;
+    stage_begin(ctx, HNT_STAGE_PF_COMPACT);
+    hipLaunchKernelGGL(pf_count, dim3((unsigned)nchunks), dim3(256), 0, st,
+                       (const uint8_t*)ctx->claim.p, ncell, nfilt, nchunks,
+                       (unsigned long long*)ctx->counts.p);
+    HNT_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp.p, tmp_bytes,
+                                             (unsigned long long*)ctx->counts.p,
+                                             (unsigned long long*)ctx->offsets.p, (int)ntab, st));
+    hipLaunchKernelGGL(pf_scatter, dim3((unsigned)nchunks), dim3(256), 0, st,
+                       (const uint8_t*)ctx->claim.p, ncell, nchunks,
+                       (const unsigned long long*)ctx->offsets.p, n, nbuff, dfields, cellval,
+                       celld2, mask, out->i, out->j, out->k, out->filt, out->delta, od2,
+                       out->capacity);
+    stage_end(ctx, HNT_STAGE_PF_COMPACT);
+    unsigned long long tail[2] = {0, 0};
+    HNT_HIP(hipMemcpyAsync(&tail[0], (unsigned long long*)ctx->offsets.p + (ntab - 1),
+                           sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HNT_HIP(hipMemcpyAsync(&tail[1], (unsigned long long*)ctx->counts.p + (ntab - 1),
+                           sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HNT_TRY(finish_call(ctx, what));
+    const int64_t total = (int64_t)(tail[0] + tail[1]);
+    out->total = total;
+    out->count = total < out->capacity ? total : out->capacity;
+    if (total > out->capacity) {
+        set_error("%s: %lld peaks found, capacity %lld", what, (long long)total,
+                  (long long)out->capacity);
+        return HNT_ERR_CAPACITY;
+    }
+    return HNT_OK;
+}
+
+dim3 mark_grid(int m) {
+    return dim3((m + OI * WI - 1) / (OI * WI), (m + TJ * WJ - 1) / (TJ * WJ), (m + TK - 1) / TK);
+}
+
+}  // namespace
 
 extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
                                     const float* fcrit, int32_t nfilt, int32_t n, int32_t nbuff,
@@ -288,58 +473,123 @@ extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
     if (m <= 0) return HNT_OK;
     hipStream_t st = ctx->stream;
     const int64_t ncell = (int64_t)m * m * m;
-    const int64_t nchunks = (ncell + CHUNK - 1) / CHUNK;
-    const int64_t ntab = nchunks * nfilt;
 
+    size_t tmp_bytes = 0;
+    HNT_TRY(reserve_claims(ctx, ncell, nfilt, tmp_bytes));
     HNT_TRY(reserve(ctx, ctx->fieldptrs, sizeof(float*) * (size_t)nfilt));
     HNT_TRY(reserve(ctx, ctx->fcrit, sizeof(float) * (size_t)nfilt));
-    HNT_TRY(reserve(ctx, ctx->claim, (size_t)ncell));
-    HNT_TRY(reserve(ctx, ctx->counts, sizeof(unsigned long long) * (size_t)ntab));
-    HNT_TRY(reserve(ctx, ctx->offsets, sizeof(unsigned long long) * (size_t)ntab));
-    size_t tmp_bytes = 0;
-    HNT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes,
-                                             (unsigned long long*)ctx->counts.p,
-                                             (unsigned long long*)ctx->offsets.p, (int)ntab, st));
-    HNT_TRY(reserve(ctx, ctx->cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
-    HNT_TRY(reserve(ctx, ctx->small, 64));
-
     HNT_HIP(hipMemcpyAsync(ctx->fieldptrs.p, smoothed, sizeof(float*) * (size_t)nfilt,
                            hipMemcpyHostToDevice, st));
     HNT_HIP(hipMemcpyAsync(ctx->fcrit.p, fcrit, sizeof(float) * (size_t)nfilt,
                            hipMemcpyHostToDevice, st));
     const float* const* dfields = (const float* const*)ctx->fieldptrs.p;
 
-    dim3 g1((m + OI * WI - 1) / (OI * WI), (m + TJ * WJ - 1) / (TJ * WJ), (m + TK - 1) / TK);
     stage_begin(ctx, HNT_STAGE_PF_MARK);
     HNT_HIP(hipMemsetAsync(ctx->claim.p, NONE, (size_t)ncell, st));
-    hipLaunchKernelGGL(pf_mark, g1, dim3(PF_THREADS), 0, st, dfields, (const float*)ctx->fcrit.p, nfilt,
-                       n, nbuff, (const int8_t*)mask, (uint8_t*)ctx->claim.p);
+    hipLaunchKernelGGL(pf_mark<false>, mark_grid(m), dim3(PF_THREADS), 0, st, dfields,
+                       (const float*)ctx->fcrit.p, nfilt, 0, n, nbuff, (const int8_t*)mask, 0,
+                       (uint8_t*)ctx->claim.p, nullptr, nullptr, nullptr);
     stage_end(ctx, HNT_STAGE_PF_MARK);
-    stage_begin(ctx, HNT_STAGE_PF_COMPACT);
-    hipLaunchKernelGGL(pf_count, dim3((unsigned)nchunks), dim3(256), 0, st,
-                       (const uint8_t*)ctx->claim.p, ncell, nfilt, nchunks,
-                       (unsigned long long*)ctx->counts.p);
-    HNT_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp.p, tmp_bytes,
-                                             (unsigned long long*)ctx->counts.p,
-                                             (unsigned long long*)ctx->offsets.p, (int)ntab, st));
-    hipLaunchKernelGGL(pf_scatter, dim3((unsigned)nchunks), dim3(256), 0, st,
-                       (const uint8_t*)ctx->claim.p, ncell, nchunks,
-                       (const unsigned long long*)ctx->offsets.p, n, nbuff, dfields, mask, out->i,
-                       out->j, out->k, out->filt, out->delta, out->capacity);
-    stage_end(ctx, HNT_STAGE_PF_COMPACT);
-    unsigned long long tail[2] = {0, 0};
-    HNT_HIP(hipMemcpyAsync(&tail[0], (unsigned long long*)ctx->offsets.p + (ntab - 1),
-                           sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    HNT_HIP(hipMemcpyAsync(&tail[1], (unsigned long long*)ctx->counts.p + (ntab - 1),
-                           sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    HNT_TRY(finish_call(ctx, "hnt_peak_find"));
-    const int64_t total = (int64_t)(tail[0] + tail[1]);
-    out->total = total;
-    out->count = total < out->capacity ? total : out->capacity;
-    if (total > out->capacity) {
-        set_error("hnt_peak_find: %lld peaks found, capacity %lld", (long long)total,
-                  (long long)out->capacity);
-        return HNT_ERR_CAPACITY;
+    return compact_claims(ctx, "hnt_peak_find", n, nbuff, nfilt, tmp_bytes, dfields, nullptr,
+                          nullptr, mask, out, nullptr);
+}
+
+extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32_t n,
+                                         int32_t nbuff, const hnt_filter_bank* bank,
+                                         int8_t* mask, hnt_peaks* out, float* d2Rf,
+                                         float* const* smoothed_out) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(delta && bank && mask && out, "hnt_peak_find_tile: NULL argument");
+    const int nfilt = bank->nfilt;
+    HNT_REQUIRE(nfilt >= 1 && nfilt <= 254, "hnt_peak_find_tile: nfilt %d not in 1..254",
+                nfilt);
+    HNT_REQUIRE(bank->Rf && bank->fcrit, "hnt_peak_find_tile: NULL filter radii / thresholds");
+    HNT_REQUIRE(bank->wsmooth == 0 || bank->wsmooth == 1 || bank->wsmooth == 3,
+                "hnt_peak_find_tile: wsmooth %d not 0, 1 or 3", bank->wsmooth);
+    HNT_REQUIRE(bank->alatt > 0.0, "hnt_peak_find_tile: alatt must be > 0");
+    HNT_REQUIRE(n >= 3 && n <= 2048, "hnt_peak_find_tile: n %d not in 3..2048", n);
+    HNT_REQUIRE(nbuff >= 1, "hnt_peak_find_tile: nbuff must be >= 1, got %d", nbuff);
+    HNT_REQUIRE(out->capacity >= 0, "hnt_peak_find_tile: negative capacity");
+    HNT_REQUIRE(out->capacity == 0 || (out->i && out->j && out->k && out->filt && out->delta),
+                "hnt_peak_find_tile: NULL output array");
+    out->count = 0;
+    out->total = 0;
+    const int m = n - 2 * nbuff;
+    if (m <= 0) return HNT_OK;
+    hipStream_t st = ctx->stream;
+    const int64_t ncell = (int64_t)m * m * m;
+    const int64_t nreal = (int64_t)n * n * n, ncplx = (int64_t)(n / 2 + 1) * n * n;
+    const bool lap = d2Rf != nullptr;
+
+    size_t tmp_bytes = 0;
+    HNT_TRY(reserve_claims(ctx, ncell, nfilt, tmp_bytes));
+    HNT_TRY(reserve(ctx, ctx->fieldptrs, sizeof(float*) * 2));
+    HNT_TRY(reserve(ctx, ctx->fcrit, sizeof(float) * (size_t)nfilt));
+    HNT_TRY(reserve(ctx, ctx->fb_spec, sizeof(float2) * (size_t)ncplx));
+    HNT_TRY(reserve(ctx, ctx->fb_work, sizeof(float2) * (size_t)ncplx));
+    HNT_TRY(reserve(ctx, ctx->fb_field, sizeof(float) * (size_t)nreal));
+    HNT_TRY(reserve(ctx, ctx->fb_val, sizeof(float) * (size_t)ncell));
+    if (lap) {
+        HNT_TRY(reserve(ctx, ctx->fb_lap, sizeof(float) * (size_t)nreal));
+        HNT_TRY(reserve(ctx, ctx->fb_d2, sizeof(float) * (size_t)ncell));
     }
-    return HNT_OK;
+    if (ctx->fft_n != n) {
+        release_fft(ctx);
+        hipfftHandle r2c = nullptr, c2r = nullptr;
+        HNT_FFT(hipfftPlan3d(&r2c, n, n, n, HIPFFT_R2C));
+        ctx->fft_r2c = r2c;
+        HNT_FFT(hipfftPlan3d(&c2r, n, n, n, HIPFFT_C2R));
+        ctx->fft_c2r = c2r;
+        ctx->fft_n = n;
+    }
+    hipfftHandle r2c = (hipfftHandle)ctx->fft_r2c, c2r = (hipfftHandle)ctx->fft_c2r;
+    HNT_FFT(hipfftSetStream(r2c, st));
+    HNT_FFT(hipfftSetStream(c2r, st));
+    float* field = (float*)ctx->fb_field.p;
+    float* lapf = lap ? (float*)ctx->fb_lap.p : nullptr;
+    const float* ptrs[2] = {field, lapf};
+    HNT_HIP(hipMemcpyAsync(ctx->fieldptrs.p, ptrs, sizeof(ptrs), hipMemcpyHostToDevice, st));
+    HNT_HIP(hipMemcpyAsync(ctx->fcrit.p, bank->fcrit, sizeof(float) * (size_t)nfilt,
+                           hipMemcpyHostToDevice, st));
+    const float* const* dptrs = (const float* const*)ctx->fieldptrs.p;
+
+    // rfft of the tile (from a copy: the caller's delta stays as it was)
+    stage_begin(ctx, HNT_STAGE_PF_SMOOTH);
+    HNT_HIP(hipMemcpyAsync(field, delta, sizeof(float) * (size_t)nreal, hipMemcpyDeviceToDevice,
+                           st));
+    HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)field, (hipfftComplex*)ctx->fb_spec.p));
+    stage_end(ctx, HNT_STAGE_PF_SMOOTH);
+    hipLaunchKernelGGL(pf_claim_init, dim3(2048), dim3(256), 0, st, (const int8_t*)mask, n,
+                       nbuff, (uint8_t*)ctx->claim.p);
+    const double dk = 2.0 * M_PI / ((double)n * bank->alatt);
+    const float inv_n3 = (float)(1.0 / (double)nreal);
+    const int64_t nwin = (int64_t)(n / 2 + 1) * (n / 2 + 1) * (n / 2 + 1);
+    const unsigned wblocks = (unsigned)std::min<int64_t>((nwin + 255) / 256, 16384);
+    for (int f = 0; f < nfilt; ++f) {
+        stage_begin(ctx, HNT_STAGE_PF_SMOOTH);
+        hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
+                           (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
+                           bank->Rf[f], bank->wsmooth, inv_n3);
+        HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fb_work.p, (hipfftReal*)field));
+        if (smoothed_out && smoothed_out[f])
+            HNT_HIP(hipMemcpyAsync(smoothed_out[f], field, sizeof(float) * (size_t)nreal,
+                                   hipMemcpyDeviceToDevice, st));
+        if (lap) {
+            hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
+                               (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
+                               bank->Rf[f], 3, inv_n3);
+            HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fb_work.p, (hipfftReal*)lapf));
+        }
+        stage_end(ctx, HNT_STAGE_PF_SMOOTH);
+        stage_begin(ctx, HNT_STAGE_PF_MARK);
+        hipLaunchKernelGGL(pf_mark<true>, mark_grid(m), dim3(PF_THREADS), 0, st, dptrs,
+                           (const float*)ctx->fcrit.p + f, 1, f, n, nbuff,
+                           (const int8_t*)mask, 1, (uint8_t*)ctx->claim.p,
+                           (float*)ctx->fb_val.p, lap ? dptrs + 1 : nullptr,
+                           lap ? (float*)ctx->fb_d2.p : nullptr);
+        stage_end(ctx, HNT_STAGE_PF_MARK);
+    }
+    return compact_claims(ctx, "hnt_peak_find_tile", n, nbuff, nfilt, tmp_bytes, dptrs,
+                          (const float*)ctx->fb_val.p, lap ? (const float*)ctx->fb_d2.p : nullptr,
+                          mask, out, d2Rf);
 }

@@ -19,7 +19,7 @@ HNT_ERR_CAPACITY = 3
 HNT_ERR_UNSUPPORTED = 4
 HNT_ERR_ALLOC = 5
 
-STAGES = ("pf_mark", "pf_compact", "sa_prep", "sa_kernel", "sa_cross")
+STAGES = ("pf_mark", "pf_compact", "sa_prep", "sa_kernel", "sa_cross", "pf_smooth")
 
 HNT_VARIANT_A = 0
 HNT_VARIANT_B = 1
@@ -106,6 +106,16 @@ class Batch(C.Structure):
     ]
 
 
+class FilterBank(C.Structure):
+    _fields_ = [
+        ("nfilt", C.c_int32),
+        ("Rf", _f64p),
+        ("fcrit", _f32p),
+        ("wsmooth", C.c_int32),
+        ("alatt", C.c_double),
+    ]
+
+
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS = 18, 128, 3
 MLP_NPARAM = 18 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 3 + 3
 HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
@@ -185,6 +195,7 @@ EXPORTED = (
     "hnt_build_shell_tables",
     "hnt_fsc_of_z",
     "hnt_peak_find",
+    "hnt_peak_find_tile",
     "hnt_shell_analyse",
     "hnt_shell_analyse_tiles",
     "hnt_plan_create",
@@ -240,6 +251,8 @@ def load(path: str | None = None):
     lib.hnt_fsc_of_z.restype = C.c_double
     lib.hnt_peak_find.argtypes = [vp, C.POINTER(vp), _f32p, C.c_int32, C.c_int32, C.c_int32,
                                   _i8p, C.POINTER(Peaks)]
+    lib.hnt_peak_find_tile.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.POINTER(FilterBank),
+                                       _i8p, C.POINTER(Peaks), _f32p, C.POINTER(vp)]
     lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                       C.POINTER(ShellTables), C.POINTER(CTab),
                                       C.POINTER(Params), C.POINTER(Results), _i8p]

@@ -236,28 +236,58 @@ def peak_find(smoothed, fcrits, nbuff: int, mask, capacity: int | None = None):
     return res
 
 
-def tophat_smooth_torch(delta_k, n, boxsize, R, wsmooth=1):
-    """Filter-bank window in k-space then irfft (src/HaloFinder/Filters.jl:72-99, GPU
-    _smooth_window_kernel!, ext/CUDAExt.jl:2831-2871).  delta_k is torch.fft.rfftn of the
-    C-order view [k, j, i].  Returns a column-major (n,n,n) float32 tensor."""
-    torch = _torch()
-    dev = delta_k.device
-    dk = 2.0 * np.pi / boxsize
-    kx = torch.arange(n // 2 + 1, device=dev, dtype=torch.float64) * dk
-    fy = torch.fft.fftfreq(n, d=1.0 / n, device=dev, dtype=torch.float64) * dk
-    k2 = kx[None, None, :] ** 2 + fy[None, :, None] ** 2 + fy[:, None, None] ** 2
-    k = torch.sqrt(k2)
-    x = k * R
-    if wsmooth == 0:
-        w = torch.exp(-x * x * 0.125)
-    elif wsmooth == 3:
-        w = torch.exp(-x * x * 0.125) * k2
-    else:
-        xs = torch.where(x > 0, x, torch.ones_like(x))
-        w = 3.0 * (torch.sin(xs) - xs * torch.cos(xs)) / xs ** 3
-    w = torch.where(k2 > 0, w.to(torch.float32).to(torch.float64), torch.ones_like(w))
-    out = torch.fft.irfftn(delta_k * w.to(delta_k.dtype), s=(n, n, n))
-    return out.to(torch.float32).permute(2, 1, 0)
+def peak_find_tile(delta, radii, fcrits, nbuff: int, mask, *, alatt: float, wsmooth: int = 1,
+                   with_lapd: bool = False, capacity: int | None = None, smoothed_out=None):
+    """Filter-bank smoothing and peak finding of one tile on the device (hnt_peak_find_tile).
+
+    delta: column-major (n, n, n) float32 device tensor (left unchanged); radii (Mpc/h) and
+    fcrits in processing order; mask: column-major (n, n, n) int8 device tensor, updated.
+    with_lapd: also return d2Rf, the k^2-Gaussian smoothing sampled at each peak.
+    smoothed_out: optional list of column-major float32 device tensors that receive the
+    smoothed fields (None entries are skipped).
+    Returns dict of device tensors i, j, k (1-based), filt, delta (FcRf), d2Rf, and total."""
+    torch = _require_gpu()
+    ctx = context(delta.device.index or 0)
+    n = delta.shape[0]
+    assert delta.shape == (n, n, n) and delta.dtype == torch.float32
+    assert delta.permute(2, 1, 0).is_contiguous()
+    assert mask.shape == (n, n, n) and mask.dtype == torch.int8 and mask.permute(2, 1, 0).is_contiguous()
+    nf = len(radii)
+    assert len(fcrits) == nf
+    m = max(n - 2 * nbuff, 0)
+    if capacity is None:
+        capacity = max(1024, m ** 3 // 64)
+    dev = delta.device
+    out = {k: torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
+           for k in ("i", "j", "k", "filt")}
+    out["delta"] = torch.empty(max(capacity, 1), dtype=torch.float32, device=dev)
+    d2 = torch.zeros(max(capacity, 1), dtype=torch.float32, device=dev)
+    pk = abi.Peaks(capacity, *[abi.as_ptr(out[k].data_ptr(), C.c_int32)
+                               for k in ("i", "j", "k", "filt")],
+                   abi.as_ptr(out["delta"].data_ptr(), C.c_float), 0, 0)
+    rf = np.ascontiguousarray(np.asarray(radii, np.float64))
+    fc = np.ascontiguousarray(np.asarray(fcrits, np.float32))
+    bank = abi.FilterBank(nf, rf.ctypes.data_as(C.POINTER(C.c_double)),
+                          fc.ctypes.data_as(C.POINTER(C.c_float)), int(wsmooth), float(alatt))
+    so = None
+    if smoothed_out is not None:
+        assert len(smoothed_out) == nf
+        for t in smoothed_out:
+            assert t is None or (t.shape == (n, n, n) and t.dtype == torch.float32
+                                 and t.permute(2, 1, 0).is_contiguous())
+        so = (C.c_void_p * nf)(*[0 if t is None else t.data_ptr() for t in smoothed_out])
+    st = ctx.lib.hnt_peak_find_tile(ctx.handle, abi.as_ptr(delta.data_ptr(), C.c_float), n,
+                                    int(nbuff), C.byref(bank),
+                                    abi.as_ptr(mask.data_ptr(), C.c_int8), C.byref(pk),
+                                    abi.as_ptr(d2.data_ptr() if with_lapd else 0, C.c_float),
+                                    so)
+    if st != abi.HNT_ERR_CAPACITY:
+        abi.check(ctx.lib, st)
+    cnt = int(pk.count)
+    res = {k: v[:cnt] for k, v in out.items()}
+    res["d2Rf"] = d2[:cnt]
+    res["total"] = int(pk.total)
+    return res
 
 
 def peak_find_tile_gpu(delta_tile, filters, fcrits, tile_masks_in, xbx, ybx, zbx, alatt,
@@ -265,18 +295,19 @@ def peak_find_tile_gpu(delta_tile, filters, fcrits, tile_masks_in, xbx, ybx, zbx
     """Mirror of peak_find_tile_gpu (src/Huntian.jl:387-412, ext/CUDAExt.jl:2938-3060).
 
     filters: list of (id, fcrit, Rf) in processing order (largest Rf first in the drivers).
-    Smoothing (one rfft, then window + irfft per filter) runs through torch.fft; the peak
-    test runs in libhuntian_hip.  Returns (peaks, Rf, FcRf, d2Rf, mask) with peaks a numpy
+    Smoothing (one rfft, then window + irfft per filter), the peak test and, with
+    ioutshear >= 1, the k^2-Gaussian sample at each peak all run in libhuntian_hip
+    (hnt_peak_find_tile).  Returns (peaks, Rf, FcRf, d2Rf, mask) with peaks a numpy
     structured array of PeakCandidate fields (i, j, k, ipp, x, y, z, delta, Rsmooth).
     Unlike the CUDA method the peak order is deterministic (filter, then k, j, i)."""
     torch = _require_gpu()
     n = int(delta_tile.shape[0])
     d = to_device_field(delta_tile)
-    dk = torch.fft.rfftn(d.permute(2, 1, 0).to(torch.float64))
-    boxsize = n * float(alatt)
-    sm = [tophat_smooth_torch(dk, n, boxsize, float(f[2]), wsmooth) for f in filters]
     mask = to_device_field(tile_masks_in, dtype=torch.int8)
-    res = peak_find(sm, fcrits, int(nbuff), mask, capacity=int(max_peaks))
+    radii = [float(f[2]) for f in filters]
+    res = peak_find_tile(d, radii, fcrits, int(nbuff), mask, alatt=float(alatt),
+                         wsmooth=int(wsmooth), with_lapd=ioutshear >= 1,
+                         capacity=int(max_peaks))
     cnt = len(res["i"])
     i = res["i"].cpu().numpy().astype(np.int64)
     j = res["j"].cpu().numpy().astype(np.int64)
@@ -287,7 +318,7 @@ def peak_find_tile_gpu(delta_tile, filters, fcrits, tile_masks_in, xbx, ybx, zbx
     peaks = np.zeros(cnt, dtype=[("i", "i8"), ("j", "i8"), ("k", "i8"), ("ipp", "i8"),
                                  ("x", "f8"), ("y", "f8"), ("z", "f8"), ("delta", "f4"),
                                  ("Rsmooth", "f8")])
-    Rf = np.array([float(filters[f][2]) for f in fidx], np.float64)
+    Rf = np.array([radii[f] for f in fidx], np.float64)
     peaks["i"], peaks["j"], peaks["k"] = i, j, k
     peaks["ipp"] = i + (j - 1) * n + (k - 1) * n * n
     peaks["x"] = float(xbx) + float(alatt) * (i - cen)
@@ -295,15 +326,7 @@ def peak_find_tile_gpu(delta_tile, filters, fcrits, tile_masks_in, xbx, ybx, zbx
     peaks["z"] = float(zbx) + float(alatt) * (k - cen)
     peaks["delta"] = dl
     peaks["Rsmooth"] = Rf
-    d2Rf = np.zeros(cnt, np.float32)
-    if ioutshear >= 1:
-        for fi, f in enumerate(filters):
-            sel = np.nonzero(fidx == fi)[0]
-            if len(sel):
-                lap = tophat_smooth_torch(dk, n, boxsize, float(f[2]), 3)
-                st = torch.as_tensor(sel, device=lap.device)
-                d2Rf[sel] = lap[res["i"][st].long() - 1, res["j"][st].long() - 1,
-                                res["k"][st].long() - 1].cpu().numpy()
+    d2Rf = res["d2Rf"].cpu().numpy() if ioutshear >= 1 else np.zeros(cnt, np.float32)
     return peaks, Rf, dl.copy(), d2Rf, np.asfortranarray(mask.cpu().numpy())
 
 

@@ -73,7 +73,8 @@ enum {
     HNT_STAGE_SA_PREP = 2,      /* shell analysis: validation, record repack, peak sort */
     HNT_STAGE_SA_KERNEL = 3,    /* shell analysis: gather and radial logic */
     HNT_STAGE_SA_CROSS = 4,     /* shell analysis: crossing pass and work-ordered peak sort */
-    HNT_NSTAGES = 5
+    HNT_STAGE_PF_SMOOTH = 5,    /* tile peak finding: FFTs and window products */
+    HNT_NSTAGES = 6
 };
 hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on);
 hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t nstages);
@@ -139,7 +140,9 @@ typedef struct {
     int32_t* filt;      /* device [capacity], 0-based index into the filter list passed */
     float* delta;       /* device [capacity], smoothed delta at the peak (FcRf) */
     int64_t count;      /* out: peaks written = min(total, capacity) */
-    int64_t total;      /* out: peaks found */
+    int64_t total;      /* out: peak cells found, i.e. the capacity an uncapped call needs.
+                           (The reference leaves a peak beyond max_peaks unmasked, so a later
+                           filter can count its cell again; its counter is not returned.) */
 } hnt_peaks;
 
 /*
@@ -160,6 +163,36 @@ typedef struct {
  */
 hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed, const float* fcrit,
                          int32_t nfilt, int32_t n, int32_t nbuff, int8_t* mask, hnt_peaks* out);
+
+/* Filter bank of hnt_peak_find_tile (the `filters`, `fcrits`, `wsmooth` arguments of
+ * peak_find_tile_gpu, src/Huntian.jl:387-412). */
+typedef struct {
+    int32_t nfilt;          /* 1..254 */
+    const double* Rf;       /* host [nfilt] smoothing radii, Mpc/h, in processing order */
+    const float* fcrit;     /* host [nfilt] peak thresholds */
+    int32_t wsmooth;        /* window: 0 Gaussian exp(-(kR/2)^2/2), 1 top-hat
+                               3(sin x - x cos x)/x^3, 3 k^2 times the Gaussian */
+    double alatt;           /* cell size, Mpc/h: the tile is periodic with box n * alatt */
+} hnt_filter_bank;
+
+/*
+ * hnt_peak_find_tile -- replaces peak_find_tile_gpu (ext/CUDAExt.jl:2938-3060, stub
+ * src/Huntian.jl:387-412): filter-bank smoothing of the tile followed by the peak test of
+ * hnt_peak_find, on the device.  delta (device n^3 Float32, column-major) is transformed once
+ * (real-to-complex, Float32, as the reference's cuFFT rfft); then for each filter in order
+ * the spectrum times W(kR) (W in FP64, rounded to Float32, k = 0 left unchanged,
+ * _smooth_window_kernel! ext/CUDAExt.jl:2831-2871) is transformed back (normalised by 1/n^3)
+ * and its peaks are claimed at once, so one smoothed field is held at a time.  Peaks, order,
+ * mask update and capacity rule are those of hnt_peak_find on the same smoothed fields.
+ * d2Rf:         device [out->capacity] or NULL.  If given, each filter's field is also
+ *               smoothed with window 3 (k^2 Gaussian) and sampled at that filter's peaks
+ *               (the lapd_s of the reference, its d2Rf output).
+ * smoothed_out: host array of nfilt device pointers (n^3 Float32 each; an entry may be
+ *               NULL), or NULL: copies of the smoothed fields, for checking.
+ */
+hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32_t n, int32_t nbuff,
+                              const hnt_filter_bank* bank, int8_t* mask, hnt_peaks* out,
+                              float* d2Rf, float* const* smoothed_out);
 
 /* ---------------------------------------------------------------- shell analysis */
 
