@@ -66,6 +66,8 @@ def main():
                     help="peak-order brick size 2^v cells (library default if omitted)")
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip timing the shell analysis under the other schedules")
+    ap.add_argument("--no-filter-bank", action="store_true",
+                    help="skip timing tile peak finding from the raw tile (smoothing included)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 4 GiB device copy (3 times), the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -221,6 +223,36 @@ def main():
             del r2, res2
             log(f"schedule {o}: {ablation[o]}")
 
+    # ---------------- tile peak finding from the raw tile: filter-bank smoothing (hipFFT
+    # rfft once, window + irfft per filter) fused with the peak test (hnt_peak_find_tile);
+    # reported beside the step, whose peak test reads the pre-smoothed fields
+    tile_pf = None
+    if not args.no_filter_bank:
+        smoothed.clear()  # the shell inputs no longer need them
+        torch.cuda.empty_cache()
+        delta = fields["delta"]
+        mask.zero_()
+        api.peak_find_tile(delta, radii, fc_peak, nbuff, mask, alatt=alatt, capacity=cap)
+        torch.cuda.synchronize()
+        ctx.set_profiling(True)
+        mask.zero_()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ft = api.peak_find_tile(delta, radii, fc_peak, nbuff, mask, alatt=alatt, capacity=cap)
+        e1.record()
+        torch.cuda.synchronize()
+        fprof = ctx.profile()
+        ctx.set_profiling(False)
+        tile_pf = dict(ms=round(e0.elapsed_time(e1), 3), candidates=len(ft["i"]),
+                       stage_ms={k: round(v[0], 3) for k, v in fprof.items() if v[1]},
+                       filters=len(radii),
+                       note="one rfft, then per filter window x spectrum, irfft (hipFFT, "
+                            "Float32) and the peak test; candidates differ from the step's "
+                            "(torch-smoothed input) only by FFT rounding")
+        del ft
+        log(f"tile peak finding: {tile_pf}")
+
     # ---------------- results summary and algorithmic bytes
     RTHL = r["RTHL"][:npk].cpu().numpy()
     used = r["nshells_used"][:npk].cpu().numpy().astype(np.int64)
@@ -327,6 +359,7 @@ def main():
                    "autotune": tuning,
                    "variant": args.variant, "candidates": npk, "halos": nhalo,
                    "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
+                   "tile_peak_find": tile_pf,
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
                    "mean_cells_reference_extent": round(float(ref_cells.mean()), 1)
