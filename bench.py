@@ -223,6 +223,30 @@ def main():
             del r2, res2
             log(f"schedule {o}: {ablation[o]}")
 
+    # ---------------- halo records of the step's results (hnt_pack_records: 33-float
+    # ExtHaloRecord with the 2LPT terms, z = 0), timed once after a warm call
+    from huntian_hip import cosmology
+
+    cos = cosmology.CosmologyParams(0.31, 0.049, 0.69, 0.68)
+    growth = cosmology.dlinear_tables(cos)
+    Rf_pk = Rf_tab[pf["filt"].long()]
+    rkw = dict(alatt=alatt, n=n, ZZon=1.0, ilpt=2, Om=cos.Om, OL=cos.OL, ioutshear=1)
+    recs = api.pack_records(pf["i"], pf["j"], pf["k"], Rf_pk, r, pf["delta"], growth=growth,
+                            **rkw)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    recs = api.pack_records(pf["i"], pf["j"], pf["k"], Rf_pk, r, pf["delta"], growth=growth,
+                            **rkw)
+    e1.record()
+    torch.cuda.synchronize()
+    records = dict(ms=round(e0.elapsed_time(e1), 3), halos=int(recs.shape[0]),
+                   floats_per_record=int(recs.shape[1]),
+                   note="hnt_pack_records over the step's results: flags, scan, scatter")
+    del recs
+    log(f"records: {records}")
+
     # ---------------- tile peak finding from the raw tile: filter-bank smoothing (hipFFT
     # rfft once, window + irfft per filter) fused with the peak test (hnt_peak_find_tile);
     # reported beside the step, whose peak test reads the pre-smoothed fields
@@ -360,6 +384,7 @@ def main():
                    "variant": args.variant, "candidates": npk, "halos": nhalo,
                    "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
                    "tile_peak_find": tile_pf,
+                   "records": records,
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
                    "mean_cells_reference_extent": round(float(ref_cells.mean()), 1)

@@ -319,7 +319,8 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
                       &ctx->mlp.z, &ctx->mlp.h, &ctx->mlp.y, &ctx->mlp.dy, &ctx->mlp.dyf,
                       &ctx->mlp.dz, &ctx->mlp.gw, &ctx->mlp.part, &ctx->mlp.gradv,
                       &ctx->fb_spec, &ctx->fb_work, &ctx->fb_field, &ctx->fb_lap,
-                      &ctx->fb_val, &ctx->fb_d2};
+                      &ctx->fb_val, &ctx->fb_d2, &ctx->rec_tabs, &ctx->rec_keep,
+                      &ctx->rec_slot};
     for (DevBuf* b : bufs) release(*b);
     release_fft(ctx);
     for (std::vector<hipEvent_t>& v : ctx->ev)

@@ -95,6 +95,7 @@ struct hnt_ctx {
     hnt::DevBuf tiles, tilemeta, slots, gtiles;  // multi-tile shell analysis
     // tile peak finding: spectrum, windowed spectrum, smoothed fields, per-cell peak values
     hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2;
+    hnt::DevBuf rec_tabs, rec_keep, rec_slot;  // halo records: tables, flags, slots
     void* fft_r2c = nullptr;  // hipfftHandle of an n^3 real-to-complex plan
     void* fft_c2r = nullptr;
     int fft_n = 0;

@@ -116,6 +116,36 @@ class FilterBank(C.Structure):
     ]
 
 
+class CosmoTables(C.Structure):
+    _fields_ = [
+        ("ntab", C.c_int32),
+        ("atab", _f64p),
+        ("D_atab", _f64p),
+        ("nchi", C.c_int32),
+        ("chi2z_z", _f64p),
+        ("chi2z_chi", _f64p),
+        ("chi2z_zmax", C.c_double),
+    ]
+
+
+class PackParams(C.Structure):
+    _fields_ = [
+        ("alatt", C.c_double),
+        ("n", C.c_int32),
+        ("xbx", C.c_double),
+        ("ybx", C.c_double),
+        ("zbx", C.c_double),
+        ("ZZon", C.c_double),
+        ("ievol", C.c_int32),
+        ("obs", C.c_double * 3),
+        ("z_max", C.c_double),
+        ("ilpt", C.c_int32),
+        ("Om", C.c_double),
+        ("OL", C.c_double),
+        ("ioutshear", C.c_int32),
+    ]
+
+
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS = 18, 128, 3
 MLP_NPARAM = 18 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 3 + 3
 HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
@@ -201,6 +231,7 @@ EXPORTED = (
     "hnt_plan_create",
     "hnt_plan_launch",
     "hnt_plan_destroy",
+    "hnt_pack_records",
     "hnt_shell_gather_full",
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
@@ -253,6 +284,9 @@ def load(path: str | None = None):
                                   _i8p, C.POINTER(Peaks)]
     lib.hnt_peak_find_tile.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.POINTER(FilterBank),
                                        _i8p, C.POINTER(Peaks), _f32p, C.POINTER(vp)]
+    lib.hnt_pack_records.argtypes = [vp, C.POINTER(Batch), C.POINTER(Results), _f32p, _f32p,
+                                     C.POINTER(CosmoTables), C.POINTER(PackParams), _f32p,
+                                     C.c_int64, C.POINTER(C.c_int64)]
     lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                       C.POINTER(ShellTables), C.POINTER(CTab),
                                       C.POINTER(Params), C.POINTER(Results), _i8p]

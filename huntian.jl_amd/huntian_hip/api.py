@@ -525,6 +525,68 @@ def _host_results(r, npk):
     return out
 
 
+def cosmo_tables_struct(growth, chi2z=None):
+    """abi.CosmoTables over host arrays: growth = (atab, D_atab[, dlnD]) from
+    cosmology.dlinear_tables; chi2z = (zs, chis, z_max) from cosmology.build_chi_to_z or
+    None.  Returns (struct, arrays to keep alive)."""
+    at = np.ascontiguousarray(np.asarray(growth[0], np.float64))
+    Dt = np.ascontiguousarray(np.asarray(growth[1], np.float64))
+    keep = [at, Dt]
+    f64 = C.POINTER(C.c_double)
+    if chi2z is not None:
+        zs = np.ascontiguousarray(np.asarray(chi2z[0], np.float64))
+        ch = np.ascontiguousarray(np.asarray(chi2z[1], np.float64))
+        keep += [zs, ch]
+        t = abi.CosmoTables(len(at), at.ctypes.data_as(f64), Dt.ctypes.data_as(f64), len(zs),
+                            zs.ctypes.data_as(f64), ch.ctypes.data_as(f64), float(chi2z[2]))
+    else:
+        t = abi.CosmoTables(len(at), at.ctypes.data_as(f64), Dt.ctypes.data_as(f64), 0,
+                            None, None, 0.0)
+    return t, keep
+
+
+def pack_params(*, alatt, n, xbx=0.0, ybx=0.0, zbx=0.0, ZZon=1.0, ievol=0, obs=(0.0, 0.0, 0.0),
+                z_max=0.0, ilpt=2, Om=0.31, OL=0.69, ioutshear=0):
+    return abi.PackParams(float(alatt), int(n), float(xbx), float(ybx), float(zbx), float(ZZon),
+                          int(ievol), (C.c_double * 3)(*[float(v) for v in obs]), float(z_max),
+                          int(ilpt), float(Om), float(OL), int(ioutshear))
+
+
+def pack_records(pi, pj, pk, Rf, results: dict, FcRf, *, growth, chi2z=None, d2Rf=None,
+                 capacity: int | None = None, **params):
+    """Halo records of one tile on the device (hnt_pack_records; the record loop of
+    src/MultiResolution.jl:1159-1219, layouts of src/IO/Catalog.jl).
+
+    pi, pj, pk, Rf: the peaks of the shell analysis (device or host); results: its result
+    dict (device FP64 tensors, as from shell_analyse); FcRf / d2Rf: float32 per peak (the
+    peak finder's delta and d2Rf).  growth / chi2z: host tables (cosmology module).
+    params: alatt, n, xbx, ybx, zbx, ZZon, ievol, obs, z_max, ilpt, Om, OL, ioutshear.
+    Returns a device float32 tensor (nhalo, 11 or 33) in peak order."""
+    torch = _require_gpu()
+    dev = results["RTHL"].device
+    ctx = context(dev.index or 0)
+    npk = int(len(pi))
+    b, keep = _batch_struct(pi, pj, pk, Rf, 0, 1.0, 1.0, None, None, dev)
+    fc = _dev1(FcRf, torch.float32, dev)
+    d2 = _dev1(d2Rf, torch.float32, dev) if d2Rf is not None else None
+    nf = 33 if int(params.get("ioutshear", 0)) >= 1 else 11
+    if capacity is None:
+        capacity = npk
+    out = torch.empty((max(int(capacity), 1), nf), dtype=torch.float32, device=dev)
+    t, tkeep = cosmo_tables_struct(growth, chi2z)
+    pp = pack_params(**params)
+    cnt = C.c_int64(0)
+    st = ctx.lib.hnt_pack_records(ctx.handle, C.byref(b), C.byref(_results_struct(results)),
+                                  abi.as_ptr(_ptr(fc), C.c_float),
+                                  abi.as_ptr(_ptr(d2), C.c_float), C.byref(t), C.byref(pp),
+                                  abi.as_ptr(out.data_ptr(), C.c_float), int(capacity),
+                                  C.byref(cnt))
+    if st != abi.HNT_ERR_CAPACITY:
+        abi.check(ctx.lib, st)
+    del keep, tkeep
+    return out[:min(int(cnt.value), int(capacity))]
+
+
 def analyse_peaks_gpu_cuda_multirf(delta, etax, etay, etaz, eta2x, eta2y, eta2z, batches,
                                    stab: ShellTables, ct_table, X1, X2, Y1, Y2, Z1, Z2, alatt,
                                    ZZon, *, fcrit_override=None, ct=None, rmax2rs=0.0,

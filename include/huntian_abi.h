@@ -314,6 +314,58 @@ hnt_status hnt_plan_create(hnt_ctx* ctx, int32_t ntiles, const hnt_fields* field
 hnt_status hnt_plan_launch(hnt_plan* plan);
 hnt_status hnt_plan_destroy(hnt_plan* plan);
 
+/* ---------------------------------------------------------------- halo records */
+
+/* Cosmology tables of hnt_pack_records: host arrays the caller builds once, as the reference
+ * driver does (src/MultiResolution.jl:721,743).
+ *  growth: Dlinear_tables (src/Cosmology/Cosmology.jl:161-284): ntab ascending scale factors
+ *          atab and D(a)/a in D_atab, normalised to D(1) = 1;
+ *  chi2z:  build_chi_to_z (:343-350): nchi redshifts (uniform from 0 to chi2z_zmax) and their
+ *          comoving distances, Mpc/h; needed in lightcone mode only (nchi = 0 otherwise). */
+typedef struct {
+    int32_t ntab;
+    const double* atab;
+    const double* D_atab;
+    int32_t nchi;
+    const double* chi2z_z;
+    const double* chi2z_chi;
+    double chi2z_zmax;
+} hnt_cosmo_tables;
+
+/* What the record loop of the tile driver knows (src/MultiResolution.jl:1159-1219). */
+typedef struct {
+    double alatt;           /* cell size, Mpc/h */
+    int32_t n;              /* tile side; peak position x = xbx + alatt (i - (n + 1)/2) */
+    double xbx, ybx, zbx;   /* tile centre, Mpc/h */
+    double ZZon;            /* 1 + z of the output (ievol == 0) */
+    int32_t ievol;          /* 1: lightcone, each peak's z from its distance to obs */
+    double obs[3];          /* observer, Mpc/h */
+    double z_max;           /* lightcone: a peak with z > z_max gives no record */
+    int32_t ilpt;           /* >= 2: 2LPT terms vx2, vy2, vz2 (else 0) */
+    double Om, OL;          /* Omega_m, Omega_Lambda of omega_m_a */
+    int32_t ioutshear;      /* 0: 11-float HaloRecord, >= 1: 33-float ExtHaloRecord */
+} hnt_pack_params;
+
+/*
+ * hnt_pack_records -- replaces the record loop of the tile driver after the shell analysis
+ * (src/MultiResolution.jl:1159-1219) in the HaloRecord / ExtHaloRecord layout of
+ * src/IO/Catalog.jl:4-46, on the device.  For every peak p of `batch` (pi, pj, pk and Rf
+ * are read) whose results->RTHL[p] > 0 (and, in lightcone mode, whose z <= z_max), in peak
+ * order: D = Dlinear_ab(1/ZZon_p) (4-point Lagrange in the growth table), the record
+ *   x y z, Sbar*D, RTHL*alatt, Sbar2*(-3/7 Omega_m(a)^(-1/143) D^2) (ilpt >= 2, else 0),
+ *   Fbarx  [+ e_v p_v, strain 11 22 33 23 13 12, d2F, zvir_half, gradpk, gradpkf, Rf,
+ *   FcRf, d2Rf, gradpkrf]
+ * in Float32, from the Float32 values of the results as the reference's GPU results are.
+ * records: device [capacity * (11 or 33)] Float32; *count: records of the tile.  FcRf:
+ * device [npeaks] (the smoothed delta at each peak, hnt_peaks.delta); d2Rf: device [npeaks]
+ * or NULL (zeros).  If *count > capacity the first `capacity` records are written and
+ * HNT_ERR_CAPACITY is returned.
+ */
+hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch, const hnt_results* results,
+                            const float* FcRf, const float* d2Rf, const hnt_cosmo_tables* tabs,
+                            const hnt_pack_params* pp, float* records, int64_t capacity,
+                            int64_t* count);
+
 /* ---------------------------------------------------------------- primitive entries */
 /* Single-primitive entry points; the reference exposes them as test harnesses
  * (src/Huntian.jl:98-217). */

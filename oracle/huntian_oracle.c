@@ -1002,3 +1002,136 @@ int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_t
     }
     return bad;
 }
+
+/* ------------------------------------------------------------------ halo records */
+
+/* _hunt (Cosmology.jl:116-130), 1-based jlo over xx[0..n) */
+static int hunt_1(const double* xx, int n, double x) {
+    int ascnd = xx[n - 1] > xx[0];
+    int jlo = 0, jhi = n + 1;
+    while (jhi - jlo > 1) {
+        int jm = (jhi + jlo) / 2;
+        if ((x > xx[jm - 1]) == ascnd) jlo = jm;
+        else jhi = jm;
+    }
+    return jlo;
+}
+
+/* Dlinear_ab(ai)[1] (Cosmology.jl:287-307) with _lagrint_2col's first column (:88-110);
+ * Julia indexes xa = atab[kkl:kkl+nlint-1] (1-based). */
+double hro_growth_D(const hnt_cosmo_tables* t, double ai) {
+    const double* at = t->atab;
+    const double* Dt = t->D_atab;
+    int ntab = t->ntab;
+    if (ai > at[0] && ai < at[ntab - 1]) {
+        int kk = hunt_1(at, ntab, ai);
+        int nlint, kkl;
+        if (kk < 2 || kk >= ntab) {
+            nlint = 2;
+            kkl = kk - 1 > 1 ? kk - 1 : 1;
+        } else {
+            nlint = 4;
+            kkl = kk - 2;
+        }
+        if (kkl < 1) return NAN; /* the reference's index would fall before its table */
+        double xa1 = at[kkl - 1], xa2 = at[kkl], x = ai;
+        double D_a;
+        if (nlint == 4) {
+            double xa3 = at[kkl + 1], xa4 = at[kkl + 2];
+            double a1 = (x - xa2) / (xa1 - xa2) * (x - xa3) / (xa1 - xa3) * (x - xa4) / (xa1 - xa4);
+            double a2 = (x - xa1) / (xa2 - xa1) * (x - xa3) / (xa2 - xa3) * (x - xa4) / (xa2 - xa4);
+            double a3 = (x - xa1) / (xa3 - xa1) * (x - xa2) / (xa3 - xa2) * (x - xa4) / (xa3 - xa4);
+            double a4 = (x - xa1) / (xa4 - xa1) * (x - xa2) / (xa4 - xa2) * (x - xa3) / (xa4 - xa3);
+            D_a = a1 * Dt[kkl - 1] + a2 * Dt[kkl] + a3 * Dt[kkl + 1] + a4 * Dt[kkl + 2];
+        } else {
+            double a1 = (x - xa2) / (xa1 - xa2);
+            double a2 = 1.0 - a1;
+            D_a = a1 * Dt[kkl - 1] + a2 * Dt[kkl];
+        }
+        return D_a * ai;
+    } else if (ai >= at[ntab - 1]) {
+        return Dt[ntab - 1] * ai;
+    }
+    return Dt[0] * ai;
+}
+
+/* chi_to_z (Cosmology.jl:357-372) */
+double hro_chi_to_z(const hnt_cosmo_tables* t, double chi_val) {
+    const double* zs = t->chi2z_z;
+    const double* chis = t->chi2z_chi;
+    int n = t->nchi;
+    if (chi_val <= chis[0]) return 0.0;
+    if (chi_val >= chis[n - 1]) return t->chi2z_zmax;
+    int lo = 1, hi = n;
+    while (hi - lo > 1) {
+        int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+        if (chis[mid - 1] <= chi_val) lo = mid;
+        else hi = mid;
+    }
+    double frac = (chi_val - chis[lo - 1]) / (chis[hi - 1] - chis[lo - 1]);
+    return zs[lo - 1] + frac * (zs[hi - 1] - zs[lo - 1]);
+}
+
+int64_t hro_pack_records(int64_t npeaks, const int32_t* pi, const int32_t* pj,
+                         const int32_t* pk, const double* Rf, const hnt_results* r,
+                         const float* FcRf, const float* d2Rf, const hnt_cosmo_tables* t,
+                         const hnt_pack_params* pp, float* out, int64_t capacity) {
+    const int nf = pp->ioutshear >= 1 ? 33 : 11;
+    const double coef2_base = -3.0 / 7.0;
+    const double cen1 = 0.5 * (pp->n + 1);  /* peak_find_tile_gpu cell centres */
+    int64_t nrec = 0;
+    for (int64_t idx = 0; idx < npeaks; ++idx) {
+        float RTHL32 = (float)r->RTHL[idx];
+        if (RTHL32 <= 0.0f) continue;
+        double px = pp->xbx + pp->alatt * ((double)pi[idx] - cen1);
+        double py = pp->ybx + pp->alatt * ((double)pj[idx] - cen1);
+        double pz = pp->zbx + pp->alatt * ((double)pk[idx] - cen1);
+        double ZZon_pk = pp->ZZon;
+        if (pp->ievol == 1) {
+            double dx = px - pp->obs[0], dy = py - pp->obs[1], dz = pz - pp->obs[2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double z_pk = hro_chi_to_z(t, rr);
+            ZZon_pk = 1.0 + z_pk;
+            if (z_pk > pp->z_max) continue;
+        }
+        double a_pk = 1.0 / ZZon_pk;
+        double D_pk = hro_growth_D(t, a_pk);
+        float D32 = (float)D_pk;
+        float RTHL_phys = (float)((double)RTHL32 * pp->alatt);
+        float S1 = (float)r->Sbar[idx * 3] * D32, S2 = (float)r->Sbar[idx * 3 + 1] * D32,
+              S3 = (float)r->Sbar[idx * 3 + 2] * D32;
+        float T1 = 0.0f, T2 = 0.0f, T3 = 0.0f;
+        if (pp->ilpt >= 2) {
+            double a3 = a_pk * a_pk * a_pk;
+            double Om_a = (pp->Om / a3) / (pp->Om / a3 + pp->OL);
+            float coef = (float)(coef2_base * pow(Om_a, -1.0 / 143) * (D_pk * D_pk));
+            T1 = (float)r->Sbar2[idx * 3] * coef;
+            T2 = (float)r->Sbar2[idx * 3 + 1] * coef;
+            T3 = (float)r->Sbar2[idx * 3 + 2] * coef;
+        }
+        if (nrec < capacity) {
+            float* o = out + nrec * nf;
+            o[0] = (float)px; o[1] = (float)py; o[2] = (float)pz;
+            o[3] = S1; o[4] = S2; o[5] = S3;
+            o[6] = RTHL_phys;
+            o[7] = T1; o[8] = T2; o[9] = T3;
+            o[10] = (float)r->Fbarx[idx];
+            if (nf == 33) {
+                const double* st = r->strain + idx * 9;
+                o[11] = (float)r->e_v[idx]; o[12] = (float)r->p_v[idx];
+                o[13] = (float)st[0]; o[14] = (float)st[4]; o[15] = (float)st[8];
+                o[16] = (float)st[5]; o[17] = (float)st[2]; o[18] = (float)st[1];
+                o[19] = (float)r->d2F[idx];
+                o[20] = (float)r->zvir_half[idx];
+                for (int L = 0; L < 3; ++L) o[21 + L] = (float)r->gradpk[idx * 3 + L];
+                for (int L = 0; L < 3; ++L) o[24 + L] = (float)r->gradpkf[idx * 3 + L];
+                o[27] = (float)Rf[idx];
+                o[28] = FcRf[idx];
+                o[29] = d2Rf ? d2Rf[idx] : 0.0f;
+                for (int L = 0; L < 3; ++L) o[30 + L] = (float)r->gradpkrf[idx * 3 + L];
+            }
+        }
+        nrec++;
+    }
+    return nrec;
+}

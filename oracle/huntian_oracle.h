@@ -60,6 +60,19 @@ int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_t
                       const hnt_ctab* ct, const hnt_params* p, hnt_results* r, int8_t* mask,
                       int32_t nthreads);
 
+/* The record loop of the tile driver after the shell analysis (MultiResolution.jl:1159-1219)
+ * with the layouts of src/IO/Catalog.jl:4-46, Dlinear_ab, omega_m_a, chi_to_z and
+ * peak_redshift (src/Cosmology/Cosmology.jl:50,70-130,287-307,357-385).  Inputs are host
+ * arrays: the 1-based peak cells, Rf, the FP64 results (used through their Float32 values,
+ * as the reference's GPU results are Float32), FcRf, d2Rf (or NULL).  Writes the records of
+ * the first `capacity` halos and returns the number of halos. */
+int64_t hro_pack_records(int64_t npeaks, const int32_t* pi, const int32_t* pj,
+                         const int32_t* pk, const double* Rf, const hnt_results* r,
+                         const float* FcRf, const float* d2Rf, const hnt_cosmo_tables* t,
+                         const hnt_pack_params* pp, float* out, int64_t capacity);
+double hro_growth_D(const hnt_cosmo_tables* t, double a);   /* Dlinear_ab(a)[1] */
+double hro_chi_to_z(const hnt_cosmo_tables* t, double chi);  /* chi_to_z */
+
 #ifdef __cplusplus
 }
 #endif

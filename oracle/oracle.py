@@ -73,6 +73,15 @@ def lib():
                                         C.POINTER(abi.Params), C.POINTER(abi.Results), _i8p,
                                         C.c_int32]
         L.hro_analyse_peaks.restype = C.c_int
+        L.hro_pack_records.argtypes = [C.c_int64, _i32p, _i32p, _i32p, _f64p,
+                                       C.POINTER(abi.Results), _f32p, _f32p,
+                                       C.POINTER(abi.CosmoTables), C.POINTER(abi.PackParams),
+                                       _f32p, C.c_int64]
+        L.hro_pack_records.restype = C.c_int64
+        L.hro_growth_D.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
+        L.hro_growth_D.restype = C.c_double
+        L.hro_chi_to_z.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
+        L.hro_chi_to_z.restype = C.c_double
         _lib = L
     return _lib
 
@@ -278,3 +287,58 @@ def analyse_peaks(flds: dict, pi, pj, pk, Rf, ir2min, t: dict, ct: dict, *, alat
         raise ValueError(f"oracle analyse_peaks failed with code {st}")
     r["strain"] = np.swapaxes(r["strain"], 1, 2)  # -> [p, L, K]
     return r
+
+
+# ---------------------------------------------------------------- halo records
+
+def cosmo_tables(growth, chi2z=None):
+    """abi.CosmoTables over host arrays (kept alive in the returned list)."""
+    at = np.ascontiguousarray(growth[0], np.float64)
+    Dt = np.ascontiguousarray(growth[1], np.float64)
+    keep = [at, Dt]
+    if chi2z is None:
+        return abi.CosmoTables(len(at), _p(at, C.c_double), _p(Dt, C.c_double), 0, None, None,
+                               0.0), keep
+    zs = np.ascontiguousarray(chi2z[0], np.float64)
+    ch = np.ascontiguousarray(chi2z[1], np.float64)
+    keep += [zs, ch]
+    return abi.CosmoTables(len(at), _p(at, C.c_double), _p(Dt, C.c_double), len(zs),
+                           _p(zs, C.c_double), _p(ch, C.c_double), float(chi2z[2])), keep
+
+
+def growth_D(growth, a):
+    t, keep = cosmo_tables(growth)
+    return lib().hro_growth_D(C.byref(t), float(a))
+
+
+def chi_to_z(chi2z, chi, growth=((0.1, 0.2, 0.3, 0.4), (1.0, 1.0, 1.0, 1.0))):
+    t, keep = cosmo_tables(growth, chi2z)
+    return lib().hro_chi_to_z(C.byref(t), float(chi))
+
+
+def pack_records(pi, pj, pk, Rf, r: dict, FcRf, *, growth, chi2z=None, d2Rf=None,
+                 capacity=None, alatt, n, xbx=0.0, ybx=0.0, zbx=0.0, ZZon=1.0, ievol=0,
+                 obs=(0.0, 0.0, 0.0), z_max=0.0, ilpt=2, Om=0.31, OL=0.69, ioutshear=0):
+    """The record loop of MultiResolution.jl:1159-1219 on host arrays; r holds FP64 results
+    (strain symmetric).  Returns (records (min(nhalo, capacity), 11 or 33) float32, nhalo)."""
+    npk = len(pi)
+    pi = np.ascontiguousarray(pi, np.int32)
+    pj = np.ascontiguousarray(pj, np.int32)
+    pk = np.ascontiguousarray(pk, np.int32)
+    Rf = np.ascontiguousarray(np.broadcast_to(np.asarray(Rf, np.float64), (npk,)))
+    fc = np.ascontiguousarray(FcRf, np.float32)
+    d2 = np.ascontiguousarray(d2Rf, np.float32) if d2Rf is not None else None
+    rr = {k: np.ascontiguousarray(v) if v is not None else None for k, v in r.items()}
+    rs = results_struct(rr)
+    t, keep = cosmo_tables(growth, chi2z)
+    pp = abi.PackParams(float(alatt), int(n), float(xbx), float(ybx), float(zbx), float(ZZon),
+                        int(ievol), (C.c_double * 3)(*[float(v) for v in obs]), float(z_max),
+                        int(ilpt), float(Om), float(OL), int(ioutshear))
+    nf = 33 if ioutshear >= 1 else 11
+    cap = npk if capacity is None else int(capacity)
+    out = np.zeros((max(cap, 1), nf), np.float32)
+    nrec = lib().hro_pack_records(npk, _p(pi, C.c_int32), _p(pj, C.c_int32), _p(pk, C.c_int32),
+                                  _p(Rf, C.c_double), C.byref(rs), _p(fc, C.c_float),
+                                  _p(d2, C.c_float), C.byref(t), C.byref(pp),
+                                  _p(out, C.c_float), cap)
+    return out[:min(nrec, cap)], int(nrec)

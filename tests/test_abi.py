@@ -34,7 +34,9 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_the_header(tmp_path):
     structs = {"hnt_shell_tables": abi.ShellTables, "hnt_ctab": abi.CTab,
                "hnt_peaks": abi.Peaks, "hnt_fields": abi.Fields, "hnt_batch": abi.Batch,
-               "hnt_params": abi.Params, "hnt_results": abi.Results, "hnt_mlp": abi.Mlp}
+               "hnt_params": abi.Params, "hnt_results": abi.Results, "hnt_mlp": abi.Mlp,
+               "hnt_filter_bank": abi.FilterBank, "hnt_cosmo_tables": abi.CosmoTables,
+               "hnt_pack_params": abi.PackParams}
     prog = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
             'int main(void) {']
     for cname, py in structs.items():

@@ -1,0 +1,90 @@
+"""hnt_pack_records against the oracle's record loop on the results of a realistic tile
+(64^3 Gaussian field, filter bank, peak finding, shell analysis on the GPU): every record
+field equal bit for bit, except the three 2LPT terms, whose factor goes through pow() --
+device and host pow may differ in the last place, so those are held to 1 Float32 ulp."""
+import numpy as np
+import pytest
+
+from huntian_hip import api, cosmology, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tile(websky):
+    import torch
+
+    n, box, nbuff, nhunt = 64, 128.0, 11, 10
+    alatt = box / n
+    delta, dk = synth.gaussian_field(n, box, 6060)
+    f = {"delta": delta}
+    f.update(synth.lpt_fields(dk, n, box))
+    radii = synth.filter_radii(1.5, 16.0, 8)
+    sm = [api.to_device_field(synth.smooth_tophat(dk, n, box, R)) for R in radii]
+    mask = api.to_device_field(np.zeros((n, n, n), np.int8), dtype=torch.int8)
+    pf = api.peak_find(sm, [1.5] * len(radii), nbuff, mask, capacity=n ** 3)
+    Rf = torch.tensor(radii, dtype=torch.float64, device="cuda")[pf["filt"].long()]
+    ir2 = torch.tensor([api.ir2min_of(R, alatt) for R in radii], dtype=torch.int32,
+                       device="cuda")[pf["filt"].long()]
+    mask.zero_()
+    r = api.shell_analyse(f, pf["i"], pf["j"], pf["k"], Rf, ir2, api.build_shell_tables(nhunt),
+                          websky, alatt=alatt, ZZon=1.0, fcrit=api.fsc_of_z(websky, 0.0),
+                          nbuff=nbuff, mask=mask)
+    assert (r["RTHL"] > 0).sum().item() > 30
+    d2 = torch.randn(len(pf["i"]), device="cuda")
+    return dict(n=n, alatt=alatt, pf=pf, Rf=Rf, r=r, d2=d2)
+
+
+COSMO = cosmology.CosmologyParams(0.31, 0.049, 0.69, 0.68)
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return cosmology.dlinear_tables(COSMO), cosmology.build_chi_to_z(COSMO, z_max=1.0, npts=500)
+
+
+@pytest.mark.parametrize("ioutshear,ilpt,ievol", [(0, 1, 0), (1, 2, 0), (1, 2, 1), (0, 2, 1)])
+def test_records_match_the_oracle_loop(oracle, tile, tables, ioutshear, ilpt, ievol):
+    growth, chi2z = tables
+    pf, r = tile["pf"], tile["r"]
+    kw = dict(alatt=tile["alatt"], n=tile["n"], xbx=180.0, ybx=-40.0, zbx=25.0, ZZon=1.25,
+              ievol=ievol, obs=(60.0, 0.0, 0.0), z_max=0.038, ilpt=ilpt, Om=COSMO.Om,
+              OL=COSMO.OL, ioutshear=ioutshear)
+    got = api.pack_records(pf["i"], pf["j"], pf["k"], tile["Rf"], r, pf["delta"],
+                           growth=growth, chi2z=chi2z if ievol else None, d2Rf=tile["d2"], **kw)
+    host = {k: v.cpu().numpy() for k, v in r.items()}
+    ref, nh = oracle.pack_records(pf["i"].cpu().numpy(), pf["j"].cpu().numpy(),
+                                  pf["k"].cpu().numpy(), tile["Rf"].cpu().numpy(), host,
+                                  pf["delta"].cpu().numpy(), growth=growth,
+                                  chi2z=chi2z if ievol else None,
+                                  d2Rf=tile["d2"].cpu().numpy(), **kw)
+    g = got.cpu().numpy()
+    assert g.shape == ref.shape == (nh, 33 if ioutshear else 11)
+    halos = int((host["RTHL"] > 0).sum())
+    if ievol:
+        assert 0 < nh < halos  # the lightcone cut drops some halos
+    else:
+        assert nh == halos
+    exact = [q for q in range(g.shape[1]) if q not in (7, 8, 9)]
+    np.testing.assert_array_equal(g[:, exact], ref[:, exact])
+    ulp = np.spacing(np.abs(ref[:, 7:10]).astype(np.float32))
+    assert np.all(np.abs(g[:, 7:10] - ref[:, 7:10]) <= ulp)
+    if ilpt < 2:
+        assert np.all(g[:, 7:10] == 0)
+
+
+def test_records_capacity_and_errors(oracle, tile, tables):
+    from huntian_hip.abi import HuntianError
+
+    growth, _ = tables
+    pf, r = tile["pf"], tile["r"]
+    kw = dict(alatt=tile["alatt"], n=tile["n"], ZZon=1.0, ilpt=2, Om=COSMO.Om, OL=COSMO.OL)
+    full = api.pack_records(pf["i"], pf["j"], pf["k"], tile["Rf"], r, pf["delta"],
+                            growth=growth, **kw)
+    cut = api.pack_records(pf["i"], pf["j"], pf["k"], tile["Rf"], r, pf["delta"],
+                           growth=growth, capacity=10, **kw)
+    assert cut.shape[0] == 10 and full.shape[0] > 10
+    assert np.array_equal(cut.cpu().numpy(), full[:10].cpu().numpy())
+    with pytest.raises(HuntianError):  # z beyond the growth table's interpolation range
+        api.pack_records(pf["i"], pf["j"], pf["k"], tile["Rf"], r, pf["delta"],
+                         growth=growth, **dict(kw, ZZon=5000.0))
