@@ -130,59 +130,78 @@ __device__ double record_ZZon(const RecArgs& a, int64_t p, const PeakPos& q) {
     return 1.0 + z;
 }
 
-__global__ __launch_bounds__(256) void rec_flag_kernel(RecArgs a, int32_t* __restrict__ keep) {
+constexpr int REC_BLOCK = 256;
+
+__global__ __launch_bounds__(REC_BLOCK) void rec_flag_kernel(RecArgs a,
+                                                             int32_t* __restrict__ keep) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.npeaks) return;
     keep[p] = record_ZZon(a, p, peak_position(a, p)) > 0.0 ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void rec_write_kernel(RecArgs a,
-                                                        const int32_t* __restrict__ keep,
-                                                        const int64_t* __restrict__ slot,
-                                                        float* __restrict__ out,
-                                                        int64_t capacity) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npeaks || !keep[p] || slot[p] >= capacity) return;
-    const PeakPos q = peak_position(a, p);
-    const double ZZon = record_ZZon(a, p, q);
-    const double apk = 1.0 / ZZon;
-    const double D = growth_D(a.atab, a.Dtab, a.ntab, apk);
-    const float Df = (float)D;
-    const hnt_results& R = a.res;
-    float* o = out + slot[p] * a.nf;
-    o[0] = (float)q.x;
-    o[1] = (float)q.y;
-    o[2] = (float)q.z;
-    for (int L = 0; L < 3; ++L) o[3 + L] = (float)R.Sbar[p * 3 + L] * Df;
-    o[6] = (float)((double)(float)R.RTHL[p] * a.pp.alatt);
-    float coef = 0.0f;
-    if (a.pp.ilpt >= 2) {
-        // omega_m_a (Cosmology.jl:50); coef = -3/7 Om_a^(-1/143) D^2 (MultiResolution.jl:1188)
-        const double a3 = apk * apk * apk;
-        const double Om_a = (a.pp.Om / a3) / (a.pp.Om / a3 + a.pp.OL);
-        coef = (float)(-3.0 / 7.0 * pow(Om_a, -1.0 / 143) * (D * D));
+// One record per peak, built in LDS at the peak's rank among the workgroup's kept peaks, then
+// the workgroup's records -- consecutive in the output -- are written as one contiguous run
+// (the per-peak rows would otherwise be 44- or 132-byte strided stores).
+
+__global__ __launch_bounds__(REC_BLOCK) void rec_write_kernel(RecArgs a,
+                                                              const int32_t* __restrict__ keep,
+                                                              const int64_t* __restrict__ slot,
+                                                              float* __restrict__ out,
+                                                              int64_t capacity) {
+    __shared__ float rows[REC_BLOCK * 33];
+    const int64_t p0 = (int64_t)blockIdx.x * REC_BLOCK;
+    const int64_t p = p0 + threadIdx.x;
+    const int64_t s0 = slot[p0];  // slot of the workgroup's first kept peak
+    const int64_t pe = min(p0 + REC_BLOCK, a.npeaks) - 1;
+    const int64_t nkeep = slot[pe] + keep[pe] - s0;
+    if (p < a.npeaks && keep[p]) {
+        const PeakPos q = peak_position(a, p);
+        const double ZZon = record_ZZon(a, p, q);
+        const double apk = 1.0 / ZZon;
+        const double D = growth_D(a.atab, a.Dtab, a.ntab, apk);
+        const float Df = (float)D;
+        const hnt_results& R = a.res;
+        float* o = rows + (slot[p] - s0) * a.nf;
+        o[0] = (float)q.x;
+        o[1] = (float)q.y;
+        o[2] = (float)q.z;
+        for (int L = 0; L < 3; ++L) o[3 + L] = (float)R.Sbar[p * 3 + L] * Df;
+        o[6] = (float)((double)(float)R.RTHL[p] * a.pp.alatt);
+        float coef = 0.0f;
+        if (a.pp.ilpt >= 2) {
+            // omega_m_a (Cosmology.jl:50); coef = -3/7 Om_a^(-1/143) D^2 (MultiResolution.jl:1188)
+            const double a3 = apk * apk * apk;
+            const double Om_a = (a.pp.Om / a3) / (a.pp.Om / a3 + a.pp.OL);
+            coef = (float)(-3.0 / 7.0 * pow(Om_a, -1.0 / 143) * (D * D));
+        }
+        for (int L = 0; L < 3; ++L)
+            o[7 + L] = a.pp.ilpt >= 2 ? (float)R.Sbar2[p * 3 + L] * coef : 0.0f;
+        o[10] = (float)R.Fbarx[p];
+        if (a.nf == 33) {
+            const double* st = R.strain + p * 9;  // symmetric 3x3
+            o[11] = (float)R.e_v[p];
+            o[12] = (float)R.p_v[p];
+            o[13] = (float)st[0];  // 11
+            o[14] = (float)st[4];  // 22
+            o[15] = (float)st[8];  // 33
+            o[16] = (float)st[5];  // 23
+            o[17] = (float)st[2];  // 13
+            o[18] = (float)st[1];  // 12
+            o[19] = (float)R.d2F[p];
+            o[20] = (float)R.zvir_half[p];
+            for (int L = 0; L < 3; ++L) o[21 + L] = (float)R.gradpk[p * 3 + L];
+            for (int L = 0; L < 3; ++L) o[24 + L] = (float)R.gradpkf[p * 3 + L];
+            o[27] = (float)a.Rf[p];
+            o[28] = a.FcRf[p];
+            o[29] = a.d2Rf ? a.d2Rf[p] : 0.0f;
+            for (int L = 0; L < 3; ++L) o[30 + L] = (float)R.gradpkrf[p * 3 + L];
+        }
     }
-    for (int L = 0; L < 3; ++L)
-        o[7 + L] = a.pp.ilpt >= 2 ? (float)R.Sbar2[p * 3 + L] * coef : 0.0f;
-    o[10] = (float)R.Fbarx[p];
-    if (a.nf == 11) return;
-    const double* st = R.strain + p * 9;  // symmetric 3x3
-    o[11] = (float)R.e_v[p];
-    o[12] = (float)R.p_v[p];
-    o[13] = (float)st[0];  // 11
-    o[14] = (float)st[4];  // 22
-    o[15] = (float)st[8];  // 33
-    o[16] = (float)st[5];  // 23
-    o[17] = (float)st[2];  // 13
-    o[18] = (float)st[1];  // 12
-    o[19] = (float)R.d2F[p];
-    o[20] = (float)R.zvir_half[p];
-    for (int L = 0; L < 3; ++L) o[21 + L] = (float)R.gradpk[p * 3 + L];
-    for (int L = 0; L < 3; ++L) o[24 + L] = (float)R.gradpkf[p * 3 + L];
-    o[27] = (float)a.Rf[p];
-    o[28] = a.FcRf[p];
-    o[29] = a.d2Rf ? a.d2Rf[p] : 0.0f;
-    for (int L = 0; L < 3; ++L) o[30 + L] = (float)R.gradpkrf[p * 3 + L];
+    __syncthreads();
+    // records s0 .. s0 + nkeep - 1, those below capacity
+    const int64_t nwrite = max((int64_t)0, min(s0 + nkeep, capacity) - s0) * a.nf;
+    float* dst = out + s0 * a.nf;
+    for (int64_t e = threadIdx.x; e < nwrite; e += REC_BLOCK) dst[e] = rows[e];
 }
 
 }  // namespace
@@ -266,12 +285,12 @@ extern "C" hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch,
     HNT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)ctx->rec_keep.p,
                                              (int64_t*)ctx->rec_slot.p, (int64_t)np, st));
     HNT_TRY(reserve(ctx, ctx->cub_tmp, tmp > 0 ? tmp : 16));
-    const unsigned blocks = (unsigned)((np + 255) / 256);
-    hipLaunchKernelGGL(rec_flag_kernel, dim3(blocks), dim3(256), 0, st, A,
+    const unsigned blocks = (unsigned)((np + REC_BLOCK - 1) / REC_BLOCK);
+    hipLaunchKernelGGL(rec_flag_kernel, dim3(blocks), dim3(REC_BLOCK), 0, st, A,
                        (int32_t*)ctx->rec_keep.p);
     HNT_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp.p, tmp, (const int32_t*)ctx->rec_keep.p,
                                              (int64_t*)ctx->rec_slot.p, (int64_t)np, st));
-    hipLaunchKernelGGL(rec_write_kernel, dim3(blocks), dim3(256), 0, st, A,
+    hipLaunchKernelGGL(rec_write_kernel, dim3(blocks), dim3(REC_BLOCK), 0, st, A,
                        (const int32_t*)ctx->rec_keep.p, (const int64_t*)ctx->rec_slot.p, records,
                        capacity);
     int64_t last_slot = 0;

@@ -566,7 +566,15 @@ def pack_records(pi, pj, pk, Rf, results: dict, FcRf, *, growth, chi2z=None, d2R
     dev = results["RTHL"].device
     ctx = context(dev.index or 0)
     npk = int(len(pi))
-    b, keep = _batch_struct(pi, pj, pk, Rf, 0, 1.0, 1.0, None, None, dev)
+    pi_, pj_, pk_ = (_dev1(x, torch.int32, dev) for x in (pi, pj, pk))
+    Rf_ = _dev1(Rf if isinstance(Rf, torch.Tensor) else
+                np.broadcast_to(np.asarray(Rf, np.float64), (npk,)), torch.float64, dev)
+    keep = [pi_, pj_, pk_, Rf_]
+    # the packer reads pi, pj, pk and Rf only
+    b = abi.Batch(npk, abi.as_ptr(_ptr(pi_), C.c_int32), abi.as_ptr(_ptr(pj_), C.c_int32),
+                  abi.as_ptr(_ptr(pk_), C.c_int32), abi.as_ptr(_ptr(Rf_), C.c_double),
+                  abi.as_ptr(0, C.c_int32), abi.as_ptr(0, C.c_double), abi.as_ptr(0, C.c_double),
+                  1.0, 1.0)
     fc = _dev1(FcRf, torch.float32, dev)
     d2 = _dev1(d2Rf, torch.float32, dev) if d2Rf is not None else None
     nf = 33 if int(params.get("ioutshear", 0)) >= 1 else 11
