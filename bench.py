@@ -244,8 +244,24 @@ def main():
     records = dict(ms=round(e0.elapsed_time(e1), 3), halos=int(recs.shape[0]),
                    floats_per_record=int(recs.shape[1]),
                    note="hnt_pack_records over the step's results: flags, scan, scatter")
-    del recs
     log(f"records: {records}")
+
+    # ---------------- catalog merge of those records (hnt_merge_catalog): Lagrangian
+    # exclusion, then with volume reduction as production runs use it; warm call, then timed
+    merge = {}
+    for reduce in (False, True):
+        api.merge_catalog(recs, volume_reduction=reduce)
+        torch.cuda.synchronize()
+        e0.record()
+        merged = api.merge_catalog(recs, volume_reduction=reduce)
+        e1.record()
+        torch.cuda.synchronize()
+        merge["reduction" if reduce else "exclusion"] = dict(
+            ms=round(e0.elapsed_time(e1), 3), halos_in=int(recs.shape[0]),
+            halos_out=int(merged.shape[0]))
+        del merged
+    log(f"merge: {merge}")
+    del recs
 
     # ---------------- tile peak finding from the raw tile: filter-bank smoothing (hipFFT
     # rfft once, window + irfft per filter) fused with the peak test (hnt_peak_find_tile);
@@ -385,6 +401,7 @@ def main():
                    "peak_find_ms": round(t_pf, 3), "shell_analysis_ms": round(t_sa, 3),
                    "tile_peak_find": tile_pf,
                    "records": records,
+                   "merge": merge,
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
                    "mean_cells_reference_extent": round(float(ref_cells.mean()), 1)

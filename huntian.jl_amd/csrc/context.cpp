@@ -322,6 +322,11 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
                       &ctx->fb_val, &ctx->fb_d2, &ctx->rec_tabs, &ctx->rec_keep,
                       &ctx->rec_slot};
     for (DevBuf* b : bufs) release(*b);
+    MergeBufs& m = ctx->mg;
+    for (DevBuf* b : {&m.a, &m.b, &m.c, &m.d, &m.rank, &m.cidx, &m.cstart, &m.cnt, &m.off, &m.e1,
+                      &m.e2, &m.e3, &m.e4, &m.state, &m.instart, &m.dv, &m.pk, &m.pv, &m.newr,
+                      &m.pos, &m.prank, &m.ext})
+        release(*b);
     release_fft(ctx);
     for (std::vector<hipEvent_t>& v : ctx->ev)
         for (hipEvent_t e : v) (void)hipEventDestroy(e);

@@ -67,6 +67,13 @@ struct SAWork {
     std::vector<int64_t> meta_host;
 };
 
+// scratch of hnt_merge_catalog: sort keys and permutations, ranks, the cell hash,
+// containment edges, decision states, overlap pairs, volume deficits, new radii
+struct MergeBufs {
+    DevBuf a, b, c, d, rank, cidx, cstart, cnt, off, e1, e2, e3, e4, state, instart, dv, pk,
+        pv, newr, pos, prank, ext;
+};
+
 // scratch of the bracket MLP (bf16 weights and activations, gradients)
 struct MlpScratch {
     DevBuf wt, x0, z, h, y, dy, dyf, dz, gw, part, gradv;
@@ -96,6 +103,7 @@ struct hnt_ctx {
     // tile peak finding: spectrum, windowed spectrum, smoothed fields, per-cell peak values
     hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2;
     hnt::DevBuf rec_tabs, rec_keep, rec_slot;  // halo records: tables, flags, slots
+    hnt::MergeBufs mg;                         // catalog merge
     void* fft_r2c = nullptr;  // hipfftHandle of an n^3 real-to-complex plan
     void* fft_c2r = nullptr;
     int fft_n = 0;

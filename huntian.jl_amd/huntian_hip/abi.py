@@ -232,6 +232,7 @@ EXPORTED = (
     "hnt_plan_launch",
     "hnt_plan_destroy",
     "hnt_pack_records",
+    "hnt_merge_catalog",
     "hnt_shell_gather_full",
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
@@ -287,6 +288,8 @@ def load(path: str | None = None):
     lib.hnt_pack_records.argtypes = [vp, C.POINTER(Batch), C.POINTER(Results), _f32p, _f32p,
                                      C.POINTER(CosmoTables), C.POINTER(PackParams), _f32p,
                                      C.c_int64, C.POINTER(C.c_int64)]
+    lib.hnt_merge_catalog.argtypes = [vp, _f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                      _f32p, C.POINTER(C.c_int64)]
     lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                       C.POINTER(ShellTables), C.POINTER(CTab),
                                       C.POINTER(Params), C.POINTER(Results), _i8p]

@@ -595,6 +595,29 @@ def pack_records(pi, pj, pk, Rf, results: dict, FcRf, *, growth, chi2z=None, d2R
     return out[:min(int(cnt.value), int(capacity))]
 
 
+def merge_catalog(records, *, volume_reduction: bool = False, fortran_ties: bool = False):
+    """Lagrangian exclusion, then optionally volume reduction, of a halo catalog on the
+    device (hnt_merge_catalog; Merger.merge_catalog, src/Merger/Merger.jl:23-89).
+
+    records: (N, 11) or (N, 33) float32 HaloRecord / ExtHaloRecord rows, on the device or
+    the host.  Returns the survivors in catalog order as a device float32 tensor with RTHL
+    replaced by the reduced radius (unchanged without volume_reduction)."""
+    torch = _require_gpu()
+    dev = records.device if isinstance(records, torch.Tensor) and records.is_cuda else \
+        torch.device("cuda", torch.cuda.current_device())
+    rec = torch.as_tensor(records, dtype=torch.float32).to(dev).contiguous()
+    if rec.ndim != 2 or rec.shape[1] not in (11, 33):
+        raise ValueError(f"records must be (N, 11) or (N, 33), got {tuple(rec.shape)}")
+    ctx = context(dev.index or 0)
+    out = torch.empty((max(rec.shape[0], 1), rec.shape[1]), dtype=torch.float32, device=dev)
+    cnt = C.c_int64(0)
+    abi.check(ctx.lib, ctx.lib.hnt_merge_catalog(
+        ctx.handle, abi.as_ptr(rec.data_ptr(), C.c_float), int(rec.shape[0]),
+        int(rec.shape[1]), int(bool(volume_reduction)), int(bool(fortran_ties)),
+        abi.as_ptr(out.data_ptr(), C.c_float), C.byref(cnt)))
+    return out[:int(cnt.value)]
+
+
 def analyse_peaks_gpu_cuda_multirf(delta, etax, etay, etaz, eta2x, eta2y, eta2z, batches,
                                    stab: ShellTables, ct_table, X1, X2, Y1, Y2, Z1, Z2, alatt,
                                    ZZon, *, fcrit_override=None, ct=None, rmax2rs=0.0,

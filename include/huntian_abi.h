@@ -366,6 +366,28 @@ hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch, const hnt_resu
                             const hnt_pack_params* pp, float* records, int64_t capacity,
                             int64_t* count);
 
+/* ---------------------------------------------------------------- catalog merge */
+
+/*
+ * hnt_merge_catalog -- replaces Merger.merge_catalog (src/Merger/Merger.jl:23-89) with the
+ * spatial hash, Lagrangian exclusion and volume reduction of src/Merger/Exclusion.jl:13-208,
+ * on the device.  records: device [nhalo * nfields] Float32 (11 or 33 fields, Catalog.jl
+ * layout; x, y, z are fields 0-2, RTHL field 6).  Halos are ranked by (-RTHL, x, y, z)
+ * (catalog order breaks exact ties); a halo is excluded when a surviving halo of higher rank
+ * contains its centre (distance < that halo's radius).  With volume_reduction, each
+ * survivor loses the caps of its overlaps with the other survivors, summed in the
+ * reference's order, r_new = cbrt(3 (4/3 pi r^3 - dV) / (4 pi)), and a halo left with no
+ * volume is removed; fortran_ties counts equal-radius pairs twice, as the Fortran code.
+ * out: device [nhalo * nfields], not overlapping records: the survivors in catalog order with
+ * RTHL = r_new; *nout: their number.  Non-finite positions or radii, or a zero extent (one
+ * halo without a positive radius), give HNT_ERR_INVALID.  Scratch kept by the context: about
+ * 80 B per halo, 16 B per containment pair, 32 B per overlapping pair (with reduction) and
+ * 4 B per hash cell (nc^3, nc = clamp(round(cbrt(nhalo)), 16, 1024)).
+ */
+hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int64_t nhalo,
+                             int32_t nfields, int32_t volume_reduction, int32_t fortran_ties,
+                             float* out, int64_t* nout);
+
 /* ---------------------------------------------------------------- primitive entries */
 /* Single-primitive entry points; the reference exposes them as test harnesses
  * (src/Huntian.jl:98-217). */

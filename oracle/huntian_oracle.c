@@ -1135,3 +1135,198 @@ int64_t hro_pack_records(int64_t npeaks, const int32_t* pi, const int32_t* pj,
     }
     return nrec;
 }
+
+/* ------------------------------------------------------------------ catalog merge */
+
+void hro_sphere_overlap(double d, double r1, double r2, double* v1, double* v2) {
+    *v1 = *v2 = 0.0;
+    if (d >= r1 + r2) return;
+    if (d <= fabs(r1 - r2)) return;
+    double h1 = (r2 - r1 + d) * (r2 + r1 - d) / (2.0 * d);
+    double h2 = (r1 - r2 + d) * (r1 + r2 - d) / (2.0 * d);
+    *v1 = (PI_D / 3.0) * (h1 * h1) * (3.0 * r1 - h1);
+    *v2 = (PI_D / 3.0) * (h2 * h2) * (3.0 * r2 - h2);
+}
+
+static const double* mg_x;
+static const double* mg_y;
+static const double* mg_z;
+static const double* mg_r;
+
+/* (-r, x, y, z), then catalog index: the stable sortperm of Merger.jl:59 */
+static int mg_cmp(const void* pa, const void* pb) {
+    int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
+    double ka[4] = {-mg_r[a], mg_x[a], mg_y[a], mg_z[a]};
+    double kb[4] = {-mg_r[b], mg_x[b], mg_y[b], mg_z[b]};
+    for (int q = 0; q < 4; ++q) {
+        if (ka[q] < kb[q]) return -1;
+        if (ka[q] > kb[q]) return 1;
+        /* Julia's isless orders -0.0 before 0.0 */
+        if (ka[q] == 0.0 && kb[q] == 0.0 && signbit(ka[q]) != signbit(kb[q]))
+            return signbit(ka[q]) ? -1 : 1;
+    }
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+static int mg_cell(double p, double o, double cs, int nc) {
+    double f = floor((p - o) / cs);
+    long c = (long)f + 1;
+    return c < 1 ? 1 : (c > nc ? nc : (int)c);
+}
+
+/* The two passes of _merge_impl (Merger.jl:47-79): survivors and the new radii in FP64.
+ * Returns 0, -1 if memory runs out, -2 for an empty domain (the reference's floor of NaN). */
+int32_t hro_merge_passes(const float* rec, int64_t N, int32_t nf, int32_t reduce,
+                         int32_t fortran_ties, uint8_t* surv, double* newr) {
+    if (N == 0) return 0;
+    double* x = malloc(sizeof(double) * N);
+    double* y = malloc(sizeof(double) * N);
+    double* z = malloc(sizeof(double) * N);
+    double* r = malloc(sizeof(double) * N);
+    int64_t* order = malloc(sizeof(int64_t) * N);
+    int64_t* rank = malloc(sizeof(int64_t) * N);
+    int32_t* ll = calloc(N, sizeof(int32_t));
+    double* dV = calloc(N, sizeof(double));
+    int32_t status = 0;
+    int32_t* hoc = NULL;
+    if (!x || !y || !z || !r || !order || !rank || !ll || !dV) {
+        status = -1;
+        goto done;
+    }
+    double rmax = 0.0, mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t m = 0; m < N; ++m) {
+        x[m] = rec[m * nf]; y[m] = rec[m * nf + 1]; z[m] = rec[m * nf + 2]; r[m] = rec[m * nf + 6];
+        order[m] = m;
+        surv[m] = 1;
+        if (r[m] > rmax) rmax = r[m];
+        double p[3] = {x[m], y[m], z[m]};
+        for (int c = 0; c < 3; ++c) {
+            if (p[c] < mn[c]) mn[c] = p[c];
+            if (p[c] > mx[c]) mx[c] = p[c];
+        }
+    }
+    mg_x = x; mg_y = y; mg_z = z; mg_r = r;
+    qsort(order, (size_t)N, sizeof(int64_t), mg_cmp);
+    for (int64_t q = 0; q < N; ++q) rank[order[q]] = q;
+    double pad = 2.0 * rmax;
+    double dmin[3] = {mn[0] - pad, mn[1] - pad, mn[2] - pad};
+    double L = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        double e = (mx[c] + pad) - dmin[c];
+        if (e > L) L = e;
+    }
+    if (!(L > 0.0)) {
+        status = -2;
+        goto done;
+    }
+    int nc = (int)nearbyint(cbrt((double)N));
+    nc = nc < 16 ? 16 : (nc > 1024 ? 1024 : nc);
+    double cs = L / nc;
+    hoc = calloc((size_t)nc * nc * nc, sizeof(int32_t));
+    if (!hoc) {
+        status = -1;
+        goto done;
+    }
+#define HOC(ix, iy, iz) hoc[((ix) - 1) + (int64_t)nc * (((iy) - 1) + (int64_t)nc * ((iz) - 1))]
+    for (int64_t m = 0; m < N; ++m) {  /* build_hash: linked lists, newest first */
+        int ix = mg_cell(x[m], dmin[0], cs, nc), iy = mg_cell(y[m], dmin[1], cs, nc),
+            iz = mg_cell(z[m], dmin[2], cs, nc);
+        ll[m] = HOC(ix, iy, iz);
+        HOC(ix, iy, iz) = (int32_t)(m + 1);
+    }
+    /* lagrangian_exclusion! */
+    for (int64_t q = 0; q < N; ++q) {
+        int64_t i = order[q];
+        if (!surv[i]) continue;
+        double ri = r[i], xi = x[i], yi = y[i], zi = z[i];
+        int dcell = (int)ceil(ri / cs);
+        int ci = mg_cell(xi, dmin[0], cs, nc), cj = mg_cell(yi, dmin[1], cs, nc),
+            ck = mg_cell(zi, dmin[2], cs, nc);
+        for (int dz = -dcell; dz <= dcell; ++dz)
+            for (int dy = -dcell; dy <= dcell; ++dy)
+                for (int dx = -dcell; dx <= dcell; ++dx) {
+                    int jx = dx + ci, jy = dy + cj, jz = dz + ck;
+                    if (jx < 1 || jx > nc || jy < 1 || jy > nc || jz < 1 || jz > nc) continue;
+                    int32_t j1 = HOC(jx, jy, jz);
+                    while (j1 > 0) {
+                        int64_t j = j1 - 1;
+                        if (j != i && surv[j]) {
+                            double rj = r[j];
+                            if (ri >= rj) {
+                                double ex = x[j] - xi, ey = y[j] - yi, ez = z[j] - zi;
+                                double dist = sqrt(ex * ex + ey * ey + ez * ez);
+                                if (dist < ri) surv[j] = 0;
+                            }
+                        }
+                        j1 = ll[j];
+                    }
+                }
+    }
+    /* volume_reduction! */
+    for (int64_t m = 0; m < N; ++m) newr[m] = r[m];
+    if (reduce) {
+        for (int64_t q = 0; q < N; ++q) {
+            int64_t i = order[q];
+            if (!surv[i]) continue;
+            double ri = r[i], xi = x[i], yi = y[i], zi = z[i];
+            int dcell = (int)ceil(2.0 * ri / cs);
+            int ci = mg_cell(xi, dmin[0], cs, nc), cj = mg_cell(yi, dmin[1], cs, nc),
+                ck = mg_cell(zi, dmin[2], cs, nc);
+            for (int dz = -dcell; dz <= dcell; ++dz)
+                for (int dy = -dcell; dy <= dcell; ++dy)
+                    for (int dx = -dcell; dx <= dcell; ++dx) {
+                        int jx = dx + ci, jy = dy + cj, jz = dz + ck;
+                        if (jx < 1 || jx > nc || jy < 1 || jy > nc || jz < 1 || jz > nc) continue;
+                        int32_t j1 = HOC(jx, jy, jz);
+                        while (j1 > 0) {
+                            int64_t j = j1 - 1;
+                            if (j != i && surv[j] && rank[j] > rank[i]) {
+                                double rj = r[j];
+                                double ex = x[j] - xi, ey = y[j] - yi, ez = z[j] - zi;
+                                double d = sqrt(ex * ex + ey * ey + ez * ez);
+                                if (d < ri + rj) {
+                                    double v1, v2;
+                                    hro_sphere_overlap(d, ri, rj, &v1, &v2);
+                                    double w = (fortran_ties && rj == ri) ? 2.0 : 1.0;
+                                    dV[i] += w * v1;
+                                    dV[j] += w * v2;
+                                }
+                            }
+                            j1 = ll[j];
+                        }
+                    }
+        }
+        for (int64_t m = 0; m < N; ++m) {
+            if (!surv[m]) continue;
+            double vol_new = 4.0 * PI_D / 3.0 * (r[m] * r[m] * r[m]) - dV[m];
+            if (vol_new <= 0.0) {
+                surv[m] = 0;
+                newr[m] = 0.0;
+            } else {
+                newr[m] = cbrt(vol_new * 3.0 / (4.0 * PI_D));
+            }
+        }
+    }
+#undef HOC
+done:
+    free(x); free(y); free(z); free(r); free(order); free(rank); free(ll); free(dV); free(hoc);
+    return status;
+}
+
+int64_t hro_merge_catalog(const float* rec, int64_t N, int32_t nf, int32_t reduce,
+                          int32_t fortran_ties, float* out) {
+    if (N == 0) return 0;
+    uint8_t* surv = malloc(N);
+    double* newr = malloc(sizeof(double) * N);
+    if (!surv || !newr) return -1;
+    int32_t st = hro_merge_passes(rec, N, nf, reduce, fortran_ties, surv, newr);
+    int64_t nout = 0;
+    for (int64_t m = 0; st == 0 && m < N; ++m) {
+        if (!surv[m]) continue;
+        for (int c = 0; c < nf; ++c) out[nout * nf + c] = rec[m * nf + c];
+        out[nout * nf + 6] = (float)newr[m];
+        nout++;
+    }
+    free(surv); free(newr);
+    return st == 0 ? nout : st;
+}

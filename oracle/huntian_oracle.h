@@ -71,6 +71,18 @@ int64_t hro_pack_records(int64_t npeaks, const int32_t* pi, const int32_t* pj,
                          const float* FcRf, const float* d2Rf, const hnt_cosmo_tables* t,
                          const hnt_pack_params* pp, float* out, int64_t capacity);
 double hro_growth_D(const hnt_cosmo_tables* t, double a);   /* Dlinear_ab(a)[1] */
+
+/* Merger.merge_catalog (src/Merger/Merger.jl:23-89) with build_hash, auto_hash_nc,
+ * lagrangian_exclusion!, sphere_overlap, volume_reduction! (src/Merger/Exclusion.jl), over
+ * host records [nhalo * nf]; writes the survivors (RTHL replaced) and returns their number,
+ * -1 if memory runs out or -2 for an empty domain. */
+int64_t hro_merge_catalog(const float* rec, int64_t nhalo, int32_t nf, int32_t volume_reduction,
+                          int32_t fortran_ties, float* out);
+/* the exclusion and reduction passes alone: survived flags and FP64 radii per halo */
+int32_t hro_merge_passes(const float* rec, int64_t nhalo, int32_t nf, int32_t volume_reduction,
+                         int32_t fortran_ties, uint8_t* survived, double* new_r);
+/* sphere_overlap (Exclusion.jl:86-101) */
+void hro_sphere_overlap(double d, double r1, double r2, double* v1, double* v2);
 double hro_chi_to_z(const hnt_cosmo_tables* t, double chi);  /* chi_to_z */
 
 #ifdef __cplusplus

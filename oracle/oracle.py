@@ -78,6 +78,13 @@ def lib():
                                        C.POINTER(abi.CosmoTables), C.POINTER(abi.PackParams),
                                        _f32p, C.c_int64]
         L.hro_pack_records.restype = C.c_int64
+        L.hro_merge_catalog.argtypes = [_f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                        _f32p]
+        L.hro_merge_catalog.restype = C.c_int64
+        L.hro_merge_passes.argtypes = [_f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_uint8), _f64p]
+        L.hro_merge_passes.restype = C.c_int32
+        L.hro_sphere_overlap.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _f64p]
         L.hro_growth_D.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
         L.hro_growth_D.restype = C.c_double
         L.hro_chi_to_z.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
@@ -342,3 +349,41 @@ def pack_records(pi, pj, pk, Rf, r: dict, FcRf, *, growth, chi2z=None, d2Rf=None
                                   _p(d2, C.c_float), C.byref(t), C.byref(pp),
                                   _p(out, C.c_float), cap)
     return out[:min(nrec, cap)], int(nrec)
+
+
+# ---------------------------------------------------------------- catalog merge
+
+def merge_catalog(records, volume_reduction=False, fortran_ties=False):
+    """Merger.merge_catalog on an (N, 11 or 33) float32 record array: (survivors, count)."""
+    rec = np.ascontiguousarray(records, np.float32)
+    n, nf = rec.shape
+    out = np.zeros((max(n, 1), nf), np.float32)
+    k = lib().hro_merge_catalog(_p(rec, C.c_float), n, nf, int(bool(volume_reduction)),
+                                int(bool(fortran_ties)), _p(out, C.c_float))
+    _merge_status(k)
+    return out[:k], int(k)
+
+
+def _merge_status(k):
+    if k == -1:
+        raise MemoryError("oracle merge_catalog: allocation failed")
+    if k == -2:
+        raise ValueError("oracle merge_catalog: empty domain")
+
+
+def merge_passes(records, volume_reduction=False, fortran_ties=False):
+    """The two passes alone: (survived bool [N], new radii float64 [N])."""
+    rec = np.ascontiguousarray(records, np.float32)
+    n, nf = rec.shape
+    surv = np.zeros(max(n, 1), np.uint8)
+    newr = np.zeros(max(n, 1), np.float64)
+    _merge_status(lib().hro_merge_passes(_p(rec, C.c_float), n, nf, int(bool(volume_reduction)),
+                                         int(bool(fortran_ties)), _p(surv, C.c_uint8),
+                                         _p(newr, C.c_double)))
+    return surv[:n].astype(bool), newr[:n]
+
+
+def sphere_overlap(d, r1, r2):
+    v1, v2 = C.c_double(), C.c_double()
+    lib().hro_sphere_overlap(float(d), float(r1), float(r2), C.byref(v1), C.byref(v2))
+    return v1.value, v2.value

@@ -5,42 +5,22 @@ device and host pow may differ in the last place, so those are held to 1 Float32
 import numpy as np
 import pytest
 
-from huntian_hip import api, cosmology, synth
+from huntian_hip import api, cosmology
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def tile(websky):
-    import torch
-
-    n, box, nbuff, nhunt = 64, 128.0, 11, 10
-    alatt = box / n
-    delta, dk = synth.gaussian_field(n, box, 6060)
-    f = {"delta": delta}
-    f.update(synth.lpt_fields(dk, n, box))
-    radii = synth.filter_radii(1.5, 16.0, 8)
-    sm = [api.to_device_field(synth.smooth_tophat(dk, n, box, R)) for R in radii]
-    mask = api.to_device_field(np.zeros((n, n, n), np.int8), dtype=torch.int8)
-    pf = api.peak_find(sm, [1.5] * len(radii), nbuff, mask, capacity=n ** 3)
-    Rf = torch.tensor(radii, dtype=torch.float64, device="cuda")[pf["filt"].long()]
-    ir2 = torch.tensor([api.ir2min_of(R, alatt) for R in radii], dtype=torch.int32,
-                       device="cuda")[pf["filt"].long()]
-    mask.zero_()
-    r = api.shell_analyse(f, pf["i"], pf["j"], pf["k"], Rf, ir2, api.build_shell_tables(nhunt),
-                          websky, alatt=alatt, ZZon=1.0, fcrit=api.fsc_of_z(websky, 0.0),
-                          nbuff=nbuff, mask=mask)
-    assert (r["RTHL"] > 0).sum().item() > 30
-    d2 = torch.randn(len(pf["i"]), device="cuda")
-    return dict(n=n, alatt=alatt, pf=pf, Rf=Rf, r=r, d2=d2)
-
-
-COSMO = cosmology.CosmologyParams(0.31, 0.049, 0.69, 0.68)
+def tile(gpu_tile):
+    return gpu_tile
 
 
 @pytest.fixture(scope="module")
-def tables():
-    return cosmology.dlinear_tables(COSMO), cosmology.build_chi_to_z(COSMO, z_max=1.0, npts=500)
+def tables(cosmo_tables):
+    return cosmo_tables["growth"], cosmo_tables["chi2z"]
+
+
+COSMO = cosmology.CosmologyParams(0.31, 0.049, 0.69, 0.68)
 
 
 @pytest.mark.parametrize("ioutshear,ilpt,ievol", [(0, 1, 0), (1, 2, 0), (1, 2, 1), (0, 2, 1)])
