@@ -68,6 +68,8 @@ def main():
                     help="skip timing the shell analysis under the other schedules")
     ap.add_argument("--no-filter-bank", action="store_true",
                     help="skip timing tile peak finding from the raw tile (smoothing included)")
+    ap.add_argument("--no-field-synthesis", action="store_true",
+                    help="skip timing the tile field synthesis")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 4 GiB device copy (3 times), the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -376,6 +378,19 @@ def main():
                           f"exit), OpenMP over peaks; {elapsed:.1f} s")
         log(f"cpu baseline: {cpu}")
 
+    # ---------------- tile field synthesis of one tile of this size (the step before the
+    # path), after the step's buffers are freed
+    synth_t = None
+    if not args.no_field_synthesis:
+        fields.clear()
+        smoothed.clear()
+        del mask, r, pf
+        results = None
+        ctx.release_scratch()
+        torch.cuda.empty_cache()
+        synth_t = time_field_synthesis(n, box, nbuff, dev)
+        log(f"field synthesis: {synth_t}")
+
     out = {
         "metric": METRIC,
         "value": round(npk / (ms_step * 1e-3), 1),
@@ -402,6 +417,7 @@ def main():
                    "tile_peak_find": tile_pf,
                    "records": records,
                    "merge": merge,
+                   "field_synthesis": synth_t,
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
                    "mean_cells_reference_extent": round(float(ref_cells.mean()), 1)
@@ -413,6 +429,58 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def time_field_synthesis(n, box, nbuff, dev):
+    """One tile's fields as the tile driver builds them (src/MultiResolution.jl:881-1010):
+    delta_self and psi1_self from one zero-padded (2n)^3 isolated convolution of the residual
+    noise, the coarse delta and psi1 interpolated onto the tile (M = 256 coarse cells over a
+    N = 4096 global grid, tile (2, 2, 2)), then the periodic 2LPT and Laplacian of the combined
+    delta.  Each piece timed once after a warm call, on synthetic noise and coarse fields."""
+    import torch
+    from huntian_hip import api, synth
+
+    pk = synth.PowerSpectrum().file
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    residual = api.fortran_empty((n, n, n), torch.float32, dev)
+    residual.copy_(torch.randn((n, n, n), generator=g, device=dev))
+    M, N, nsub = 256, 4096, n - 2 * nbuff
+    coarse = [torch.randn((M, M, M), generator=g, device=dev) for _ in range(4)]
+    kernels = [(0, 0, 0), (1, 1, 0), (1, 2, 0), (1, 3, 0)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        fn()  # warm (plans, scratch)
+        torch.cuda.synchronize()
+        ev[0].record()
+        out = fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return out, round(ev[0].elapsed_time(ev[1]), 3)
+
+    selfs, t_conv = timed(lambda: api.isolated_convolve_gpu_multi(
+        residual, pk, box, n, kernels=kernels, return_device=True))
+    del residual
+
+    def interp_all():
+        return [api.interpolate_to_tile_gpu(c, 2, 2, 2, nsub, n, N, M, return_device=True)
+                for c in coarse]
+    longs, t_interp = timed(interp_all)
+    delta_tile = selfs[0] + longs[0]
+    del selfs, longs
+    psi2, t_2lpt = timed(lambda: api.compute_2lpt_gpu(delta_tile, n, box, return_device=True))
+    del psi2
+    lap, t_lap = timed(lambda: api.compute_laplacian_gpu(delta_tile, n, box,
+                                                         return_device=True))
+    del lap, delta_tile
+    api.context(0).release_scratch()
+    torch.cuda.empty_cache()
+    return dict(isolated_convolve_ms=t_conv, interpolate_ms=t_interp, compute_2lpt_ms=t_2lpt,
+                laplacian_ms=t_lap, total_ms=round(t_conv + t_interp + t_2lpt + t_lap, 3),
+                note=f"{n}^3 tile: isolated convolution of delta + psi1 x3 over the "
+                     f"{2 * n}^3 padded box (1 R2C + 4 C2R), 4 tri-cubic interpolations "
+                     f"from {M}^3, 2LPT (2 R2C + 9 C2R at {n}^3), Laplacian (1 + 1)")
 
 
 def run_batched(args, cfg):

@@ -305,29 +305,44 @@ hnt_status hnt_ctx_create(int device, hnt_ctx** out) {
     return HNT_OK;
 }
 
-hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
-    clear_error();
-    if (!ctx) return HNT_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+// The per-call scratch: everything sized by a call's inputs (not the uploaded tables, the
+// interpolation table or the network, which plans may reference).
+static void release_scratch(hnt_ctx* ctx) {
     release_work(ctx->sa);
-    DevBuf* bufs[] = {&ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp,
-                      &ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small,
-                      &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
-                      &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
-                      &ctx->tables.kw_idx, &ctx->ctab.table, &ctx->mlp.wt, &ctx->mlp.x0,
-                      &ctx->mlp.z, &ctx->mlp.h, &ctx->mlp.y, &ctx->mlp.dy, &ctx->mlp.dyf,
-                      &ctx->mlp.dz, &ctx->mlp.gw, &ctx->mlp.part, &ctx->mlp.gradv,
-                      &ctx->fb_spec, &ctx->fb_work, &ctx->fb_field, &ctx->fb_lap,
-                      &ctx->fb_val, &ctx->fb_d2, &ctx->rec_tabs, &ctx->rec_keep,
-                      &ctx->rec_slot};
-    for (DevBuf* b : bufs) release(*b);
+    for (DevBuf* b : {&ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp, &ctx->fb_spec,
+                      &ctx->fb_work, &ctx->fb_field, &ctx->fb_lap, &ctx->fb_val, &ctx->fb_d2,
+                      &ctx->rec_tabs, &ctx->rec_keep, &ctx->rec_slot, &ctx->fs_real,
+                      &ctx->fs_base, &ctx->fs_work, &ctx->fs_aux, &ctx->fs_tab})
+        release(*b);
     MergeBufs& m = ctx->mg;
     for (DevBuf* b : {&m.a, &m.b, &m.c, &m.d, &m.rank, &m.cidx, &m.cstart, &m.cnt, &m.off, &m.e1,
                       &m.e2, &m.e3, &m.e4, &m.state, &m.instart, &m.dv, &m.pk, &m.pv, &m.newr,
                       &m.pos, &m.prank, &m.ext})
         release(*b);
     release_fft(ctx);
+    release_field_plans(ctx);
+}
+
+hnt_status hnt_ctx_release_scratch(hnt_ctx* ctx) {
+    HNT_TRY(begin_call(ctx));
+    HNT_HIP(hipStreamSynchronize(ctx->stream));
+    release_scratch(ctx);
+    return HNT_OK;
+}
+
+hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
+    clear_error();
+    if (!ctx) return HNT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_scratch(ctx);
+    DevBuf* bufs[] = {&ctx->fieldptrs, &ctx->fcrit, &ctx->akk, &ctx->small,
+                      &ctx->tables.cells, &ctx->tables.start, &ctx->tables.count,
+                      &ctx->tables.r2, &ctx->tables.rad, &ctx->tables.kw,
+                      &ctx->tables.kw_idx, &ctx->ctab.table, &ctx->mlp.wt, &ctx->mlp.x0,
+                      &ctx->mlp.z, &ctx->mlp.h, &ctx->mlp.y, &ctx->mlp.dy, &ctx->mlp.dyf,
+                      &ctx->mlp.dz, &ctx->mlp.gw, &ctx->mlp.part, &ctx->mlp.gradv};
+    for (DevBuf* b : bufs) release(*b);
     for (std::vector<hipEvent_t>& v : ctx->ev)
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

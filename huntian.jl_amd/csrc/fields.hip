@@ -1,0 +1,513 @@
+// fields.hip -- tile field synthesis: isolated (zero-padded) convolutions with the tabulated
+// sqrt P(k) and the LPT transfers, periodic 2LPT and Laplacian, isolated Poisson displacement
+// and short-range potential, and the tri-cubic coarse -> tile interpolation.
+//
+// Replaces the CUDA methods of ext/CUDAExt.jl:2346-2825 and 3062-3176 (stubs
+// src/Huntian.jl:283-385), called by the tile driver src/MultiResolution.jl:881-1010.
+//
+// Every transform is one hipFFT R2C or C2R over the whole tile (plans cached per size).  The
+// k-space steps are single passes that read the shared spectrum and write the per-output
+// copy, so one forward transform serves all outputs without a separate copy; the crop, the
+// sign flip and the 2LPT source update are fused into one pass each over the real field.
+// k-space arithmetic follows the reference kernels operation for operation (Float64
+// frequencies and coefficients, Float32 spectrum); the tri-cubic interpolation is Float32 in
+// the reference's order, so it matches the reference bit for bit.
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+
+#include "hnt_internal.h"
+
+using namespace hnt;
+
+#define HNT_FFT(expr)                                                                   \
+    do {                                                                                \
+        const hipfftResult r_ = (expr);                                                 \
+        if (r_ != HIPFFT_SUCCESS) {                                                     \
+            hnt::set_error("%s: hipFFT status %d (%s:%d)", #expr, (int)r_, __FILE__,    \
+                           __LINE__);                                                   \
+            return HNT_ERR_DEVICE;                                                      \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+constexpr int FT = 256;  // threads per workgroup
+
+unsigned grid_for(int64_t n) {
+    return (unsigned)std::min<int64_t>((n + FT - 1) / FT, 1 << 20);
+}
+
+// _interp_pk_table (ext/CUDAExt.jl:2351-2364), Float32
+__device__ __forceinline__ float interp_pk(const float* __restrict__ tab, float log_k,
+                                           float log_k_min, float d_log_k, int ntab) {
+    const float fpos = (log_k - log_k_min) / d_log_k;
+    const int i0 = (int)floorf(fpos) + 1;  // 1-based
+    if (i0 < 1) return tab[0];
+    if (i0 >= ntab) return tab[ntab - 1];
+    const float frac = fpos - (float)(i0 - 1);
+    return tab[i0 - 1] + frac * (tab[i0] - tab[i0 - 1]);
+}
+
+__device__ __forceinline__ double pick(int d, double kx, double ky, double kz) {
+    return d == 1 ? kx : (d == 2 ? ky : kz);
+}
+
+// _transfer_kernel! (ext/CUDAExt.jl:2367-2432): work = transfer(base), over the half spectrum
+// (nk = n2/2 + 1, x fastest).  ky, kz of index > n2/2 are negative (fftfreq).
+__global__ void fs_transfer(const float2* __restrict__ base, float2* __restrict__ work, int n2,
+                            double dk, const float* __restrict__ tab, int ntab, float log_k_min,
+                            float d_log_k, int id, int d1, int d2) {
+    const int nk = n2 / 2 + 1, half = n2 / 2;
+    const int64_t total = (int64_t)nk * n2 * n2;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int ix = (int)(t % nk);
+        const int64_t rem = t / nk;
+        const int iy = (int)(rem % n2), iz = (int)(rem / n2);
+        const double kx = (double)ix * dk;
+        const double ky = (double)(iy < half ? iy : iy - n2) * dk;
+        const double kz = (double)(iz < half ? iz : iz - n2) * dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 == 0.0) {
+            work[t] = make_float2(0.0f, 0.0f);
+            continue;
+        }
+        const double sqrt_pk =
+            (double)interp_pk(tab, (float)(log(k2) * 0.5), log_k_min, d_log_k, ntab);
+        const float2 v = base[t];
+        float2 o;
+        if (id == 0) {
+            o = make_float2((float)((double)v.x * sqrt_pk), (float)((double)v.y * sqrt_pk));
+        } else if (id == 1 || id == 2) {
+            const double coeff = sqrt_pk * pick(d1, kx, ky, kz) / k2;
+            const double a = v.x, b = v.y;
+            o = id == 1 ? make_float2((float)(-b * coeff), (float)(a * coeff))
+                        : make_float2((float)(b * coeff), (float)(-a * coeff));
+        } else {
+            float c;
+            if (id == 3) c = (float)(-sqrt_pk * pick(d1, kx, ky, kz) * pick(d2, kx, ky, kz) / k2);
+            else if (id == 4) c = (float)(sqrt_pk * k2);
+            else c = (float)(-sqrt_pk / k2);
+            o = make_float2(v.x * c, v.y * c);
+        }
+        work[t] = o;
+    }
+}
+
+// _hermitize_planes_kernel! (ext/CUDAExt.jl:2439-2473) on the kx = 0 and kx = n2/2 planes:
+// the lower cell of each conjugate pair (iy, iz) ~ (-iy, -iz) writes both.
+__global__ void fs_hermitize(float2* __restrict__ a, int n2) {
+    const int nk = n2 / 2 + 1;
+    const int64_t total = (int64_t)n2 * n2;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int iy = (int)(t % n2), iz = (int)(t / n2);
+        const int iyc = iy == 0 ? 0 : n2 - iy, izc = iz == 0 ? 0 : n2 - iz;
+        const bool self = iy == iyc && iz == izc;
+        const bool lower = iy < iyc || (iy == iyc && iz < izc);
+        if (!(lower || self)) continue;
+        for (int p = 0; p < 2; ++p) {
+            const int ix = p == 0 ? 0 : nk - 1;
+            float2* me = a + ix + (int64_t)nk * (iy + (int64_t)n2 * iz);
+            if (self) {
+                me->y = 0.0f;
+            } else {
+                float2* co = a + ix + (int64_t)nk * (iyc + (int64_t)n2 * izc);
+                const float2 vm = *me, vc = *co;
+                const float2 avg = make_float2((vm.x + vc.x) * 0.5f, (vm.y - vc.y) * 0.5f);
+                *me = avg;
+                *co = make_float2(avg.x, -avg.y);
+            }
+        }
+    }
+}
+
+// _periodic_kernel! (ext/CUDAExt.jl:2629-2677): dst = kernel(src) on an n^3 half spectrum;
+// k = 0 zeroed, and the Nyquist planes too except for phi_ij (id 3).  ky, kz of index n/2
+// count as +n/2 (iy <= nyq + 1, 1-based).  id 2: -i k_d1/k^2, 3: -k_d1 k_d2/k^2, 4: k^2.
+__global__ void fs_periodic(const float2* __restrict__ src, float2* __restrict__ dst, int n,
+                            double dk, int id, int d1, int d2) {
+    const int nk = n / 2 + 1, nyq = n / 2;
+    const int64_t total = (int64_t)nk * n * n;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int ix = (int)(t % nk);
+        const int64_t rem = t / nk;
+        const int iy = (int)(rem % n), iz = (int)(rem / n);
+        const double kx = (double)ix * dk;
+        const double ky = (double)(iy <= nyq ? iy : iy - n) * dk;
+        const double kz = (double)(iz <= nyq ? iz : iz - n) * dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 == 0.0 || (id != 3 && (ix == nk - 1 || iy == nyq || iz == nyq))) {
+            dst[t] = make_float2(0.0f, 0.0f);
+            continue;
+        }
+        const float2 v = src[t];
+        float2 o;
+        if (id == 2) {
+            const double coeff = pick(d1, kx, ky, kz) / k2;
+            const double a = v.x, b = v.y;
+            o = make_float2((float)(b * coeff), (float)(-a * coeff));
+        } else {
+            const float c = id == 3 ? (float)(-pick(d1, kx, ky, kz) * pick(d2, kx, ky, kz) / k2)
+                                    : (float)k2;
+            o = make_float2(v.x * c, v.y * c);
+        }
+        dst[t] = o;
+    }
+}
+
+// _short_pot_kernel (ext/CUDAExt.jl:2722), in place on a (2n)^3 half spectrum; the index
+// n2/2 counts as +n2/2 (only k^2 enters).
+__global__ void fs_short_pot(float2* __restrict__ a, int n2, double dk, double rs2) {
+    const int nk = n2 / 2 + 1, half = n2 / 2;
+    const int64_t total = (int64_t)nk * n2 * n2;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int ix = (int)(t % nk);
+        const int64_t rem = t / nk;
+        const int iy = (int)(rem % n2), iz = (int)(rem / n2);
+        const double kx = (double)ix * dk;
+        const double ky = (double)(iy <= half ? iy : iy - n2) * dk;
+        const double kz = (double)(iz <= half ? iz : iz - n2) * dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const float c = (float)(k2 == 0.0 ? -rs2 : -(1.0 - exp(-k2 * rs2)) / k2);
+        const float2 v = a[t];
+        a[t] = make_float2(v.x * c, v.y * c);
+    }
+}
+
+// dst (n2^3) = src (nin^3) in the low corner, zeros elsewhere; one thread per 4 x-cells
+// (n2 is even and a multiple of 4 when nin is even; else one cell per thread).
+__global__ void fs_pad(const float* __restrict__ src, int nin, int n2, float* __restrict__ dst) {
+    const int64_t total = (int64_t)n2 * n2 * n2;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int ix = (int)(t % n2);
+        const int64_t rem = t / n2;
+        const int iy = (int)(rem % n2), iz = (int)(rem / n2);
+        dst[t] = (ix < nin && iy < nin && iz < nin)
+                     ? src[ix + (int64_t)nin * (iy + (int64_t)nin * iz)]
+                     : 0.0f;
+    }
+}
+
+// out (n^3) = scale * big[s + i, s + j, s + k] of an m^3 field (scale: the inverse
+// transform's 1/m^3, negated where the reference negates the result)
+// (in place when m == n and s == 0: each element is read and written by one thread)
+__global__ void fs_crop(const float* big, int m, int n, int s, float scale, float* out) {
+    const int64_t total = (int64_t)n * n * n;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int i = (int)(t % n);
+        const int64_t rem = t / n;
+        const int j = (int)(rem % n), k = (int)(rem / n);
+        out[t] = big[(s + i) + (int64_t)m * ((s + j) + (int64_t)m * (s + k))] * scale;
+    }
+}
+
+// src2 = delta^2 * 0.5 (Float32, compute_2lpt_gpu's broadcast)
+__global__ void fs_src2_init(const float* __restrict__ d, int64_t n3, float* __restrict__ s2) {
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < n3;
+         t += (int64_t)gridDim.x * FT) {
+        const float v = d[t];
+        s2[t] = (v * v) * 0.5f;
+    }
+}
+
+// src2 -= phi^2 * coef, phi = the unnormalised inverse transform times scale (1/n^3)
+__global__ void fs_src2_sub(const float* __restrict__ phi, int64_t n3, float scale, float coef,
+                            float* __restrict__ s2) {
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < n3;
+         t += (int64_t)gridDim.x * FT) {
+        const float v = phi[t] * scale;
+        s2[t] = s2[t] - (v * v) * coef;
+    }
+}
+
+// mod1(i, M) of the reference (_mod1_i32, ext/CUDAExt.jl:3074-3081)
+__device__ __forceinline__ int mod1(int i, int M) {
+    int r = (i - 1) % M;
+    if (r < 0) r += M;
+    return r + 1;
+}
+
+// _catmull_rom_f32 (ext/CUDAExt.jl:3065-3072)
+__device__ __forceinline__ void catmull_rom(float t, float w[4]) {
+    const float t2 = t * t, t3 = t2 * t;
+    w[0] = -0.5f * t3 + t2 - 0.5f * t;
+    w[1] = 1.5f * t3 - 2.5f * t2 + 1.0f;
+    w[2] = -1.5f * t3 + 2.0f * t2 + 0.5f * t;
+    w[3] = 0.5f * t3 - 0.5f * t2;
+}
+
+// _tricubic_kernel! (ext/CUDAExt.jl:3083-3140): one thread per tile cell
+__global__ void fs_tricubic(const float* __restrict__ coarse, int M, int N, int block, int i0,
+                            int j0, int k0, int nmesh, float* __restrict__ out) {
+    const int64_t total = (int64_t)nmesh * nmesh * nmesh;
+    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * FT) {
+        const int li = (int)(t % nmesh) + 1;
+        const int64_t rem = t / nmesh;
+        const int lj = (int)(rem % nmesh) + 1, lk = (int)(rem / nmesh) + 1;
+        const int gi = mod1(i0 + li - 1, N), gj = mod1(j0 + lj - 1, N), gk = mod1(k0 + lk - 1, N);
+        const double fx = ((double)gi - 0.5) / (double)block + 0.5;
+        const double fy = ((double)gj - 0.5) / (double)block + 0.5;
+        const double fz = ((double)gk - 0.5) / (double)block + 0.5;
+        const int ix = (int)floor(fx), iy = (int)floor(fy), iz = (int)floor(fz);
+        float wx[4], wy[4], wz[4];
+        catmull_rom((float)(fx - (double)ix), wx);
+        catmull_rom((float)(fy - (double)iy), wy);
+        catmull_rom((float)(fz - (double)iz), wz);
+        int xs[4], ys[4], zs[4];
+        for (int q = 0; q < 4; ++q) {
+            xs[q] = mod1(ix - 1 + q, M) - 1;
+            ys[q] = mod1(iy - 1 + q, M) - 1;
+            zs[q] = mod1(iz - 1 + q, M) - 1;
+        }
+        float val = 0.0f;
+        for (int c = 0; c < 4; ++c) {
+            const int64_t zoff = (int64_t)M * M * zs[c];
+            for (int b = 0; b < 4; ++b) {
+                const int64_t yoff = zoff + (int64_t)M * ys[b];
+                for (int a = 0; a < 4; ++a)
+                    val += wx[a] * wy[b] * wz[c] * coarse[yoff + xs[a]];
+            }
+        }
+        out[t] = val;
+    }
+}
+
+// hipFFT R2C / C2R plans of an n^3 real field (cached per size on the context)
+hnt_status plans_for(hnt_ctx* ctx, int n, hipfftHandle& r2c, hipfftHandle& c2r) {
+    for (hnt_ctx::FftPlans& p : ctx->fs_plans)
+        if (p.n == n) {
+            r2c = (hipfftHandle)p.r2c;
+            c2r = (hipfftHandle)p.c2r;
+            HNT_FFT(hipfftSetStream(r2c, ctx->stream));
+            HNT_FFT(hipfftSetStream(c2r, ctx->stream));
+            return HNT_OK;
+        }
+    // keep at most two sizes (the padded and the periodic tile)
+    if (ctx->fs_plans.size() >= 2) release_field_plans(ctx);
+    r2c = nullptr;
+    c2r = nullptr;
+    HNT_FFT(hipfftPlan3d(&r2c, n, n, n, HIPFFT_R2C));
+    const hipfftResult rc = hipfftPlan3d(&c2r, n, n, n, HIPFFT_C2R);
+    if (rc != HIPFFT_SUCCESS) {
+        (void)hipfftDestroy(r2c);
+        set_error("hipfftPlan3d C2R %d^3: hipFFT status %d", n, (int)rc);
+        return HNT_ERR_DEVICE;
+    }
+    ctx->fs_plans.push_back({n, (void*)r2c, (void*)c2r});
+    HNT_FFT(hipfftSetStream(r2c, ctx->stream));
+    HNT_FFT(hipfftSetStream(c2r, ctx->stream));
+    return HNT_OK;
+}
+
+int64_t cube(int n) { return (int64_t)n * n * n; }
+int64_t half_spectrum(int n) { return (int64_t)(n / 2 + 1) * n * n; }
+
+// Zero-pad a device n^3 field to (2n)^3 in fs_real and transform it into fs_base.
+hnt_status padded_rfft(hnt_ctx* ctx, const float* f, int nin, int n2, hipfftHandle r2c) {
+    hipStream_t st = ctx->stream;
+    HNT_TRY(reserve(ctx, ctx->fs_real, sizeof(float) * (size_t)cube(n2)));
+    HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)half_spectrum(n2)));
+    hipLaunchKernelGGL(fs_pad, dim3(grid_for(cube(n2))), dim3(FT), 0, st, f, nin, n2,
+                       (float*)ctx->fs_real.p);
+    HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)ctx->fs_real.p, (hipfftComplex*)ctx->fs_base.p));
+    return HNT_OK;
+}
+
+}  // namespace
+
+extern "C" hnt_status hnt_isolated_convolve(hnt_ctx* ctx, const float* noise, int32_t n_input,
+                                            int32_t n, double boxsize_local,
+                                            const hnt_pk_table* table,
+                                            const hnt_transfer* kernels, int32_t nkernels,
+                                            int32_t nshell, float* const* out) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(noise && table && table->sqrt_pk && kernels && out,
+                "hnt_isolated_convolve: NULL argument");
+    HNT_REQUIRE(n >= 1 && nshell >= 0 && n_input == n + 2 * nshell && n_input <= 2048,
+                "hnt_isolated_convolve: n_input %d must be n + 2 nshell (n %d, nshell %d), "
+                "at most 2048", n_input, n, nshell);
+    HNT_REQUIRE(boxsize_local > 0.0 && table->n >= 2 && table->d_log_k > 0.0f,
+                "hnt_isolated_convolve: boxsize %g, table of %d points, step %g", boxsize_local,
+                table->n, (double)table->d_log_k);
+    HNT_REQUIRE(nkernels >= 1, "hnt_isolated_convolve: no transfer");
+    for (int q = 0; q < nkernels; ++q) {
+        const hnt_transfer& k = kernels[q];
+        HNT_REQUIRE(out[q] && k.id >= 0 && k.id <= 5 &&
+                        ((k.id != 1 && k.id != 2 && k.id != 3) || (k.dim1 >= 1 && k.dim1 <= 3)) &&
+                        (k.id != 3 || (k.dim2 >= 1 && k.dim2 <= 3)),
+                    "hnt_isolated_convolve: transfer %d: id %d dims %d %d", q, k.id, k.dim1,
+                    k.dim2);
+    }
+    hipStream_t st = ctx->stream;
+    const int n2 = 2 * n_input;
+    const double dx = boxsize_local / n;
+    const double dk = 2.0 * M_PI / (n2 * dx);
+    hipfftHandle r2c, c2r;
+    HNT_TRY(plans_for(ctx, n2, r2c, c2r));
+    HNT_TRY(reserve(ctx, ctx->fs_tab, sizeof(float) * (size_t)table->n));
+    HNT_HIP(hipMemcpyAsync(ctx->fs_tab.p, table->sqrt_pk, sizeof(float) * (size_t)table->n,
+                           hipMemcpyHostToDevice, st));
+    HNT_TRY(padded_rfft(ctx, noise, n_input, n2, r2c));
+    HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
+    const int64_t nh = half_spectrum(n2);
+    for (int q = 0; q < nkernels; ++q) {
+        const hnt_transfer& k = kernels[q];
+        hipLaunchKernelGGL(fs_transfer, dim3(grid_for(nh)), dim3(FT), 0, st,
+                           (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2, dk,
+                           (const float*)ctx->fs_tab.p, table->n, table->log_k_min,
+                           table->d_log_k, k.id, k.dim1, k.dim2);
+        hipLaunchKernelGGL(fs_hermitize, dim3(grid_for((int64_t)n2 * n2)), dim3(FT), 0, st,
+                           (float2*)ctx->fs_work.p, n2);
+        HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fs_work.p, (hipfftReal*)ctx->fs_real.p));
+        hipLaunchKernelGGL(fs_crop, dim3(grid_for(cube(n))), dim3(FT), 0, st,
+                           (const float*)ctx->fs_real.p, n2, n, nshell,
+                           (float)(1.0 / (double)cube(n2)), out[q]);
+    }
+    return finish_call(ctx, "hnt_isolated_convolve");
+}
+
+extern "C" hnt_status hnt_compute_2lpt(hnt_ctx* ctx, const float* delta, int32_t n,
+                                       double boxsize_local, float* const* psi2) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(delta && psi2 && psi2[0] && psi2[1] && psi2[2], "hnt_compute_2lpt: NULL argument");
+    HNT_REQUIRE(n >= 2 && n <= 4096 && boxsize_local > 0.0, "hnt_compute_2lpt: n %d, boxsize %g",
+                n, boxsize_local);
+    hipStream_t st = ctx->stream;
+    const int64_t n3 = cube(n), nh = half_spectrum(n);
+    const double dk = 2.0 * M_PI / boxsize_local;
+    hipfftHandle r2c, c2r;
+    HNT_TRY(plans_for(ctx, n, r2c, c2r));
+    HNT_TRY(reserve(ctx, ctx->fs_real, sizeof(float) * (size_t)n3));
+    HNT_TRY(reserve(ctx, ctx->fs_aux, sizeof(float) * (size_t)n3));
+    HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)nh));
+    HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)nh));
+    float* real = (float*)ctx->fs_real.p;
+    float* src2 = (float*)ctx->fs_aux.p;
+    float2* base = (float2*)ctx->fs_base.p;
+    float2* work = (float2*)ctx->fs_work.p;
+    const unsigned g3 = grid_for(n3), gh = grid_for(nh);
+    // the transform may use its input as scratch: delta is copied first
+    HNT_HIP(hipMemcpyAsync(real, delta, sizeof(float) * (size_t)n3, hipMemcpyDeviceToDevice, st));
+    HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)real, (hipfftComplex*)base));
+    hipLaunchKernelGGL(fs_src2_init, dim3(g3), dim3(FT), 0, st, delta, n3, src2);
+    const float inv = (float)(1.0 / (double)n3);
+    // src2 = delta^2/2 - sum_i phi_ii^2/2 - sum_{i<j} phi_ij^2 (compute_2lpt_gpu's order)
+    const int pairs[6][2] = {{1, 1}, {2, 2}, {3, 3}, {1, 2}, {1, 3}, {2, 3}};
+    for (int q = 0; q < 6; ++q) {
+        hipLaunchKernelGGL(fs_periodic, dim3(gh), dim3(FT), 0, st, (const float2*)base, work, n,
+                           dk, 3, pairs[q][0], pairs[q][1]);
+        HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)real));
+        hipLaunchKernelGGL(fs_src2_sub, dim3(g3), dim3(FT), 0, st, (const float*)real, n3, inv,
+                           q < 3 ? 0.5f : 1.0f, src2);
+    }
+    HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)src2, (hipfftComplex*)base));
+    for (int d = 0; d < 3; ++d) {
+        hipLaunchKernelGGL(fs_periodic, dim3(gh), dim3(FT), 0, st, (const float2*)base, work, n,
+                           dk, 2, d + 1, 0);
+        HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)psi2[d]));
+        hipLaunchKernelGGL(fs_crop, dim3(g3), dim3(FT), 0, st, (const float*)psi2[d], n, n, 0,
+                           inv, psi2[d]);
+    }
+    return finish_call(ctx, "hnt_compute_2lpt");
+}
+
+extern "C" hnt_status hnt_compute_laplacian(hnt_ctx* ctx, const float* delta, int32_t n,
+                                            double boxsize_local, float* lapd) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(delta && lapd, "hnt_compute_laplacian: NULL argument");
+    HNT_REQUIRE(n >= 2 && n <= 4096 && boxsize_local > 0.0,
+                "hnt_compute_laplacian: n %d, boxsize %g", n, boxsize_local);
+    hipStream_t st = ctx->stream;
+    const int64_t n3 = cube(n), nh = half_spectrum(n);
+    hipfftHandle r2c, c2r;
+    HNT_TRY(plans_for(ctx, n, r2c, c2r));
+    HNT_TRY(reserve(ctx, ctx->fs_real, sizeof(float) * (size_t)n3));
+    HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)nh));
+    HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)nh));
+    float* real = (float*)ctx->fs_real.p;
+    float2* base = (float2*)ctx->fs_base.p;
+    float2* work = (float2*)ctx->fs_work.p;
+    HNT_HIP(hipMemcpyAsync(real, delta, sizeof(float) * (size_t)n3, hipMemcpyDeviceToDevice, st));
+    HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)real, (hipfftComplex*)base));
+    hipLaunchKernelGGL(fs_periodic, dim3(grid_for(nh)), dim3(FT), 0, st, (const float2*)base,
+                       work, n, 2.0 * M_PI / boxsize_local, 4, 0, 0);
+    HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)lapd));
+    hipLaunchKernelGGL(fs_crop, dim3(grid_for(n3)), dim3(FT), 0, st, (const float*)lapd, n, n, 0,
+                       (float)(1.0 / (double)n3), lapd);
+    return finish_call(ctx, "hnt_compute_laplacian");
+}
+
+extern "C" hnt_status hnt_isolated_poisson_psi(hnt_ctx* ctx, const float* dS, int32_t n,
+                                               double boxsize_local, float* const* psi) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(dS && psi && psi[0] && psi[1] && psi[2], "hnt_isolated_poisson_psi: NULL argument");
+    HNT_REQUIRE(n >= 1 && n <= 2048 && boxsize_local > 0.0,
+                "hnt_isolated_poisson_psi: n %d, boxsize %g", n, boxsize_local);
+    hipStream_t st = ctx->stream;
+    const int n2 = 2 * n;
+    hipfftHandle r2c, c2r;
+    HNT_TRY(plans_for(ctx, n2, r2c, c2r));
+    HNT_TRY(padded_rfft(ctx, dS, n, n2, r2c));
+    HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
+    const double dk = 2.0 * M_PI / (2.0 * boxsize_local);
+    for (int d = 0; d < 3; ++d) {
+        hipLaunchKernelGGL(fs_periodic, dim3(grid_for(half_spectrum(n2))), dim3(FT), 0, st,
+                           (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2, dk, 2,
+                           d + 1, 0);
+        HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fs_work.p, (hipfftReal*)ctx->fs_real.p));
+        // the 1LPT displacement is the negative of kernel 2's output
+        hipLaunchKernelGGL(fs_crop, dim3(grid_for(cube(n))), dim3(FT), 0, st,
+                           (const float*)ctx->fs_real.p, n2, n, 0,
+                           (float)(-1.0 / (double)cube(n2)), psi[d]);
+    }
+    return finish_call(ctx, "hnt_isolated_poisson_psi");
+}
+
+extern "C" hnt_status hnt_isolated_short_pot(hnt_ctx* ctx, const float* delta, int32_t n,
+                                             double boxsize_local, double rs, float* out) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(delta && out, "hnt_isolated_short_pot: NULL argument");
+    HNT_REQUIRE(n >= 1 && n <= 2048 && boxsize_local > 0.0 && rs >= 0.0,
+                "hnt_isolated_short_pot: n %d, boxsize %g, rs %g", n, boxsize_local, rs);
+    hipStream_t st = ctx->stream;
+    const int n2 = 2 * n;
+    hipfftHandle r2c, c2r;
+    HNT_TRY(plans_for(ctx, n2, r2c, c2r));
+    HNT_TRY(padded_rfft(ctx, delta, n, n2, r2c));
+    hipLaunchKernelGGL(fs_short_pot, dim3(grid_for(half_spectrum(n2))), dim3(FT), 0, st,
+                       (float2*)ctx->fs_base.p, n2, 2.0 * M_PI / (2.0 * boxsize_local), rs * rs);
+    HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fs_base.p, (hipfftReal*)ctx->fs_real.p));
+    hipLaunchKernelGGL(fs_crop, dim3(grid_for(cube(n))), dim3(FT), 0, st,
+                       (const float*)ctx->fs_real.p, n2, n, 0, (float)(1.0 / (double)cube(n2)),
+                       out);
+    return finish_call(ctx, "hnt_isolated_short_pot");
+}
+
+extern "C" hnt_status hnt_interpolate_to_tile(hnt_ctx* ctx, const float* coarse, int32_t M,
+                                              int32_t N, int32_t it, int32_t jt, int32_t kt,
+                                              int32_t nsub, int32_t nmesh, float* tile) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(coarse && tile, "hnt_interpolate_to_tile: NULL argument");
+    HNT_REQUIRE(M >= 1 && N >= M && N % M == 0 && nmesh >= 1 && nsub >= 1 && nmesh >= nsub &&
+                    nmesh <= 4096 && N < (1 << 30),
+                "hnt_interpolate_to_tile: M %d, N %d, nsub %d, nmesh %d", M, N, nsub, nmesh);
+    HNT_REQUIRE(it >= 1 && jt >= 1 && kt >= 1, "hnt_interpolate_to_tile: tile (%d, %d, %d)", it,
+                jt, kt);
+    const int block = N / M;
+    const int nbuff = (nmesh - nsub) / 2;
+    const int64_t i0 = (int64_t)(it - 1) * nsub + 1 - nbuff;
+    const int64_t j0 = (int64_t)(jt - 1) * nsub + 1 - nbuff;
+    const int64_t k0 = (int64_t)(kt - 1) * nsub + 1 - nbuff;
+    HNT_REQUIRE(std::max(i0, std::max(j0, k0)) + nmesh < (1LL << 31),
+                "hnt_interpolate_to_tile: tile start beyond Int32");
+    hipLaunchKernelGGL(fs_tricubic, dim3(grid_for(cube(nmesh))), dim3(FT), 0, ctx->stream, coarse,
+                       M, N, block, (int)i0, (int)j0, (int)k0, nmesh, tile);
+    return finish_call(ctx, "hnt_interpolate_to_tile");
+}

@@ -104,6 +104,11 @@ struct hnt_ctx {
     hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2;
     hnt::DevBuf rec_tabs, rec_keep, rec_slot;  // halo records: tables, flags, slots
     hnt::MergeBufs mg;                         // catalog merge
+    // tile field synthesis: hipFFT plan pairs by size {n, r2c, c2r}, real and two complex
+    // scratch arrays, the uploaded sqrt P(k) table
+    struct FftPlans { int n; void* r2c; void* c2r; };
+    std::vector<FftPlans> fs_plans;
+    hnt::DevBuf fs_real, fs_base, fs_work, fs_aux, fs_tab;
     void* fft_r2c = nullptr;  // hipfftHandle of an n^3 real-to-complex plan
     void* fft_c2r = nullptr;
     int fft_n = 0;
@@ -127,6 +132,7 @@ hnt_status reserve(hnt_ctx* ctx, DevBuf& b, size_t bytes);
 void release(DevBuf& b);
 void release_work(SAWork& w);
 void release_fft(hnt_ctx* ctx);  // FFT plans of hnt_peak_find_tile
+void release_field_plans(hnt_ctx* ctx);  // FFT plans of the field-synthesis entries
 hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t);
 hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);

@@ -376,6 +376,14 @@ void release_fft(hnt_ctx* ctx) {
     ctx->fft_r2c = ctx->fft_c2r = nullptr;
     ctx->fft_n = 0;
 }
+
+void release_field_plans(hnt_ctx* ctx) {
+    for (hnt_ctx::FftPlans& p : ctx->fs_plans) {
+        if (p.r2c) (void)hipfftDestroy((hipfftHandle)p.r2c);
+        if (p.c2r) (void)hipfftDestroy((hipfftHandle)p.c2r);
+    }
+    ctx->fs_plans.clear();
+}
 }  // namespace hnt
 
 #define HNT_FFT(expr)                                                                   \

@@ -116,6 +116,19 @@ class FilterBank(C.Structure):
     ]
 
 
+class PkTable(C.Structure):
+    _fields_ = [
+        ("sqrt_pk", _f32p),
+        ("n", C.c_int32),
+        ("log_k_min", C.c_float),
+        ("d_log_k", C.c_float),
+    ]
+
+
+class Transfer(C.Structure):
+    _fields_ = [("id", C.c_int32), ("dim1", C.c_int32), ("dim2", C.c_int32)]
+
+
 class CosmoTables(C.Structure):
     _fields_ = [
         ("ntab", C.c_int32),
@@ -215,6 +228,7 @@ def as_ptr(addr: int, ctype):
 EXPORTED = (
     "hnt_ctx_create",
     "hnt_ctx_destroy",
+    "hnt_ctx_release_scratch",
     "hnt_ctx_set_stream",
     "hnt_last_error",
     "hnt_abi_version",
@@ -233,6 +247,12 @@ EXPORTED = (
     "hnt_plan_destroy",
     "hnt_pack_records",
     "hnt_merge_catalog",
+    "hnt_isolated_convolve",
+    "hnt_compute_2lpt",
+    "hnt_compute_laplacian",
+    "hnt_isolated_poisson_psi",
+    "hnt_isolated_short_pot",
+    "hnt_interpolate_to_tile",
     "hnt_shell_gather_full",
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
@@ -269,6 +289,7 @@ def load(path: str | None = None):
     vp = C.c_void_p
     lib.hnt_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
     lib.hnt_ctx_destroy.argtypes = [vp]
+    lib.hnt_ctx_release_scratch.argtypes = [vp]
     lib.hnt_ctx_set_stream.argtypes = [vp, vp]
     lib.hnt_last_error.argtypes = [C.c_char_p, C.c_size_t]
     lib.hnt_last_error.restype = C.c_size_t
@@ -290,6 +311,15 @@ def load(path: str | None = None):
                                      C.c_int64, C.POINTER(C.c_int64)]
     lib.hnt_merge_catalog.argtypes = [vp, _f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                       _f32p, C.POINTER(C.c_int64)]
+    fpp = C.POINTER(_f32p)
+    lib.hnt_isolated_convolve.argtypes = [vp, _f32p, C.c_int32, C.c_int32, C.c_double,
+                                          C.POINTER(PkTable), C.POINTER(Transfer), C.c_int32,
+                                          C.c_int32, fpp]
+    lib.hnt_compute_2lpt.argtypes = [vp, _f32p, C.c_int32, C.c_double, fpp]
+    lib.hnt_compute_laplacian.argtypes = [vp, _f32p, C.c_int32, C.c_double, _f32p]
+    lib.hnt_isolated_poisson_psi.argtypes = [vp, _f32p, C.c_int32, C.c_double, fpp]
+    lib.hnt_isolated_short_pot.argtypes = [vp, _f32p, C.c_int32, C.c_double, C.c_double, _f32p]
+    lib.hnt_interpolate_to_tile.argtypes = [vp, _f32p] + [C.c_int32] * 7 + [_f32p]
     lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                       C.POINTER(ShellTables), C.POINTER(CTab),
                                       C.POINTER(Params), C.POINTER(Results), _i8p]

@@ -63,6 +63,10 @@ class Context:
         for k, v in knobs.items():
             abi.check(self.lib, self.lib.hnt_ctx_set_tuning(self.handle, abi.TUNING[k], int(v)))
 
+    def release_scratch(self):
+        """Free the device scratch and FFT plans kept between calls (hnt_ctx_release_scratch)."""
+        abi.check(self.lib, self.lib.hnt_ctx_release_scratch(self.handle))
+
     def set_profiling(self, on: bool = True):
         abi.check(self.lib, self.lib.hnt_ctx_set_profiling(self.handle, int(bool(on))))
 
@@ -795,3 +799,149 @@ def kernel_strain_gpu(rad, Gshell, Gshellf, SRshell, mp, mlow, mupp, wRnor, aRno
                                                  hlatt_2, d(out)))
     E = np.array([[out[0], out[1], out[2]], [out[1], out[3], out[4]], [out[2], out[4], out[5]]])
     return E, out[6:9].copy(), out[9:12].copy()
+
+
+# ---------------------------------------------------------------- tile field synthesis
+
+def pk_table(pk, boxsize_local: float, n: int, n_input: int, pk_table_n: int = 8192):
+    """The sqrt(P(k) dk^3 n2^3) table of isolated_convolve_gpu (ext/CUDAExt.jl:2489-2497):
+    pk_table_n points uniform in ln k from ln(dk/2) to ln(1.5 sqrt(3) (n2/2) dk), n2 = 2
+    n_input, dk = 2 pi / (n2 boxsize_local / n).  Returns (float32 values, log_k_min, d_log_k)
+    with both scalars rounded to float32, as the reference passes them."""
+    n2 = 2 * int(n_input)
+    dx = float(boxsize_local) / int(n)
+    dk = 2.0 * np.pi / (n2 * dx)
+    k_max = np.sqrt(3.0) * (n2 / 2) * dk
+    log_k = np.linspace(np.log(dk * 0.5), np.log(k_max * 1.5), int(pk_table_n))
+    norm = dk ** 3 * float(n2) ** 3
+    kk = np.exp(log_k)
+    try:
+        pv = np.asarray(pk(kk), dtype=np.float64)
+        if pv.shape != kk.shape:
+            raise ValueError
+    except Exception:
+        pv = np.array([float(pk(k)) for k in kk])
+    vals = np.sqrt(pv * norm).astype(np.float32)
+    return vals, np.float32(log_k[0]), np.float32(log_k[1] - log_k[0])
+
+
+def _field_out(n, dev, k=1):
+    torch = _torch()
+    return [fortran_empty((n, n, n), torch.float32, dev) for _ in range(k)]
+
+
+def _host_or_device(outs, return_device):
+    if return_device:
+        return outs
+    return [o.cpu().numpy() for o in outs]
+
+
+def _fptrs(outs):
+    return (C.POINTER(C.c_float) * len(outs))(*[abi.as_ptr(o.data_ptr(), C.c_float)
+                                               for o in outs])
+
+
+def isolated_convolve_gpu_multi(noise, pk, boxsize_local, n, *, kernels, nshell=0,
+                                pk_table_n=8192, threads=256, return_device=False):
+    """Mirror of isolated_convolve_gpu_multi (ext/CUDAExt.jl:2538-2622): one zero-padded
+    forward transform of the noise, then per (kernel_fn_id, dim1, dim2) the transfer, the
+    Hermitian DC/Nyquist planes, the inverse transform and the n^3 crop (hnt_isolated_convolve).
+    noise: (n_input,)*3 indexed [i, j, k], host or device.  Returns one array per kernel
+    (host numpy, or column-major device tensors with return_device)."""
+    torch = _require_gpu()
+    x = to_device_field(noise)
+    n_input = int(x.shape[0])
+    dev = x.device
+    ctx = context(dev.index or 0)
+    vals, lkmin, dlk = pk_table(pk, boxsize_local, n, n_input, pk_table_n)
+    tab = abi.PkTable(abi.as_ptr(vals.ctypes.data, C.c_float), len(vals), float(lkmin),
+                      float(dlk))
+    ks = list(kernels)
+    tr = (abi.Transfer * len(ks))(*[abi.Transfer(int(a), int(b), int(c)) for a, b, c in ks])
+    outs = _field_out(int(n), dev, len(ks))
+    abi.check(ctx.lib, ctx.lib.hnt_isolated_convolve(
+        ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float), n_input, int(n),
+        float(boxsize_local), C.byref(tab), tr, len(ks), int(nshell), _fptrs(outs)))
+    del threads
+    return _host_or_device(outs, return_device)
+
+
+def isolated_convolve_gpu(noise, pk, boxsize_local, n, *, kernel_fn_id=0, dim1=0, dim2=0,
+                          nshell=0, pk_table_n=8192, threads=256, return_device=False):
+    """Mirror of isolated_convolve_gpu (ext/CUDAExt.jl:2475-2536)."""
+    return isolated_convolve_gpu_multi(noise, pk, boxsize_local, n,
+                                       kernels=[(kernel_fn_id, dim1, dim2)], nshell=nshell,
+                                       pk_table_n=pk_table_n, threads=threads,
+                                       return_device=return_device)[0]
+
+
+def compute_2lpt_gpu(delta_tile, nmesh, boxsize_local, *, threads=256, return_device=False):
+    """Mirror of compute_2lpt_gpu (ext/CUDAExt.jl:2724-2791): the three periodic 2LPT
+    displacements of the tile's delta (hnt_compute_2lpt)."""
+    _require_gpu()
+    x = to_device_field(delta_tile)
+    if tuple(x.shape) != (int(nmesh),) * 3:
+        raise ValueError(f"delta_tile must be {nmesh}^3, got {tuple(x.shape)}")
+    ctx = context(x.device.index or 0)
+    outs = _field_out(int(nmesh), x.device, 3)
+    abi.check(ctx.lib, ctx.lib.hnt_compute_2lpt(ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float),
+                                                int(nmesh), float(boxsize_local), _fptrs(outs)))
+    return _host_or_device(outs, return_device)
+
+
+def compute_laplacian_gpu(delta_tile, nmesh, boxsize_local, *, threads=256,
+                          return_device=False):
+    """Mirror of compute_laplacian_gpu (ext/CUDAExt.jl:2793-2825): irfft(k^2 rfft(delta))."""
+    _require_gpu()
+    x = to_device_field(delta_tile)
+    if tuple(x.shape) != (int(nmesh),) * 3:
+        raise ValueError(f"delta_tile must be {nmesh}^3, got {tuple(x.shape)}")
+    ctx = context(x.device.index or 0)
+    out = _field_out(int(nmesh), x.device)[0]
+    abi.check(ctx.lib, ctx.lib.hnt_compute_laplacian(
+        ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float), int(nmesh), float(boxsize_local),
+        abi.as_ptr(out.data_ptr(), C.c_float)))
+    return out if return_device else out.cpu().numpy()
+
+
+def isolated_poisson_psi_gpu(dS, boxsize_local, *, threads=256):
+    """Mirror of isolated_poisson_psi_gpu (ext/CUDAExt.jl:2679-2701): a tuple of three
+    column-major device tensors."""
+    _require_gpu()
+    x = to_device_field(dS)
+    n = int(x.shape[0])
+    ctx = context(x.device.index or 0)
+    outs = _field_out(n, x.device, 3)
+    abi.check(ctx.lib, ctx.lib.hnt_isolated_poisson_psi(
+        ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float), n, float(boxsize_local), _fptrs(outs)))
+    return tuple(outs)
+
+
+def isolated_short_pot_gpu(delta, boxsize_local, rs):
+    """Mirror of isolated_short_pot_gpu (ext/CUDAExt.jl:2703-2722): a column-major device
+    tensor."""
+    _require_gpu()
+    x = to_device_field(delta)
+    n = int(x.shape[0])
+    ctx = context(x.device.index or 0)
+    out = _field_out(n, x.device)[0]
+    abi.check(ctx.lib, ctx.lib.hnt_isolated_short_pot(
+        ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float), n, float(boxsize_local), float(rs),
+        abi.as_ptr(out.data_ptr(), C.c_float)))
+    return out
+
+
+def interpolate_to_tile_gpu(coarse, it, jt, kt, nsub, nmesh, N, M, *, threads=256,
+                            return_device=False):
+    """Mirror of interpolate_to_tile_gpu (ext/CUDAExt.jl:3142-3176): tri-cubic Catmull-Rom
+    interpolation of the periodic M^3 coarse grid onto tile (it, jt, kt) (1-based)."""
+    _require_gpu()
+    x = to_device_field(coarse)
+    if tuple(x.shape) != (int(M),) * 3:
+        raise ValueError(f"coarse must be {M}^3, got {tuple(x.shape)}")
+    ctx = context(x.device.index or 0)
+    out = _field_out(int(nmesh), x.device)[0]
+    abi.check(ctx.lib, ctx.lib.hnt_interpolate_to_tile(
+        ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float), int(M), int(N), int(it), int(jt),
+        int(kt), int(nsub), int(nmesh), abi.as_ptr(out.data_ptr(), C.c_float)))
+    return out if return_device else out.cpu().numpy()

@@ -55,6 +55,11 @@ typedef struct hnt_ctx hnt_ctx;
 /* Context on HIP device `device` (0-based).  Replaces set_cuda_device! (src/Huntian.jl:414-424). */
 hnt_status hnt_ctx_create(int device, hnt_ctx** out);
 hnt_status hnt_ctx_destroy(hnt_ctx* ctx);
+/* Free the device scratch the context keeps between calls (sized by the largest call so far:
+ * cell records, spectra, sort and hash buffers) and its FFT plans; uploaded tables and
+ * networks stay, and plans keep their own buffers.  No counterpart in the reference, which
+ * frees its temporaries after every call (CUDA.unsafe_free!). */
+hnt_status hnt_ctx_release_scratch(hnt_ctx* ctx);
 /* Run subsequent calls on the caller's hipStream_t, used as given (NULL is the device's
  * default stream).  A new context uses a stream of its own until this is called. */
 hnt_status hnt_ctx_set_stream(hnt_ctx* ctx, void* hip_stream);
@@ -387,6 +392,81 @@ hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch, const hnt_resu
 hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int64_t nhalo,
                              int32_t nfields, int32_t volume_reduction, int32_t fortran_ties,
                              float* out, int64_t* nout);
+
+/* ---------------------------------------------------------------- tile field synthesis */
+/* The GPU methods that build a tile's input fields (src/MultiResolution.jl:881-1010 calls
+ * them through the stubs of src/Huntian.jl:283-385).  Fields are device Float32 arrays,
+ * column-major [i,j,k] (i fastest), as the reference's CuArrays.  Transforms use hipFFT;
+ * inverse transforms are normalised (1/n^3), as AbstractFFTs' irfft. */
+
+/* sqrt(P(k) dk^3 n2^3) tabulated at pk_table_n points uniform in ln k from ln(dk/2) to
+ * ln(1.5 sqrt(3) (n2/2) dk) (host array; built by the caller as isolated_convolve_gpu does,
+ * ext/CUDAExt.jl:2489-2497), with log_k_min = ln k of the first point and d_log_k the step,
+ * both as Float32. */
+typedef struct {
+    const float* sqrt_pk;
+    int32_t n;
+    float log_k_min;
+    float d_log_k;
+} hnt_pk_table;
+
+/* One output of an isolated convolution: id 0 delta (sqrt P), 1 1LPT (i k_d1/k^2 sqrt P),
+ * 2 2LPT (-i k_d1/k^2 sqrt P), 3 phi_ij (-k_d1 k_d2/k^2 sqrt P), 4 k^2 sqrt P,
+ * 5 -sqrt P/k^2; dims 1..3 (the reference's kernel_fn_id, dim1, dim2). */
+typedef struct {
+    int32_t id;
+    int32_t dim1;
+    int32_t dim2;
+} hnt_transfer;
+
+/* hnt_isolated_convolve -- replaces isolated_convolve_gpu and isolated_convolve_gpu_multi
+ * (ext/CUDAExt.jl:2346-2622; stubs src/Huntian.jl:283-324).  noise: device n_input^3
+ * (n_input = n + 2 nshell); the noise is zero-padded to n2 = 2 n_input, transformed once,
+ * and for each of the nkernels transfers: multiplied by the table's sqrt P(k) and the
+ * transfer (k = 0 set to 0), the kx = 0 and kx = Nyquist planes made Hermitian (conjugate
+ * pairs averaged, self-conjugate modes made real), inverse transformed, and the cells
+ * nshell+1 .. nshell+n of each axis written to out[q] (device n^3).  dk = 2 pi / (n2 dx),
+ * dx = boxsize_local / n.  Scratch kept by the context: n2^3 floats and two half spectra
+ * ((n2/2+1) n2^2 complex), ~100 GB at n_input = 1024. */
+hnt_status hnt_isolated_convolve(hnt_ctx* ctx, const float* noise, int32_t n_input, int32_t n,
+                                 double boxsize_local, const hnt_pk_table* table,
+                                 const hnt_transfer* kernels, int32_t nkernels, int32_t nshell,
+                                 float* const* out);
+
+/* hnt_compute_2lpt -- replaces compute_2lpt_gpu (ext/CUDAExt.jl:2724-2791; stub
+ * src/Huntian.jl:326-342): on the periodic n^3 tile, src2 = delta^2/2 - sum_i phi_ii^2/2 -
+ * sum_{i<j} phi_ij^2 with phi_ij = irfft(-k_i k_j/k^2 rfft(delta)) (k = 0 zeroed), then
+ * psi2[d] = irfft(-i k_d/k^2 rfft(src2)) with k = 0 and the Nyquist planes zeroed;
+ * dk = 2 pi / boxsize_local.  delta is not modified; psi2: three device n^3 outputs. */
+hnt_status hnt_compute_2lpt(hnt_ctx* ctx, const float* delta, int32_t n, double boxsize_local,
+                            float* const* psi2);
+
+/* hnt_compute_laplacian -- replaces compute_laplacian_gpu (ext/CUDAExt.jl:2793-2825; stub
+ * src/Huntian.jl:344-353): lapd = irfft(k^2 rfft(delta)) with k = 0 and the Nyquist planes
+ * zeroed (k^2, not -k^2, as the reference). */
+hnt_status hnt_compute_laplacian(hnt_ctx* ctx, const float* delta, int32_t n,
+                                 double boxsize_local, float* lapd);
+
+/* hnt_isolated_poisson_psi -- replaces isolated_poisson_psi_gpu (ext/CUDAExt.jl:2679-2701;
+ * stub src/Huntian.jl:355-363): psi[d] = the first n^3 cells of irfft(i k_d/k^2 rfft(dS
+ * zero-padded to 2n)), k = 0 and the padded Nyquist planes zeroed, dk = 2 pi/(2 boxsize). */
+hnt_status hnt_isolated_poisson_psi(hnt_ctx* ctx, const float* dS, int32_t n,
+                                    double boxsize_local, float* const* psi);
+
+/* hnt_isolated_short_pot -- replaces isolated_short_pot_gpu (ext/CUDAExt.jl:2703-2722; stub
+ * src/Huntian.jl:365-373): the first n^3 cells of irfft(-(1 - exp(-k^2 rs^2))/k^2 rfft(delta
+ * zero-padded to 2n)), -rs^2 at k = 0; rs in Mpc/h. */
+hnt_status hnt_isolated_short_pot(hnt_ctx* ctx, const float* delta, int32_t n,
+                                  double boxsize_local, double rs, float* out);
+
+/* hnt_interpolate_to_tile -- replaces interpolate_to_tile_gpu (ext/CUDAExt.jl:3062-3176;
+ * stub src/Huntian.jl:375-385): tri-cubic Catmull-Rom interpolation of the periodic M^3
+ * coarse grid (device) onto tile (it, jt, kt)'s nmesh^3 cells, cell-centred on both grids,
+ * in Float32 with the reference's operation order (block = N / M, nbuff = (nmesh-nsub)/2,
+ * the tile starts at global cell (it-1) nsub + 1 - nbuff, wrapped mod N). */
+hnt_status hnt_interpolate_to_tile(hnt_ctx* ctx, const float* coarse, int32_t M, int32_t N,
+                                   int32_t it, int32_t jt, int32_t kt, int32_t nsub,
+                                   int32_t nmesh, float* tile);
 
 /* ---------------------------------------------------------------- primitive entries */
 /* Single-primitive entry points; the reference exposes them as test harnesses

@@ -36,7 +36,8 @@ def test_struct_layouts_match_the_header(tmp_path):
                "hnt_peaks": abi.Peaks, "hnt_fields": abi.Fields, "hnt_batch": abi.Batch,
                "hnt_params": abi.Params, "hnt_results": abi.Results, "hnt_mlp": abi.Mlp,
                "hnt_filter_bank": abi.FilterBank, "hnt_cosmo_tables": abi.CosmoTables,
-               "hnt_pack_params": abi.PackParams}
+               "hnt_pack_params": abi.PackParams, "hnt_pk_table": abi.PkTable,
+               "hnt_transfer": abi.Transfer}
     prog = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
             'int main(void) {']
     for cname, py in structs.items():
