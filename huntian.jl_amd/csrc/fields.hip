@@ -53,45 +53,88 @@ __device__ __forceinline__ double pick(int d, double kx, double ky, double kz) {
     return d == 1 ? kx : (d == 2 ? ky : kz);
 }
 
+// Rows: every kernel below walks whole rows along x, one wavefront per row (the row index is
+// a 32-bit (iy, iz) pair, so no 64-bit division per element), lanes over x.
+constexpr int WAVES = FT / 64;
+
+#define FOR_ROWS(nrows)                                                                     \
+    for (int row = (int)blockIdx.x * WAVES + (int)(threadIdx.x >> 6); row < (nrows);        \
+         row += (int)gridDim.x * WAVES)
+
+unsigned grid_rows(int64_t nrows) {
+    return (unsigned)std::min<int64_t>((nrows + WAVES - 1) / WAVES, 1 << 20);
+}
+
 // _transfer_kernel! (ext/CUDAExt.jl:2367-2432): work = transfer(base), over the half spectrum
-// (nk = n2/2 + 1, x fastest).  ky, kz of index > n2/2 are negative (fftfreq).
+// (nk = n2/2 + 1, x fastest).  ky, kz of index >= n2/2 are negative (fftfreq).  The rows
+// (iy, iz), (-iy, iz), (iy, -iz), (-iy, -iz) share k^2 bit for bit (the negated frequency is
+// the exact negation), so one wavefront takes the group and looks sqrt P up once per ix.
 __global__ void fs_transfer(const float2* __restrict__ base, float2* __restrict__ work, int n2,
                             double dk, const float* __restrict__ tab, int ntab, float log_k_min,
                             float d_log_k, int id, int d1, int d2) {
     const int nk = n2 / 2 + 1, half = n2 / 2;
-    const int64_t total = (int64_t)nk * n2 * n2;
-    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * FT) {
-        const int ix = (int)(t % nk);
-        const int64_t rem = t / nk;
-        const int iy = (int)(rem % n2), iz = (int)(rem / n2);
-        const double kx = (double)ix * dk;
-        const double ky = (double)(iy < half ? iy : iy - n2) * dk;
-        const double kz = (double)(iz < half ? iz : iz - n2) * dk;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        if (k2 == 0.0) {
-            work[t] = make_float2(0.0f, 0.0f);
-            continue;
+    const int lane = threadIdx.x & 63;
+    FOR_ROWS((half + 1) * (half + 1)) {
+        const int ay = row % (half + 1), az = row / (half + 1);
+        // the group's rows: index and signed frequency; index half is -half and unpaired
+        int ry[2], rz[2];
+        double sy[2], sz[2];
+        int ny = 1, nz = 1;
+        ry[0] = ay;
+        sy[0] = ay < half ? (double)ay * dk : (double)(ay - n2) * dk;
+        if (ay != 0 && ay != half) {
+            ry[1] = n2 - ay;
+            sy[1] = (double)(-ay) * dk;
+            ny = 2;
         }
-        const double sqrt_pk =
-            (double)interp_pk(tab, (float)(log(k2) * 0.5), log_k_min, d_log_k, ntab);
-        const float2 v = base[t];
-        float2 o;
-        if (id == 0) {
-            o = make_float2((float)((double)v.x * sqrt_pk), (float)((double)v.y * sqrt_pk));
-        } else if (id == 1 || id == 2) {
-            const double coeff = sqrt_pk * pick(d1, kx, ky, kz) / k2;
-            const double a = v.x, b = v.y;
-            o = id == 1 ? make_float2((float)(-b * coeff), (float)(a * coeff))
-                        : make_float2((float)(b * coeff), (float)(-a * coeff));
-        } else {
-            float c;
-            if (id == 3) c = (float)(-sqrt_pk * pick(d1, kx, ky, kz) * pick(d2, kx, ky, kz) / k2);
-            else if (id == 4) c = (float)(sqrt_pk * k2);
-            else c = (float)(-sqrt_pk / k2);
-            o = make_float2(v.x * c, v.y * c);
+        rz[0] = az;
+        sz[0] = az < half ? (double)az * dk : (double)(az - n2) * dk;
+        if (az != 0 && az != half) {
+            rz[1] = n2 - az;
+            sz[1] = (double)(-az) * dk;
+            nz = 2;
         }
-        work[t] = o;
+        const double kyz = sy[0] * sy[0];
+        const double kzz = sz[0] * sz[0];
+        for (int ix = lane; ix < nk; ix += 64) {
+            const double kx = (double)ix * dk;
+            const double k2 = kx * kx + kyz + kzz;
+            const double sqrt_pk =
+                k2 == 0.0 ? 0.0
+                          : (double)interp_pk(tab, (float)(log(k2) * 0.5), log_k_min, d_log_k,
+                                              ntab);
+            for (int qz = 0; qz < nz; ++qz)
+                for (int qy = 0; qy < ny; ++qy) {
+                    const int64_t t = ((int64_t)ry[qy] + (int64_t)n2 * rz[qz]) * nk + ix;
+                    if (k2 == 0.0) {
+                        work[t] = make_float2(0.0f, 0.0f);
+                        continue;
+                    }
+                    const double ky = sy[qy], kz = sz[qz];
+                    const float2 v = base[t];
+                    float2 o;
+                    if (id == 0) {
+                        o = make_float2((float)((double)v.x * sqrt_pk),
+                                        (float)((double)v.y * sqrt_pk));
+                    } else if (id == 1 || id == 2) {
+                        const double coeff = sqrt_pk * pick(d1, kx, ky, kz) / k2;
+                        const double a = v.x, b = v.y;
+                        o = id == 1 ? make_float2((float)(-b * coeff), (float)(a * coeff))
+                                    : make_float2((float)(b * coeff), (float)(-a * coeff));
+                    } else {
+                        float c;
+                        if (id == 3)
+                            c = (float)(-sqrt_pk * pick(d1, kx, ky, kz) * pick(d2, kx, ky, kz) /
+                                        k2);
+                        else if (id == 4)
+                            c = (float)(sqrt_pk * k2);
+                        else
+                            c = (float)(-sqrt_pk / k2);
+                        o = make_float2(v.x * c, v.y * c);
+                    }
+                    work[t] = o;
+                }
+        }
     }
 }
 
@@ -129,32 +172,36 @@ __global__ void fs_hermitize(float2* __restrict__ a, int n2) {
 __global__ void fs_periodic(const float2* __restrict__ src, float2* __restrict__ dst, int n,
                             double dk, int id, int d1, int d2) {
     const int nk = n / 2 + 1, nyq = n / 2;
-    const int64_t total = (int64_t)nk * n * n;
-    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * FT) {
-        const int ix = (int)(t % nk);
-        const int64_t rem = t / nk;
-        const int iy = (int)(rem % n), iz = (int)(rem / n);
-        const double kx = (double)ix * dk;
+    const int lane = threadIdx.x & 63;
+    FOR_ROWS(n * n) {
+        const int iy = row % n, iz = row / n;
         const double ky = (double)(iy <= nyq ? iy : iy - n) * dk;
         const double kz = (double)(iz <= nyq ? iz : iz - n) * dk;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        if (k2 == 0.0 || (id != 3 && (ix == nk - 1 || iy == nyq || iz == nyq))) {
-            dst[t] = make_float2(0.0f, 0.0f);
-            continue;
-        }
-        const float2 v = src[t];
-        float2 o;
-        if (id == 2) {
-            const double coeff = pick(d1, kx, ky, kz) / k2;
-            const double a = v.x, b = v.y;
-            o = make_float2((float)(b * coeff), (float)(-a * coeff));
-        } else {
-            const float c = id == 3 ? (float)(-pick(d1, kx, ky, kz) * pick(d2, kx, ky, kz) / k2)
+        const double kyz = ky * ky;
+        const bool nyq_row = iy == nyq || iz == nyq;
+        const int64_t r0 = (int64_t)row * nk;
+        for (int ix = lane; ix < nk; ix += 64) {
+            const int64_t t = r0 + ix;
+            const double kx = (double)ix * dk;
+            const double k2 = kx * kx + kyz + kz * kz;
+            if (k2 == 0.0 || (id != 3 && (ix == nk - 1 || nyq_row))) {
+                dst[t] = make_float2(0.0f, 0.0f);
+                continue;
+            }
+            const float2 v = src[t];
+            float2 o;
+            if (id == 2) {
+                const double coeff = pick(d1, kx, ky, kz) / k2;
+                const double a = v.x, b = v.y;
+                o = make_float2((float)(b * coeff), (float)(-a * coeff));
+            } else {
+                const float c = id == 3
+                                    ? (float)(-pick(d1, kx, ky, kz) * pick(d2, kx, ky, kz) / k2)
                                     : (float)k2;
-            o = make_float2(v.x * c, v.y * c);
+                o = make_float2(v.x * c, v.y * c);
+            }
+            dst[t] = o;
         }
-        dst[t] = o;
     }
 }
 
@@ -162,48 +209,48 @@ __global__ void fs_periodic(const float2* __restrict__ src, float2* __restrict__
 // n2/2 counts as +n2/2 (only k^2 enters).
 __global__ void fs_short_pot(float2* __restrict__ a, int n2, double dk, double rs2) {
     const int nk = n2 / 2 + 1, half = n2 / 2;
-    const int64_t total = (int64_t)nk * n2 * n2;
-    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * FT) {
-        const int ix = (int)(t % nk);
-        const int64_t rem = t / nk;
-        const int iy = (int)(rem % n2), iz = (int)(rem / n2);
-        const double kx = (double)ix * dk;
+    const int lane = threadIdx.x & 63;
+    FOR_ROWS(n2 * n2) {
+        const int iy = row % n2, iz = row / n2;
         const double ky = (double)(iy <= half ? iy : iy - n2) * dk;
         const double kz = (double)(iz <= half ? iz : iz - n2) * dk;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        const float c = (float)(k2 == 0.0 ? -rs2 : -(1.0 - exp(-k2 * rs2)) / k2);
-        const float2 v = a[t];
-        a[t] = make_float2(v.x * c, v.y * c);
+        const double kyz = ky * ky;
+        const int64_t r0 = (int64_t)row * nk;
+        for (int ix = lane; ix < nk; ix += 64) {
+            const double kx = (double)ix * dk;
+            const double k2 = kx * kx + kyz + kz * kz;
+            const float c = (float)(k2 == 0.0 ? -rs2 : -(1.0 - exp(-k2 * rs2)) / k2);
+            const float2 v = a[r0 + ix];
+            a[r0 + ix] = make_float2(v.x * c, v.y * c);
+        }
     }
 }
 
-// dst (n2^3) = src (nin^3) in the low corner, zeros elsewhere; one thread per 4 x-cells
-// (n2 is even and a multiple of 4 when nin is even; else one cell per thread).
+// dst (n2^3) = src (nin^3) in the low corner, zeros elsewhere
 __global__ void fs_pad(const float* __restrict__ src, int nin, int n2, float* __restrict__ dst) {
-    const int64_t total = (int64_t)n2 * n2 * n2;
-    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * FT) {
-        const int ix = (int)(t % n2);
-        const int64_t rem = t / n2;
-        const int iy = (int)(rem % n2), iz = (int)(rem / n2);
-        dst[t] = (ix < nin && iy < nin && iz < nin)
-                     ? src[ix + (int64_t)nin * (iy + (int64_t)nin * iz)]
-                     : 0.0f;
+    const int lane = threadIdx.x & 63;
+    FOR_ROWS(n2 * n2) {
+        const int iy = row % n2, iz = row / n2;
+        float* d = dst + (int64_t)row * n2;
+        if (iy < nin && iz < nin) {
+            const float* sr = src + (int64_t)nin * (iy + (int64_t)nin * iz);
+            for (int ix = lane; ix < n2; ix += 64) d[ix] = ix < nin ? sr[ix] : 0.0f;
+        } else {
+            for (int ix = lane; ix < n2; ix += 64) d[ix] = 0.0f;
+        }
     }
 }
 
 // out (n^3) = scale * big[s + i, s + j, s + k] of an m^3 field (scale: the inverse
-// transform's 1/m^3, negated where the reference negates the result)
-// (in place when m == n and s == 0: each element is read and written by one thread)
+// transform's 1/m^3, negated where the reference negates the result); in place when m == n
+// and s == 0 (each element is read and written by one lane)
 __global__ void fs_crop(const float* big, int m, int n, int s, float scale, float* out) {
-    const int64_t total = (int64_t)n * n * n;
-    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * FT) {
-        const int i = (int)(t % n);
-        const int64_t rem = t / n;
-        const int j = (int)(rem % n), k = (int)(rem / n);
-        out[t] = big[(s + i) + (int64_t)m * ((s + j) + (int64_t)m * (s + k))] * scale;
+    const int lane = threadIdx.x & 63;
+    FOR_ROWS(n * n) {
+        const int j = row % n, k = row / n;
+        const float* b = big + (int64_t)m * ((s + j) + (int64_t)m * (s + k)) + s;
+        float* o = out + (int64_t)row * n;
+        for (int i = lane; i < n; i += 64) o[i] = b[i] * scale;
     }
 }
 
@@ -242,40 +289,41 @@ __device__ __forceinline__ void catmull_rom(float t, float w[4]) {
     w[3] = 0.5f * t3 - 0.5f * t2;
 }
 
-// _tricubic_kernel! (ext/CUDAExt.jl:3083-3140): one thread per tile cell
+// Catmull-Rom weights and wrapped coarse indices (0-based) of global fine cell g (1-based)
+__device__ __forceinline__ void stencil(int g, int block, int M, float w[4], int idx[4]) {
+    const double f = ((double)g - 0.5) / (double)block + 0.5;
+    const int i = (int)floor(f);
+    catmull_rom((float)(f - (double)i), w);
+    for (int q = 0; q < 4; ++q) idx[q] = mod1(i - 1 + q, M) - 1;
+}
+
+// _tricubic_kernel! (ext/CUDAExt.jl:3083-3140): one wavefront per tile row (lj, lk), whose y
+// and z weights and coarse rows are the same for every lane; lanes over li.  The sum runs
+// over (z, y, x) with ((wx wy) wz) f as the reference does.
 __global__ void fs_tricubic(const float* __restrict__ coarse, int M, int N, int block, int i0,
                             int j0, int k0, int nmesh, float* __restrict__ out) {
-    const int64_t total = (int64_t)nmesh * nmesh * nmesh;
-    for (int64_t t = (int64_t)blockIdx.x * FT + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * FT) {
-        const int li = (int)(t % nmesh) + 1;
-        const int64_t rem = t / nmesh;
-        const int lj = (int)(rem % nmesh) + 1, lk = (int)(rem / nmesh) + 1;
-        const int gi = mod1(i0 + li - 1, N), gj = mod1(j0 + lj - 1, N), gk = mod1(k0 + lk - 1, N);
-        const double fx = ((double)gi - 0.5) / (double)block + 0.5;
-        const double fy = ((double)gj - 0.5) / (double)block + 0.5;
-        const double fz = ((double)gk - 0.5) / (double)block + 0.5;
-        const int ix = (int)floor(fx), iy = (int)floor(fy), iz = (int)floor(fz);
-        float wx[4], wy[4], wz[4];
-        catmull_rom((float)(fx - (double)ix), wx);
-        catmull_rom((float)(fy - (double)iy), wy);
-        catmull_rom((float)(fz - (double)iz), wz);
-        int xs[4], ys[4], zs[4];
-        for (int q = 0; q < 4; ++q) {
-            xs[q] = mod1(ix - 1 + q, M) - 1;
-            ys[q] = mod1(iy - 1 + q, M) - 1;
-            zs[q] = mod1(iz - 1 + q, M) - 1;
-        }
-        float val = 0.0f;
-        for (int c = 0; c < 4; ++c) {
-            const int64_t zoff = (int64_t)M * M * zs[c];
-            for (int b = 0; b < 4; ++b) {
-                const int64_t yoff = zoff + (int64_t)M * ys[b];
-                for (int a = 0; a < 4; ++a)
-                    val += wx[a] * wy[b] * wz[c] * coarse[yoff + xs[a]];
+    const int lane = threadIdx.x & 63;
+    FOR_ROWS(nmesh * nmesh) {
+        const int lj = row % nmesh + 1, lk = row / nmesh + 1;
+        float wy[4], wz[4];
+        int ys[4], zs[4];
+        stencil(mod1(j0 + lj - 1, N), block, M, wy, ys);
+        stencil(mod1(k0 + lk - 1, N), block, M, wz, zs);
+        float* o = out + (int64_t)row * nmesh;
+        for (int li = lane + 1; li <= nmesh; li += 64) {
+            float wx[4];
+            int xs[4];
+            stencil(mod1(i0 + li - 1, N), block, M, wx, xs);
+            float val = 0.0f;
+            for (int c = 0; c < 4; ++c) {
+                const float* cz = coarse + (int64_t)M * M * zs[c];
+                for (int b = 0; b < 4; ++b) {
+                    const float* cy = cz + (int64_t)M * ys[b];
+                    for (int a = 0; a < 4; ++a) val += wx[a] * wy[b] * wz[c] * cy[xs[a]];
+                }
             }
+            o[li - 1] = val;
         }
-        out[t] = val;
     }
 }
 
@@ -314,7 +362,7 @@ hnt_status padded_rfft(hnt_ctx* ctx, const float* f, int nin, int n2, hipfftHand
     hipStream_t st = ctx->stream;
     HNT_TRY(reserve(ctx, ctx->fs_real, sizeof(float) * (size_t)cube(n2)));
     HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)half_spectrum(n2)));
-    hipLaunchKernelGGL(fs_pad, dim3(grid_for(cube(n2))), dim3(FT), 0, st, f, nin, n2,
+    hipLaunchKernelGGL(fs_pad, dim3(grid_rows((int64_t)n2 * n2)), dim3(FT), 0, st, f, nin, n2,
                        (float*)ctx->fs_real.p);
     HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)ctx->fs_real.p, (hipfftComplex*)ctx->fs_base.p));
     return HNT_OK;
@@ -359,14 +407,15 @@ extern "C" hnt_status hnt_isolated_convolve(hnt_ctx* ctx, const float* noise, in
     const int64_t nh = half_spectrum(n2);
     for (int q = 0; q < nkernels; ++q) {
         const hnt_transfer& k = kernels[q];
-        hipLaunchKernelGGL(fs_transfer, dim3(grid_for(nh)), dim3(FT), 0, st,
+        hipLaunchKernelGGL(fs_transfer, dim3(grid_rows((int64_t)(n2 / 2 + 1) * (n2 / 2 + 1))),
+                           dim3(FT), 0, st,
                            (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2, dk,
                            (const float*)ctx->fs_tab.p, table->n, table->log_k_min,
                            table->d_log_k, k.id, k.dim1, k.dim2);
         hipLaunchKernelGGL(fs_hermitize, dim3(grid_for((int64_t)n2 * n2)), dim3(FT), 0, st,
                            (float2*)ctx->fs_work.p, n2);
         HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fs_work.p, (hipfftReal*)ctx->fs_real.p));
-        hipLaunchKernelGGL(fs_crop, dim3(grid_for(cube(n))), dim3(FT), 0, st,
+        hipLaunchKernelGGL(fs_crop, dim3(grid_rows((int64_t)n * n)), dim3(FT), 0, st,
                            (const float*)ctx->fs_real.p, n2, n, nshell,
                            (float)(1.0 / (double)cube(n2)), out[q]);
     }
@@ -392,7 +441,7 @@ extern "C" hnt_status hnt_compute_2lpt(hnt_ctx* ctx, const float* delta, int32_t
     float* src2 = (float*)ctx->fs_aux.p;
     float2* base = (float2*)ctx->fs_base.p;
     float2* work = (float2*)ctx->fs_work.p;
-    const unsigned g3 = grid_for(n3), gh = grid_for(nh);
+    const unsigned g3 = grid_for(n3), gh = grid_rows((int64_t)n * n), gc = gh;
     // the transform may use its input as scratch: delta is copied first
     HNT_HIP(hipMemcpyAsync(real, delta, sizeof(float) * (size_t)n3, hipMemcpyDeviceToDevice, st));
     HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)real, (hipfftComplex*)base));
@@ -412,7 +461,7 @@ extern "C" hnt_status hnt_compute_2lpt(hnt_ctx* ctx, const float* delta, int32_t
         hipLaunchKernelGGL(fs_periodic, dim3(gh), dim3(FT), 0, st, (const float2*)base, work, n,
                            dk, 2, d + 1, 0);
         HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)psi2[d]));
-        hipLaunchKernelGGL(fs_crop, dim3(g3), dim3(FT), 0, st, (const float*)psi2[d], n, n, 0,
+        hipLaunchKernelGGL(fs_crop, dim3(gc), dim3(FT), 0, st, (const float*)psi2[d], n, n, 0,
                            inv, psi2[d]);
     }
     return finish_call(ctx, "hnt_compute_2lpt");
@@ -436,10 +485,10 @@ extern "C" hnt_status hnt_compute_laplacian(hnt_ctx* ctx, const float* delta, in
     float2* work = (float2*)ctx->fs_work.p;
     HNT_HIP(hipMemcpyAsync(real, delta, sizeof(float) * (size_t)n3, hipMemcpyDeviceToDevice, st));
     HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)real, (hipfftComplex*)base));
-    hipLaunchKernelGGL(fs_periodic, dim3(grid_for(nh)), dim3(FT), 0, st, (const float2*)base,
+    hipLaunchKernelGGL(fs_periodic, dim3(grid_rows((int64_t)n * n)), dim3(FT), 0, st, (const float2*)base,
                        work, n, 2.0 * M_PI / boxsize_local, 4, 0, 0);
     HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)lapd));
-    hipLaunchKernelGGL(fs_crop, dim3(grid_for(n3)), dim3(FT), 0, st, (const float*)lapd, n, n, 0,
+    hipLaunchKernelGGL(fs_crop, dim3(grid_rows((int64_t)n * n)), dim3(FT), 0, st, (const float*)lapd, n, n, 0,
                        (float)(1.0 / (double)n3), lapd);
     return finish_call(ctx, "hnt_compute_laplacian");
 }
@@ -458,12 +507,12 @@ extern "C" hnt_status hnt_isolated_poisson_psi(hnt_ctx* ctx, const float* dS, in
     HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
     const double dk = 2.0 * M_PI / (2.0 * boxsize_local);
     for (int d = 0; d < 3; ++d) {
-        hipLaunchKernelGGL(fs_periodic, dim3(grid_for(half_spectrum(n2))), dim3(FT), 0, st,
+        hipLaunchKernelGGL(fs_periodic, dim3(grid_rows((int64_t)n2 * n2)), dim3(FT), 0, st,
                            (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2, dk, 2,
                            d + 1, 0);
         HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fs_work.p, (hipfftReal*)ctx->fs_real.p));
         // the 1LPT displacement is the negative of kernel 2's output
-        hipLaunchKernelGGL(fs_crop, dim3(grid_for(cube(n))), dim3(FT), 0, st,
+        hipLaunchKernelGGL(fs_crop, dim3(grid_rows((int64_t)n * n)), dim3(FT), 0, st,
                            (const float*)ctx->fs_real.p, n2, n, 0,
                            (float)(-1.0 / (double)cube(n2)), psi[d]);
     }
@@ -481,10 +530,10 @@ extern "C" hnt_status hnt_isolated_short_pot(hnt_ctx* ctx, const float* delta, i
     hipfftHandle r2c, c2r;
     HNT_TRY(plans_for(ctx, n2, r2c, c2r));
     HNT_TRY(padded_rfft(ctx, delta, n, n2, r2c));
-    hipLaunchKernelGGL(fs_short_pot, dim3(grid_for(half_spectrum(n2))), dim3(FT), 0, st,
+    hipLaunchKernelGGL(fs_short_pot, dim3(grid_rows((int64_t)n2 * n2)), dim3(FT), 0, st,
                        (float2*)ctx->fs_base.p, n2, 2.0 * M_PI / (2.0 * boxsize_local), rs * rs);
     HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fs_base.p, (hipfftReal*)ctx->fs_real.p));
-    hipLaunchKernelGGL(fs_crop, dim3(grid_for(cube(n))), dim3(FT), 0, st,
+    hipLaunchKernelGGL(fs_crop, dim3(grid_rows((int64_t)n * n)), dim3(FT), 0, st,
                        (const float*)ctx->fs_real.p, n2, n, 0, (float)(1.0 / (double)cube(n2)),
                        out);
     return finish_call(ctx, "hnt_isolated_short_pot");
@@ -507,7 +556,7 @@ extern "C" hnt_status hnt_interpolate_to_tile(hnt_ctx* ctx, const float* coarse,
     const int64_t k0 = (int64_t)(kt - 1) * nsub + 1 - nbuff;
     HNT_REQUIRE(std::max(i0, std::max(j0, k0)) + nmesh < (1LL << 31),
                 "hnt_interpolate_to_tile: tile start beyond Int32");
-    hipLaunchKernelGGL(fs_tricubic, dim3(grid_for(cube(nmesh))), dim3(FT), 0, ctx->stream, coarse,
+    hipLaunchKernelGGL(fs_tricubic, dim3(grid_rows((int64_t)nmesh * nmesh)), dim3(FT), 0, ctx->stream, coarse,
                        M, N, block, (int)i0, (int)j0, (int)k0, nmesh, tile);
     return finish_call(ctx, "hnt_interpolate_to_tile");
 }
