@@ -321,6 +321,8 @@ static void release_scratch(hnt_ctx* ctx) {
         release(*b);
     release_fft(ctx);
     release_field_plans(ctx);
+    for (hnt_ctx::FftTw& t : ctx->fft_tw) release(t.buf);
+    ctx->fft_tw.clear();
 }
 
 hnt_status hnt_ctx_release_scratch(hnt_ctx* ctx) {
@@ -353,6 +355,11 @@ hnt_status hnt_ctx_destroy(hnt_ctx* ctx) {
 hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value) {
     clear_error();
     HNT_REQUIRE(ctx, "hnt_ctx_set_tuning: NULL context");
+    if (what == HNT_TUNE_FFT) {
+        HNT_REQUIRE(value == 0 || value == 1, "hnt_ctx_set_tuning: fft %d (0 or 1)", value);
+        ctx->fft_mode = value;
+        return HNT_OK;
+    }
     if (what == HNT_TUNE_BRICK_LOG2) {
         HNT_REQUIRE(value >= 0 && value <= 13, "hnt_ctx_set_tuning: brick_log2 %d", value);
         ctx->brick_log2 = value;

@@ -1,0 +1,514 @@
+// fft.hip -- power-of-two 3-D real FFTs for the field-synthesis entries.
+//
+// A forward real-to-complex transform of a w^3 field zero-padded to L^3, and an inverse
+// complex-to-real transform whose output is cropped to n^3 cells starting at s, with an
+// element-wise operator (transfer, periodic kernel, short-range kernel) applied as the
+// spectrum is first read.  Layouts are hipFFT's for an L^3 real field with x fastest:
+// half spectrum (L/2 + 1) x L x L complex.  Both transforms are unnormalised.
+//
+// Each axis is one pass over HBM.  Along y and z a workgroup loads C contiguous x-columns of
+// length L (C x 8 B segments, c fastest across lanes) into LDS, runs a Stockham radix-8/4 FFT
+// on every column, and writes back only the requested range; along x a workgroup takes G rows,
+// packs the even/odd reals of a row into an L/2-point complex FFT and untwists it.  The zero
+// padding is pruned (the forward x pass reads only the w^2 non-empty rows, y only the w
+// non-empty planes) and so is the crop (the inverse z pass writes n of L planes, y works on
+// those n planes only and writes n rows each, x transforms the n^2 kept rows).  The inverse x
+// pass uses only the real parts of the kx = 0 and kx = L/2 coefficients, which is the same as
+// making those two planes Hermitian (averaging conjugate pairs) before the transform.
+#include <cmath>
+
+#include "hnt_internal.h"
+
+namespace hnt {
+
+namespace {
+
+// Workgroup shapes: 256 threads and 35 KB of LDS (four workgroups per CU, some loading or
+// storing while others transform) for columns up to 1024 and for all rows; 512 threads and
+// 68 KB (four 2048-point columns, two workgroups per CU) for 2048-point columns, where the
+// small shape would read 16-B pieces of each line.  (Eight columns with 1024 threads, one
+// workgroup per CU, measured the same.)
+constexpr int SMALL_T = 256, SMALL_LDS = 4360;  // 4 padded columns of 1024, 2 of 2048
+constexpr int BIG_T = 512, BIG_LDS = 8712;      // 4 padded columns of 2048
+
+int ilog2(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+// LDS slot of element i of a column: one spare slot after every 16, so that the strided
+// writes of the first Stockham stages (stride R complex) spread over the banks
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int padded_len(int L) { return L + (L >> 4) + 1; }
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * w, or a * conj(w) for the inverse direction
+template <bool INV>
+__device__ __forceinline__ float2 ctw(float2 a, float2 w) {
+    return INV ? make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y)
+               : make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+// -i a (forward) or +i a (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+    const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+    const float2 a3 = rot<INV>(csub(v1, v3));
+    v0 = cadd(a0, a2);
+    v2 = csub(a0, a2);
+    v1 = cadd(a1, a3);
+    v3 = csub(a1, a3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[R]) {
+    if constexpr (R == 2) {
+        const float2 t = v[0];
+        v[0] = cadd(t, v[1]);
+        v[1] = csub(t, v[1]);
+    } else if constexpr (R == 4) {
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+    } else {
+        float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4<INV>(e0, e1, e2, e3);
+        dft4<INV>(o0, o1, o2, o3);
+        const float h = 0.70710678118654752f;
+        // o1 *= w8, o2 *= w8^2, o3 *= w8^3 with w8 = exp(-+ i pi / 4)
+        o1 = INV ? make_float2((o1.x - o1.y) * h, (o1.x + o1.y) * h)
+                 : make_float2((o1.x + o1.y) * h, (o1.y - o1.x) * h);
+        o2 = rot<INV>(o2);
+        o3 = INV ? make_float2(-(o3.x + o3.y) * h, (o3.x - o3.y) * h)
+                 : make_float2((o3.y - o3.x) * h, -(o3.x + o3.y) * h);
+        v[0] = cadd(e0, o0);
+        v[4] = csub(e0, o0);
+        v[1] = cadd(e1, o1);
+        v[5] = csub(e1, o1);
+        v[2] = cadd(e2, o2);
+        v[6] = csub(e2, o2);
+        v[3] = cadd(e3, o3);
+        v[7] = csub(e3, o3);
+    }
+}
+
+// One Stockham stage of radix R on `ncol` columns of length L held in LDS (column c at
+// buf + c * LS): butterfly j of a column reads x[j + r L/R], twiddles by W^(r k L/(Ns R)),
+// k = j mod Ns, and writes x[(j/Ns) Ns R + k + r Ns].  All reads precede all writes.
+template <int R, bool INV, int NT>
+__device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, int lgNs,
+                                      const float2* __restrict__ W) {
+    constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+    constexpr int MAXB = 16 / R;  // butterflies per thread: <= 4096 NT/256 points / (R NT)
+    const int lgnb = lgL - LGR;
+    const int nb = 1 << lgnb;
+    const int total = nb * ncol;
+    const int Ns = 1 << lgNs;
+    const int tw_shift = lgL - lgNs - LGR;
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int g = (int)threadIdx.x + q * NT;
+        if (g < total) {
+            const int c = g >> lgnb, j = g & (nb - 1);
+            const float2* col = buf + c * LS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = col[pad(j + r * nb)];
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[q][r] = ctw<INV>(v[q][r], W[(k * r) << tw_shift]);
+            dft<R, INV>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int g = (int)threadIdx.x + q * NT;
+        if (g < total) {
+            const int c = g >> lgnb, j = g & (nb - 1);
+            const int k = j & (Ns - 1);
+            float2* col = buf + c * LS;
+            const int d0 = ((j >> lgNs) << (lgNs + LGR)) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) col[pad(d0 + r * Ns)] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// FFT of length 2^lgL of every column, in place, natural order in and out.
+template <bool INV, int NT>
+__device__ __forceinline__ void stockham(float2* buf, int LS, int ncol, int lgL,
+                                         const float2* __restrict__ W) {
+    int lgNs = 0;
+    while (lgNs < lgL) {
+        const int rem = lgL - lgNs;
+        if (rem == 1) {
+            stage<2, INV, NT>(buf, LS, ncol, lgL, lgNs, W);
+            lgNs += 1;
+        } else if (rem == 2 || rem == 4) {
+            stage<4, INV, NT>(buf, LS, ncol, lgL, lgNs, W);
+            lgNs += 2;
+        } else {
+            stage<8, INV, NT>(buf, LS, ncol, lgL, lgNs, W);
+            lgNs += 3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- spectrum operators
+
+__device__ __forceinline__ double pick(int d, double kx, double ky, double kz) {
+    return d == 1 ? kx : (d == 2 ? ky : kz);
+}
+
+// Operator on element (ix, iy, iz) of the half spectrum of an L^3 field, as the reference
+// kernels compute it (_transfer_kernel!, _periodic_kernel!, _short_pot_kernel).
+__device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, int iy, int iz) {
+    const int L = op.L, half = L / 2;
+    if (op.kind == FFT_OP_TRANSFER) {
+        const double kx = (double)ix * op.dk;
+        const double ky = (double)(iy < half ? iy : iy - L) * op.dk;
+        const double kz = (double)(iz < half ? iz : iz - L) * op.dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 == 0.0) return make_float2(0.0f, 0.0f);
+        const int ay = iy <= half ? iy : L - iy, az = iz <= half ? iz : L - iz;
+        const double sqrt_pk =
+            (double)op.amp[ix + (int64_t)(half + 1) * (ay + (int64_t)(half + 1) * az)];
+        if (op.id == 0)
+            return make_float2((float)((double)v.x * sqrt_pk), (float)((double)v.y * sqrt_pk));
+        if (op.id == 1 || op.id == 2) {
+            const double coeff = sqrt_pk * pick(op.d1, kx, ky, kz) / k2;
+            const double a = v.x, b = v.y;
+            return op.id == 1 ? make_float2((float)(-b * coeff), (float)(a * coeff))
+                              : make_float2((float)(b * coeff), (float)(-a * coeff));
+        }
+        float c;
+        if (op.id == 3)
+            c = (float)(-sqrt_pk * pick(op.d1, kx, ky, kz) * pick(op.d2, kx, ky, kz) / k2);
+        else if (op.id == 4)
+            c = (float)(sqrt_pk * k2);
+        else
+            c = (float)(-sqrt_pk / k2);
+        return make_float2(v.x * c, v.y * c);
+    }
+    if (op.kind == FFT_OP_PERIODIC) {
+        const double kx = (double)ix * op.dk;
+        const double ky = (double)(iy <= half ? iy : iy - L) * op.dk;
+        const double kz = (double)(iz <= half ? iz : iz - L) * op.dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 == 0.0 || (op.id != 3 && (ix == half || iy == half || iz == half)))
+            return make_float2(0.0f, 0.0f);
+        if (op.id == 2) {
+            const double coeff = pick(op.d1, kx, ky, kz) / k2;
+            const double a = v.x, b = v.y;
+            return make_float2((float)(b * coeff), (float)(-a * coeff));
+        }
+        const float c = op.id == 3
+                            ? (float)(-pick(op.d1, kx, ky, kz) * pick(op.d2, kx, ky, kz) / k2)
+                            : (float)k2;
+        return make_float2(v.x * c, v.y * c);
+    }
+    if (op.kind == FFT_OP_SHORTPOT) {
+        const double kx = (double)ix * op.dk;
+        const double ky = (double)(iy <= half ? iy : iy - L) * op.dk;
+        const double kz = (double)(iz <= half ? iz : iz - L) * op.dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const float c = (float)(k2 == 0.0 ? -op.rs2 : -(1.0 - exp(-k2 * op.rs2)) / k2);
+        return make_float2(v.x * c, v.y * c);
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Transform along a strided axis: element (x, a, b) at x + a strideA + b strideB, x over the
+// nx contiguous columns (C per workgroup), a the transformed axis (length 2^lgL), b =
+// b0 + blockIdx.y.  Reads a in [r0, r1) (zero elsewhere), applies `op` when set (zpass:
+// a = iz, b = iy), writes a in [w0, w1).  In place when in == out: a workgroup reads all of
+// its columns before it writes.
+// Workgroups are dealt to the 8 XCDs in turn; tile t = xcd * per + (id / 8) keeps the tiles
+// that share 128-B lines (adjacent x) on one XCD and its L2, one after another.
+template <bool INV, int NT, int NLDS>
+__global__ __launch_bounds__(NT) void fft_cols(
+    const float2* in, float2* out, int lgL, int64_t strideA, int64_t strideB, int nx, int lgC,
+    int gx, int ntiles, int b0, int r0, int r1, int w0, int w1, FftOp op,
+    const float2* __restrict__ W) {
+    __shared__ float2 buf[NLDS];
+    const int per = (ntiles + 7) >> 3;
+    const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int L = 1 << lgL, C = 1 << lgC, LS = padded_len(L);
+    const int x0 = (tile % gx) * C;
+    const int b = b0 + tile / gx;
+    const int64_t boff = (int64_t)b * strideB;
+    for (int e = threadIdx.x; e < C * L; e += NT) {
+        const int c = e & (C - 1), a = e >> lgC, x = x0 + c;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (x < nx && a >= r0 && a < r1) {
+            v = in[x + a * strideA + boff];
+            if (op.kind != FFT_OP_NONE) v = apply_op(op, v, x, b, a);
+        }
+        buf[c * LS + pad(a)] = v;
+    }
+    __syncthreads();
+    stockham<INV, NT>(buf, LS, C, lgL, W);
+    for (int e = threadIdx.x; e < C * L; e += NT) {
+        const int c = e & (C - 1), a = e >> lgC, x = x0 + c;
+        if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[c * LS + pad(a)];
+    }
+}
+
+// Forward x pass: rows (iy, iz) of a w^3 real field (iy, iz < w), zero-extended to L = 2M,
+// into rows of the (M + 1) x L x L half spectrum.  z[m] = x[2m] + i x[2m + 1], Z = FFT_M(z),
+// X[k] = E[k] + W_L^k O[k] with E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i.
+__global__ __launch_bounds__(SMALL_T) void fft_rows_r2c(
+    const float* __restrict__ f, int w, int lgM, int lgG, float2* __restrict__ spec,
+    const float2* __restrict__ WM, const float2* __restrict__ WL) {
+    __shared__ float2 buf[SMALL_LDS];
+    const int M = 1 << lgM, G = 1 << lgG, LS = padded_len(M), L = 2 * M, nk = M + 1;
+    const int row0 = (int)blockIdx.x * G;
+    const int nrows = w * w;
+    for (int e = threadIdx.x; e < G * M; e += SMALL_T) {
+        const int g = e >> lgM, m = e & (M - 1), row = row0 + g;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (row < nrows) {
+            const float* src = f + (int64_t)row * w;
+            const int x = 2 * m;
+            v = make_float2(x < w ? src[x] : 0.0f, x + 1 < w ? src[x + 1] : 0.0f);
+        }
+        buf[g * LS + pad(m)] = v;
+    }
+    __syncthreads();
+    stockham<false, SMALL_T>(buf, LS, G, lgM, WM);
+    for (int e = threadIdx.x; e < G * nk; e += SMALL_T) {
+        const int g = e / nk, k = e - g * nk, row = row0 + g;
+        if (row >= nrows) continue;
+        const int iy = row % w, iz = row / w;
+        const float2 zk = buf[g * LS + pad(k & (M - 1))];
+        const float2 zm = buf[g * LS + pad((M - k) & (M - 1))];
+        const float2 ev = make_float2((zk.x + zm.x) * 0.5f, (zk.y - zm.y) * 0.5f);
+        // (zk - conj zm) / 2i
+        const float2 od = make_float2((zk.y + zm.y) * 0.5f, -(zk.x - zm.x) * 0.5f);
+        const float2 t = ctw<false>(od, WL[k]);
+        spec[k + (int64_t)nk * (iy + (int64_t)L * iz)] = cadd(ev, t);
+    }
+}
+
+// Inverse x pass: rows (s + jy, s + jz), jy, jz < n, of the half spectrum to real rows, of
+// which x in [s, s + n) is kept: out[(jy + n jz) n + x - s] = scale * x (mode 0), or
+// s2 -= (scale * x)^2 * coef (mode 1).  X[0] and X[M] enter through their real parts.
+// Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) W_L^-k, z = IFFT_M(Z), x[2m] = Re z[m],
+// x[2m + 1] = Im z[m].
+__global__ __launch_bounds__(SMALL_T) void fft_rows_c2r(
+    const float2* __restrict__ spec, int lgM, int lgG, int s, int n, float scale, int mode,
+    float coef, float* __restrict__ out, const float2* __restrict__ WM,
+    const float2* __restrict__ WL) {
+    __shared__ float2 buf[SMALL_LDS];
+    const int M = 1 << lgM, G = 1 << lgG, LS = padded_len(M), L = 2 * M, nk = M + 1;
+    const int row0 = (int)blockIdx.x * G;
+    const int nrows = n * n;
+    for (int e = threadIdx.x; e < G * nk; e += SMALL_T) {
+        const int g = e / nk, k = e - g * nk, row = row0 + g;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (row < nrows) {
+            const int iy = s + row % n, iz = s + row / n;
+            v = spec[k + (int64_t)nk * (iy + (int64_t)L * iz)];
+        }
+        buf[g * LS + pad(k)] = v;
+    }
+    __syncthreads();
+    // untwist into Z (registers first: Z[k] needs X[M - k] of the same row)
+    constexpr int MAXB = SMALL_LDS / SMALL_T + 1;
+    float2 z[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int e = (int)threadIdx.x + q * SMALL_T;
+        if (e < G * M) {
+            const int g = e >> lgM, k = e & (M - 1);
+            float2 a = buf[g * LS + pad(k)];
+            float2 b = buf[g * LS + pad(M - k)];
+            if (k == 0) {
+                a.y = 0.0f;
+                b.y = 0.0f;
+            }
+            const float2 sum = make_float2(a.x + b.x, a.y - b.y);   // a + conj b
+            const float2 dif = make_float2(a.x - b.x, a.y + b.y);   // a - conj b
+            const float2 t = ctw<true>(dif, WL[k]);                 // dif W_L^-k
+            z[q] = make_float2(sum.x - t.y, sum.y + t.x);           // sum + i t
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int e = (int)threadIdx.x + q * SMALL_T;
+        if (e < G * M) buf[(e >> lgM) * LS + pad(e & (M - 1))] = z[q];
+    }
+    __syncthreads();
+    stockham<true, SMALL_T>(buf, LS, G, lgM, WM);
+    for (int e = threadIdx.x; e < G * n; e += SMALL_T) {
+        const int g = e / n, xo = e - g * n, row = row0 + g;
+        if (row >= nrows) continue;
+        const int xx = s + xo;
+        const float2 zz = buf[g * LS + pad(xx >> 1)];
+        const float v = ((xx & 1) ? zz.y : zz.x) * scale;
+        const int64_t o = (int64_t)row * n + xo;
+        if (mode == 0) out[o] = v;
+        else out[o] = out[o] - (v * v) * coef;
+    }
+}
+
+// sqrt P per (ix, |iy|, |iz|) of the transfer: the (L/2 + 1)^3 table the inverse z pass reads
+// (rows of equal k^2 share it; k^2 as _transfer_kernel! computes it).
+__global__ void fft_amp(float* __restrict__ amp, int L, double dk, const float* __restrict__ tab,
+                        int ntab, float log_k_min, float d_log_k) {
+    const int half = L / 2, nh = half + 1;
+    const int lane = threadIdx.x & 63;
+    for (int row = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); row < nh * nh;
+         row += (int)gridDim.x * 4) {
+        const int ay = row % nh, az = row / nh;
+        const double ky = (double)ay * dk, kz = (double)az * dk;
+        const double kyz = ky * ky, kzz = kz * kz;
+        for (int ix = lane; ix < nh; ix += 64) {
+            const double kx = (double)ix * dk;
+            const double k2 = kx * kx + kyz + kzz;
+            float v = 0.0f;
+            if (k2 != 0.0) {
+                const float lk = (float)(log(k2) * 0.5);
+                const float fpos = (lk - log_k_min) / d_log_k;
+                const int i0 = (int)floorf(fpos) + 1;
+                if (i0 < 1) v = tab[0];
+                else if (i0 >= ntab) v = tab[ntab - 1];
+                else {
+                    const float frac = fpos - (float)(i0 - 1);
+                    v = tab[i0 - 1] + frac * (tab[i0] - tab[i0 - 1]);
+                }
+            }
+            amp[ix + (int64_t)nh * row] = v;
+        }
+    }
+}
+
+// grid of the XCD-dealt column tiles: 8 * ceil(tiles / 8) workgroups
+unsigned cols_grid(unsigned gx, int nb) {
+    const int64_t t = (int64_t)gx * nb;
+    return (unsigned)(((t + 7) / 8) * 8);
+}
+
+bool cols_big(int L) { return L >= 2048; }
+
+int cols_lgC(int L) {
+    // as many padded columns as the shape's LDS holds, at most 8 (64-B segments)
+    const int cap = cols_big(L) ? BIG_LDS : SMALL_LDS;
+    int lg = 3;
+    while (lg > 0 && (1 << lg) * padded_len(L) > cap) --lg;
+    return lg;
+}
+
+int rows_lgG(int M) {
+    int lg = 4;
+    while (lg > 0 && (1 << lg) * padded_len(M) > SMALL_LDS) --lg;
+    return lg;
+}
+
+template <bool INV>
+void launch_cols(hipStream_t st, int L, unsigned gx, int nb, const float2* in, float2* out,
+                 int64_t strideA, int64_t strideB, int nx, int b0, int r0, int r1, int w0,
+                 int w1, const FftOp& op, const float2* W) {
+    const int lgC = cols_lgC(L);
+    if (cols_big(L))
+        hipLaunchKernelGGL((fft_cols<INV, BIG_T, BIG_LDS>), dim3(cols_grid(gx, nb)), dim3(BIG_T),
+                           0, st, in, out, ilog2(L), strideA, strideB, nx, lgC, (int)gx,
+                           (int)gx * nb, b0, r0, r1, w0, w1, op, W);
+    else
+        hipLaunchKernelGGL((fft_cols<INV, SMALL_T, SMALL_LDS>), dim3(cols_grid(gx, nb)),
+                           dim3(SMALL_T), 0, st, in, out, ilog2(L), strideA, strideB, nx, lgC,
+                           (int)gx, (int)gx * nb, b0, r0, r1, w0, w1, op, W);
+}
+
+}  // namespace
+
+bool fft_supported(int L) { return L >= 8 && L <= 2048 && (L & (L - 1)) == 0; }
+
+// Forward twiddles W[k] = exp(-2 pi i k / L), k < L, computed in double, kept per L.
+hnt_status fft_twiddles(hnt_ctx* ctx, int L, const float2** out) {
+    for (hnt_ctx::FftTw& t : ctx->fft_tw)
+        if (t.L == L) {
+            *out = (const float2*)t.buf.p;
+            return HNT_OK;
+        }
+    std::vector<float2> h(L);
+    for (int k = 0; k < L; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)L;
+        h[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    ctx->fft_tw.push_back({L, DevBuf()});
+    DevBuf& b = ctx->fft_tw.back().buf;
+    HNT_TRY(reserve(ctx, b, sizeof(float2) * (size_t)L));
+    HNT_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(float2) * (size_t)L, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HNT_HIP(hipStreamSynchronize(ctx->stream));
+    *out = (const float2*)b.p;
+    return HNT_OK;
+}
+
+hnt_status fft_forward(hnt_ctx* ctx, const float* f, int w, int L, float2* spec) {
+    hipStream_t st = ctx->stream;
+    const int M = L / 2, nk = M + 1;
+    const float2 *WL, *WM;
+    HNT_TRY(fft_twiddles(ctx, L, &WL));
+    HNT_TRY(fft_twiddles(ctx, M, &WM));
+    const int lgG = rows_lgG(M);
+    const int64_t rows = (int64_t)w * w;
+    hipLaunchKernelGGL(fft_rows_r2c, dim3((unsigned)((rows + (1 << lgG) - 1) >> lgG)),
+                       dim3(SMALL_T), 0, st, f, w, ilog2(M), lgG, spec, WM, WL);
+    FftOp none;
+    const int lgC = cols_lgC(L);
+    const unsigned gx = (unsigned)((nk + (1 << lgC) - 1) >> lgC);
+    // y: planes iz < w, rows iy < w are filled
+    launch_cols<false>(st, L, gx, w, spec, spec, (int64_t)nk, (int64_t)nk * L, nk, 0, 0, w, 0,
+                       L, none, WL);
+    // z: every (ix, iy) column, planes iz < w are filled
+    launch_cols<false>(st, L, gx, L, spec, spec, (int64_t)nk * L, (int64_t)nk, nk, 0, 0, w, 0,
+                       L, none, WL);
+    return HNT_OK;
+}
+
+hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, int s, int n,
+                       const FftOp& op, float scale, int mode, float coef, float* out) {
+    hipStream_t st = ctx->stream;
+    const int M = L / 2, nk = M + 1;
+    const float2 *WL, *WM;
+    HNT_TRY(fft_twiddles(ctx, L, &WL));
+    HNT_TRY(fft_twiddles(ctx, M, &WM));
+    const int lgC = cols_lgC(L);
+    const unsigned gx = (unsigned)((nk + (1 << lgC) - 1) >> lgC);
+    FftOp none;
+    // z: every (ix, iy) column, the operator applied as read; planes iz in [s, s + n) kept
+    launch_cols<true>(st, L, gx, L, spec, work, (int64_t)nk * L, (int64_t)nk, nk, 0, 0, L, s,
+                      s + n, op, WL);
+    // y: planes iz in [s, s + n), rows iy in [s, s + n) kept
+    launch_cols<true>(st, L, gx, n, work, work, (int64_t)nk, (int64_t)nk * L, nk, s, 0, L, s,
+                      s + n, none, WL);
+    const int lgG = rows_lgG(M);
+    const int64_t rows = (int64_t)n * n;
+    hipLaunchKernelGGL(fft_rows_c2r, dim3((unsigned)((rows + (1 << lgG) - 1) >> lgG)),
+                       dim3(SMALL_T), 0, st, (const float2*)work, ilog2(M), lgG, s, n, scale,
+                       mode, coef, out, WM, WL);
+    return HNT_OK;
+}
+
+hnt_status fft_amp_table(hnt_ctx* ctx, int L, double dk, const float* tab, int ntab,
+                         float log_k_min, float d_log_k, float* amp) {
+    const int64_t nh = L / 2 + 1;
+    hipLaunchKernelGGL(fft_amp, dim3((unsigned)std::min<int64_t>((nh * nh + 3) / 4, 1 << 20)),
+                       dim3(256), 0, ctx->stream, amp, L, dk, tab, ntab, log_k_min, d_log_k);
+    return HNT_OK;
+}
+
+}  // namespace hnt

@@ -327,8 +327,16 @@ __global__ void fs_tricubic(const float* __restrict__ coarse, int M, int N, int 
     }
 }
 
-// hipFFT R2C / C2R plans of an n^3 real field (cached per size on the context)
+// hipFFT R2C / C2R plans of an n^3 real field (cached per size on the context).  The 32-bit
+// plan call gives wrong transforms from 2^31 elements on (2048^3: an impulse response loses
+// its axis symmetry at 1e-3, a smooth field is off by half its norm), so larger sizes are
+// refused; the library's own FFTs cover the power-of-two sizes up to 2048.
 hnt_status plans_for(hnt_ctx* ctx, int n, hipfftHandle& r2c, hipfftHandle& c2r) {
+    if ((int64_t)n * n * n >= (1LL << 31)) {
+        set_error("hipFFT path: %d^3 real transforms are not supported (2^31 elements or more); "
+                  "power-of-two sizes up to 2048 use the library's FFTs (tuning fft = 0)", n);
+        return HNT_ERR_UNSUPPORTED;
+    }
     for (hnt_ctx::FftPlans& p : ctx->fs_plans)
         if (p.n == n) {
             r2c = (hipfftHandle)p.r2c;
@@ -397,11 +405,36 @@ extern "C" hnt_status hnt_isolated_convolve(hnt_ctx* ctx, const float* noise, in
     const int n2 = 2 * n_input;
     const double dx = boxsize_local / n;
     const double dk = 2.0 * M_PI / (n2 * dx);
-    hipfftHandle r2c, c2r;
-    HNT_TRY(plans_for(ctx, n2, r2c, c2r));
     HNT_TRY(reserve(ctx, ctx->fs_tab, sizeof(float) * (size_t)table->n));
     HNT_HIP(hipMemcpyAsync(ctx->fs_tab.p, table->sqrt_pk, sizeof(float) * (size_t)table->n,
                            hipMemcpyHostToDevice, st));
+    if (ctx->fft_mode == 0 && fft_supported(n2)) {
+        // library FFTs: pruned forward transform, one sqrt P table, pruned inverse per output
+        const int64_t nhalf = n2 / 2 + 1;
+        HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)half_spectrum(n2)));
+        HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
+        HNT_TRY(reserve(ctx, ctx->fs_aux, sizeof(float) * (size_t)(nhalf * nhalf * nhalf)));
+        float2* base = (float2*)ctx->fs_base.p;
+        HNT_TRY(fft_forward(ctx, noise, n_input, n2, base));
+        HNT_TRY(fft_amp_table(ctx, n2, dk, (const float*)ctx->fs_tab.p, table->n,
+                              table->log_k_min, table->d_log_k, (float*)ctx->fs_aux.p));
+        const float scale = (float)(1.0 / (double)cube(n2));
+        for (int q = 0; q < nkernels; ++q) {
+            FftOp op;
+            op.kind = FFT_OP_TRANSFER;
+            op.L = n2;
+            op.dk = dk;
+            op.amp = (const float*)ctx->fs_aux.p;
+            op.id = kernels[q].id;
+            op.d1 = kernels[q].dim1;
+            op.d2 = kernels[q].dim2;
+            HNT_TRY(fft_inverse(ctx, base, (float2*)ctx->fs_work.p, n2, nshell, n, op, scale, 0,
+                                0.0f, out[q]));
+        }
+        return finish_call(ctx, "hnt_isolated_convolve");
+    }
+    hipfftHandle r2c, c2r;
+    HNT_TRY(plans_for(ctx, n2, r2c, c2r));
     HNT_TRY(padded_rfft(ctx, noise, n_input, n2, r2c));
     HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
     const int64_t nh = half_spectrum(n2);
@@ -431,6 +464,36 @@ extern "C" hnt_status hnt_compute_2lpt(hnt_ctx* ctx, const float* delta, int32_t
     hipStream_t st = ctx->stream;
     const int64_t n3 = cube(n), nh = half_spectrum(n);
     const double dk = 2.0 * M_PI / boxsize_local;
+    const float inv3 = (float)(1.0 / (double)n3);
+    if (ctx->fft_mode == 0 && fft_supported(n)) {
+        HNT_TRY(reserve(ctx, ctx->fs_aux, sizeof(float) * (size_t)n3));
+        HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)nh));
+        HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)nh));
+        float* s2 = (float*)ctx->fs_aux.p;
+        float2* base = (float2*)ctx->fs_base.p;
+        float2* work = (float2*)ctx->fs_work.p;
+        HNT_TRY(fft_forward(ctx, delta, n, n, base));
+        hipLaunchKernelGGL(fs_src2_init, dim3(grid_for(n3)), dim3(FT), 0, st, delta, n3, s2);
+        const int prs[6][2] = {{1, 1}, {2, 2}, {3, 3}, {1, 2}, {1, 3}, {2, 3}};
+        FftOp op;
+        op.kind = FFT_OP_PERIODIC;
+        op.L = n;
+        op.dk = dk;
+        for (int q = 0; q < 6; ++q) {
+            op.id = 3;
+            op.d1 = prs[q][0];
+            op.d2 = prs[q][1];
+            HNT_TRY(fft_inverse(ctx, base, work, n, 0, n, op, inv3, 1, q < 3 ? 0.5f : 1.0f, s2));
+        }
+        HNT_TRY(fft_forward(ctx, s2, n, n, base));
+        for (int d = 0; d < 3; ++d) {
+            op.id = 2;
+            op.d1 = d + 1;
+            op.d2 = 0;
+            HNT_TRY(fft_inverse(ctx, base, work, n, 0, n, op, inv3, 0, 0.0f, psi2[d]));
+        }
+        return finish_call(ctx, "hnt_compute_2lpt");
+    }
     hipfftHandle r2c, c2r;
     HNT_TRY(plans_for(ctx, n, r2c, c2r));
     HNT_TRY(reserve(ctx, ctx->fs_real, sizeof(float) * (size_t)n3));
@@ -475,6 +538,19 @@ extern "C" hnt_status hnt_compute_laplacian(hnt_ctx* ctx, const float* delta, in
                 "hnt_compute_laplacian: n %d, boxsize %g", n, boxsize_local);
     hipStream_t st = ctx->stream;
     const int64_t n3 = cube(n), nh = half_spectrum(n);
+    if (ctx->fft_mode == 0 && fft_supported(n)) {
+        HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)nh));
+        HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)nh));
+        HNT_TRY(fft_forward(ctx, delta, n, n, (float2*)ctx->fs_base.p));
+        FftOp op;
+        op.kind = FFT_OP_PERIODIC;
+        op.L = n;
+        op.dk = 2.0 * M_PI / boxsize_local;
+        op.id = 4;
+        HNT_TRY(fft_inverse(ctx, (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n, 0, n,
+                            op, (float)(1.0 / (double)n3), 0, 0.0f, lapd));
+        return finish_call(ctx, "hnt_compute_laplacian");
+    }
     hipfftHandle r2c, c2r;
     HNT_TRY(plans_for(ctx, n, r2c, c2r));
     HNT_TRY(reserve(ctx, ctx->fs_real, sizeof(float) * (size_t)n3));
@@ -501,11 +577,28 @@ extern "C" hnt_status hnt_isolated_poisson_psi(hnt_ctx* ctx, const float* dS, in
                 "hnt_isolated_poisson_psi: n %d, boxsize %g", n, boxsize_local);
     hipStream_t st = ctx->stream;
     const int n2 = 2 * n;
+    const double dk = 2.0 * M_PI / (2.0 * boxsize_local);
+    if (ctx->fft_mode == 0 && fft_supported(n2)) {
+        HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)half_spectrum(n2)));
+        HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
+        HNT_TRY(fft_forward(ctx, dS, n, n2, (float2*)ctx->fs_base.p));
+        FftOp op;
+        op.kind = FFT_OP_PERIODIC;
+        op.L = n2;
+        op.dk = dk;
+        op.id = 2;
+        for (int d = 0; d < 3; ++d) {
+            op.d1 = d + 1;
+            // the 1LPT displacement is the negative of kernel 2's output
+            HNT_TRY(fft_inverse(ctx, (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2,
+                                0, n, op, (float)(-1.0 / (double)cube(n2)), 0, 0.0f, psi[d]));
+        }
+        return finish_call(ctx, "hnt_isolated_poisson_psi");
+    }
     hipfftHandle r2c, c2r;
     HNT_TRY(plans_for(ctx, n2, r2c, c2r));
     HNT_TRY(padded_rfft(ctx, dS, n, n2, r2c));
     HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
-    const double dk = 2.0 * M_PI / (2.0 * boxsize_local);
     for (int d = 0; d < 3; ++d) {
         hipLaunchKernelGGL(fs_periodic, dim3(grid_rows((int64_t)n2 * n2)), dim3(FT), 0, st,
                            (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2, dk, 2,
@@ -527,6 +620,19 @@ extern "C" hnt_status hnt_isolated_short_pot(hnt_ctx* ctx, const float* delta, i
                 "hnt_isolated_short_pot: n %d, boxsize %g, rs %g", n, boxsize_local, rs);
     hipStream_t st = ctx->stream;
     const int n2 = 2 * n;
+    if (ctx->fft_mode == 0 && fft_supported(n2)) {
+        HNT_TRY(reserve(ctx, ctx->fs_base, sizeof(float2) * (size_t)half_spectrum(n2)));
+        HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
+        HNT_TRY(fft_forward(ctx, delta, n, n2, (float2*)ctx->fs_base.p));
+        FftOp op;
+        op.kind = FFT_OP_SHORTPOT;
+        op.L = n2;
+        op.dk = 2.0 * M_PI / (2.0 * boxsize_local);
+        op.rs2 = rs * rs;
+        HNT_TRY(fft_inverse(ctx, (const float2*)ctx->fs_base.p, (float2*)ctx->fs_work.p, n2, 0,
+                            n, op, (float)(1.0 / (double)cube(n2)), 0, 0.0f, out));
+        return finish_call(ctx, "hnt_isolated_short_pot");
+    }
     hipfftHandle r2c, c2r;
     HNT_TRY(plans_for(ctx, n2, r2c, c2r));
     HNT_TRY(padded_rfft(ctx, delta, n, n2, r2c));

@@ -109,6 +109,11 @@ struct hnt_ctx {
     struct FftPlans { int n; void* r2c; void* c2r; };
     std::vector<FftPlans> fs_plans;
     hnt::DevBuf fs_real, fs_base, fs_work, fs_aux, fs_tab;
+    // the library's own power-of-two FFTs: twiddle tables by length; fft_mode 0 uses them
+    // where the size allows, 1 always uses hipFFT
+    struct FftTw { int L; hnt::DevBuf buf; };
+    std::vector<FftTw> fft_tw;
+    int fft_mode = 0;
     void* fft_r2c = nullptr;  // hipfftHandle of an n^3 real-to-complex plan
     void* fft_c2r = nullptr;
     int fft_n = 0;
@@ -133,6 +138,29 @@ void release(DevBuf& b);
 void release_work(SAWork& w);
 void release_fft(hnt_ctx* ctx);  // FFT plans of hnt_peak_find_tile
 void release_field_plans(hnt_ctx* ctx);  // FFT plans of the field-synthesis entries
+
+// Power-of-two 3-D real FFTs (fft.hip).  Operators applied to the spectrum as the inverse
+// transform first reads it (the reference's _transfer_kernel!, _periodic_kernel!,
+// _short_pot_kernel).
+enum { FFT_OP_NONE = 0, FFT_OP_TRANSFER = 1, FFT_OP_PERIODIC = 2, FFT_OP_SHORTPOT = 3 };
+struct FftOp {
+    int kind = FFT_OP_NONE;
+    int L = 0;                 // transform length
+    double dk = 0.0;
+    const float* amp = nullptr;  // sqrt P per (ix, |iy|, |iz|), (L/2 + 1)^3, for TRANSFER
+    int id = 0, d1 = 0, d2 = 0;
+    double rs2 = 0.0;
+};
+bool fft_supported(int L);
+hnt_status fft_twiddles(hnt_ctx* ctx, int L, const float2** out);
+// spec ((L/2+1) x L x L) = R2C of the w^3 field f zero-padded to L^3
+hnt_status fft_forward(hnt_ctx* ctx, const float* f, int w, int L, float2* spec);
+// out (n^3) = scale * C2R(op(spec)) cropped to [s, s + n)^3 (mode 0), or out -= (that)^2 * coef
+// (mode 1); work: a spectrum-sized scratch
+hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, int s, int n,
+                       const FftOp& op, float scale, int mode, float coef, float* out);
+hnt_status fft_amp_table(hnt_ctx* ctx, int L, double dk, const float* tab, int ntab,
+                         float log_k_min, float d_log_k, float* amp);
 hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t);
 hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);

@@ -165,7 +165,7 @@ HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
 ORDERS = {"crossing": HNT_ORDER_CROSSING, "plain": HNT_ORDER_PLAIN,
           "learned": HNT_ORDER_LEARNED}
 PRECISIONS = {"fp64": 0, "fp32_kahan": 1}
-TUNING = {"brick_log2": 0, "waves_per_cu": 1, "gang": 2}
+TUNING = {"brick_log2": 0, "waves_per_cu": 1, "gang": 2, "fft": 3}
 
 
 class Mlp(C.Structure):

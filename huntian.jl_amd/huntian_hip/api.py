@@ -59,7 +59,8 @@ class Context:
 
     def set_tuning(self, **knobs):
         """Launch-shape knobs (brick_log2, waves_per_cu, gang = lanes per peak: 1, 16 or
-        64); speed only, never results."""
+        64): speed only, never results; fft = 0 (the library's FFTs) or 1 (hipFFT) for the
+        field-synthesis entries, which differ by the transforms' Float32 rounding."""
         for k, v in knobs.items():
             abi.check(self.lib, self.lib.hnt_ctx_set_tuning(self.handle, abi.TUNING[k], int(v)))
 

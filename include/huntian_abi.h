@@ -94,8 +94,11 @@ hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t
  *                16: sixteen lanes share a peak and their partial sums are added in lane
  *                order (the same FP64 sums in another order, ~1e-13 relative);
  *                64: a wavefront loads a peak's cells together into LDS and 22 of its
- *                lanes each run one sum in the reference's order (bit for bit, like 1) */
-enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1, HNT_TUNE_GANG = 2 };
+ *                lanes each run one sum in the reference's order (bit for bit, like 1)
+ *  FFT:          transforms of the field-synthesis entries.  0 (default): the library's own
+ *                FFTs for power-of-two sizes 8..2048 (hipFFT otherwise); 1: hipFFT always.
+ *                Both are Float32 transforms; results differ by their rounding. */
+enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1, HNT_TUNE_GANG = 2, HNT_TUNE_FFT = 3 };
 hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value);
 
 /* ---------------------------------------------------------------- shells and tables */

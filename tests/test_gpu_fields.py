@@ -124,3 +124,82 @@ def test_bad_arguments_are_rejected():
         api.isolated_convolve_gpu(noise, pk_test, 10.0, 8, kernel_fn_id=3, dim1=1)
     with pytest.raises(HuntianError):  # N not a multiple of M
         api.interpolate_to_tile_gpu(np.zeros((7, 7, 7), np.float32), 1, 1, 1, 8, 10, 50, 7)
+
+
+@pytest.fixture
+def hipfft_mode():
+    """Switch the field entries to hipFFT for one test, back to the library FFTs after."""
+    ctx = api.context(0)
+    yield lambda on: ctx.set_tuning(fft=1 if on else 0)
+    ctx.set_tuning(fft=0)
+
+
+@pytest.mark.parametrize("L", [512, 1024])
+def test_library_fft_matches_hipfft(hipfft_mode, L):
+    """The library's pruned transforms against hipFFT's full ones on the device (1024: 16
+    columns per workgroup, 512-point rows): the isolated short-range potential of an
+    (L/2)^3 field and the periodic Laplacian of an L^3 field.  Both are Float32
+    transforms: relative L2 <= 2e-6."""
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(L)
+    n = L // 2
+    f = api.fortran_empty((n, n, n), torch.float32, "cuda")
+    f.copy_(torch.randn((n, n, n), generator=g, device="cuda"))
+    ours = api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)
+    hipfft_mode(True)
+    ref = api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)
+    hipfft_mode(False)
+    err = float(torch.linalg.vector_norm(ours - ref) / torch.linalg.vector_norm(ref))
+    print("short pot rel L2", L, err, flush=True)
+    assert err <= 2e-6
+    del f, ours, ref
+    d = api.fortran_empty((L, L, L), torch.float32, "cuda")
+    d.copy_(torch.randn((L, L, L), generator=g, device="cuda"))
+    ours = api.compute_laplacian_gpu(d, L, float(L), return_device=True)
+    hipfft_mode(True)
+    ref = api.compute_laplacian_gpu(d, L, float(L), return_device=True)
+    hipfft_mode(False)
+    err = float(torch.linalg.vector_norm(ours - ref) / torch.linalg.vector_norm(ref))
+    print("laplacian rel L2", L, err, flush=True)
+    assert err <= 2e-6
+    api.context(0).release_scratch()
+
+
+def test_library_fft_at_2048(hipfft_mode):
+    """L = 2048 (8 columns per workgroup, 1024-point rows), where hipFFT's 32-bit plan is
+    refused: the isolated short-range potential of a unit impulse is the kernel's Green
+    function, symmetric under any permutation of the axes and, being short-ranged (rs = 3
+    cells), the same near the origin as at L = 1024 (checked against hipFFT above) up to the
+    k-grid's discretisation (1e-5); a smooth Gaussian blob ten times wider than rs comes back
+    as about -rs^2 times itself."""
+    import torch
+
+    def impulse(L):
+        n = L // 2
+        f = api.fortran_empty((n, n, n), torch.float32, "cuda")
+        f.zero_()
+        f[0, 0, 0] = 1.0
+        return api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)[:8, :8, :8].cpu()
+
+    mid = impulse(1024)
+    big = impulse(2048)
+    for perm in ((1, 0, 2), (2, 1, 0), (0, 2, 1), (1, 2, 0)):
+        assert float((big - big.permute(*perm)).abs().max()) <= 1e-6 * float(big.abs().max())
+    np.testing.assert_allclose(big.numpy(), mid.numpy(), rtol=1e-5, atol=1e-7)
+    n = 1024
+    x = torch.arange(n, device="cuda", dtype=torch.float32) - n / 2
+    gx = torch.exp(-(x / 40.0) ** 2)
+    f = api.fortran_empty((n, n, n), torch.float32, "cuda")
+    f.copy_(gx[:, None, None] * gx[None, :, None] * gx[None, None, :])
+    out = api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)
+    ratio = float(out[512, 512, 512] / f[512, 512, 512])
+    assert -9.0 < ratio < -8.9, ratio
+    hipfft_mode(True)
+    from huntian_hip.abi import HuntianError
+    with pytest.raises(HuntianError):
+        api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)
+    hipfft_mode(False)
+    del f, out
+    api.context(0).release_scratch()
