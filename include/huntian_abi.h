@@ -429,8 +429,8 @@ typedef struct {
  * transfer (k = 0 set to 0), the kx = 0 and kx = Nyquist planes made Hermitian (conjugate
  * pairs averaged, self-conjugate modes made real), inverse transformed, and the cells
  * nshell+1 .. nshell+n of each axis written to out[q] (device n^3).  dk = 2 pi / (n2 dx),
- * dx = boxsize_local / n.  Scratch kept by the context: n2^3 floats and two half spectra
- * ((n2/2+1) n2^2 complex), ~100 GB at n_input = 1024. */
+ * dx = boxsize_local / n.  Scratch kept by the context: two half spectra ((n2/2+1) n2^2
+ * complex) and a (n2/2+1)^3 float sqrt P table, ~73 GB at n_input = 1024. */
 hnt_status hnt_isolated_convolve(hnt_ctx* ctx, const float* noise, int32_t n_input, int32_t n,
                                  double boxsize_local, const hnt_pk_table* table,
                                  const hnt_transfer* kernels, int32_t nkernels, int32_t nshell,
