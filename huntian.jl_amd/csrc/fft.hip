@@ -23,13 +23,13 @@ namespace hnt {
 
 namespace {
 
-// Workgroup shapes: 256 threads and 35 KB of LDS (four workgroups per CU, some loading or
-// storing while others transform) for columns up to 1024 and for all rows; 512 threads and
-// 68 KB (four 2048-point columns, two workgroups per CU) for 2048-point columns, where the
-// small shape would read 16-B pieces of each line.  (Eight columns with 1024 threads, one
-// workgroup per CU, measured the same.)
-constexpr int SMALL_T = 256, SMALL_LDS = 4360;  // 4 padded columns of 1024, 2 of 2048
-constexpr int BIG_T = 512, BIG_LDS = 8712;      // 4 padded columns of 2048
+// Workgroup shapes hold at most 8 points per thread (<= 64 VGPRs: 8 waves per SIMD, some
+// workgroups loading or storing while others transform): 512 threads and 35 KB of LDS (four
+// workgroups per CU) for columns up to 1024 and for all rows; 1024 threads and 68 KB (four
+// 2048-point columns, two workgroups per CU) for 2048-point columns, where the small shape
+// would read 16-B pieces of each line.
+constexpr int SMALL_T = 512, SMALL_LDS = 4360, SMALL_DATA = 4096;  // 4 padded columns of 1024
+constexpr int BIG_T = 1024, BIG_LDS = 8712, BIG_DATA = 8192;       // 4 padded columns of 2048
 
 int ilog2(int v) {
     int l = 0;
@@ -100,11 +100,11 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 // One Stockham stage of radix R on `ncol` columns of length L held in LDS (column c at
 // buf + c * LS): butterfly j of a column reads x[j + r L/R], twiddles by W^(r k L/(Ns R)),
 // k = j mod Ns, and writes x[(j/Ns) Ns R + k + r Ns].  All reads precede all writes.
-template <int R, bool INV, int NT>
+template <int R, bool INV, int NT, int DATA>
 __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, int lgNs,
                                       const float2* __restrict__ W) {
     constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
-    constexpr int MAXB = 16 / R;  // butterflies per thread: <= 4096 NT/256 points / (R NT)
+    constexpr int MAXB = (DATA + R * NT - 1) / (R * NT);  // butterflies per thread
     const int lgnb = lgL - LGR;
     const int nb = 1 << lgnb;
     const int total = nb * ncol;
@@ -142,20 +142,20 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
 }
 
 // FFT of length 2^lgL of every column, in place, natural order in and out.
-template <bool INV, int NT>
+template <bool INV, int NT, int DATA>
 __device__ __forceinline__ void stockham(float2* buf, int LS, int ncol, int lgL,
                                          const float2* __restrict__ W) {
     int lgNs = 0;
     while (lgNs < lgL) {
         const int rem = lgL - lgNs;
         if (rem == 1) {
-            stage<2, INV, NT>(buf, LS, ncol, lgL, lgNs, W);
+            stage<2, INV, NT, DATA>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 1;
         } else if (rem == 2 || rem == 4) {
-            stage<4, INV, NT>(buf, LS, ncol, lgL, lgNs, W);
+            stage<4, INV, NT, DATA>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 2;
         } else {
-            stage<8, INV, NT>(buf, LS, ncol, lgL, lgNs, W);
+            stage<8, INV, NT, DATA>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 3;
         }
     }
@@ -234,8 +234,8 @@ __device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, in
 // its columns before it writes.
 // Workgroups are dealt to the 8 XCDs in turn; tile t = xcd * per + (id / 8) keeps the tiles
 // that share 128-B lines (adjacent x) on one XCD and its L2, one after another.
-template <bool INV, int NT, int NLDS>
-__global__ __launch_bounds__(NT) void fft_cols(
+template <bool INV, int NT, int NLDS, int DATA, int WAVES>
+__global__ __launch_bounds__(NT, WAVES) void fft_cols(
     const float2* in, float2* out, int lgL, int64_t strideA, int64_t strideB, int nx, int lgC,
     int gx, int ntiles, int b0, int r0, int r1, int w0, int w1, FftOp op,
     const float2* __restrict__ W) {
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(NT) void fft_cols(
         buf[c * LS + pad(a)] = v;
     }
     __syncthreads();
-    stockham<INV, NT>(buf, LS, C, lgL, W);
+    stockham<INV, NT, DATA>(buf, LS, C, lgL, W);
     for (int e = threadIdx.x; e < C * L; e += NT) {
         const int c = e & (C - 1), a = e >> lgC, x = x0 + c;
         if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[c * LS + pad(a)];
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(NT) void fft_cols(
 // Forward x pass: rows (iy, iz) of a w^3 real field (iy, iz < w), zero-extended to L = 2M,
 // into rows of the (M + 1) x L x L half spectrum.  z[m] = x[2m] + i x[2m + 1], Z = FFT_M(z),
 // X[k] = E[k] + W_L^k O[k] with E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i.
-__global__ __launch_bounds__(SMALL_T) void fft_rows_r2c(
+__global__ __launch_bounds__(SMALL_T, 8) void fft_rows_r2c(
     const float* __restrict__ f, int w, int lgM, int lgG, float2* __restrict__ spec,
     const float2* __restrict__ WM, const float2* __restrict__ WL) {
     __shared__ float2 buf[SMALL_LDS];
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(SMALL_T) void fft_rows_r2c(
         buf[g * LS + pad(m)] = v;
     }
     __syncthreads();
-    stockham<false, SMALL_T>(buf, LS, G, lgM, WM);
+    stockham<false, SMALL_T, SMALL_DATA>(buf, LS, G, lgM, WM);
     for (int e = threadIdx.x; e < G * nk; e += SMALL_T) {
         const int g = e / nk, k = e - g * nk, row = row0 + g;
         if (row >= nrows) continue;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(SMALL_T) void fft_rows_r2c(
 // s2 -= (scale * x)^2 * coef (mode 1).  X[0] and X[M] enter through their real parts.
 // Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) W_L^-k, z = IFFT_M(Z), x[2m] = Re z[m],
 // x[2m + 1] = Im z[m].
-__global__ __launch_bounds__(SMALL_T) void fft_rows_c2r(
+__global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r(
     const float2* __restrict__ spec, int lgM, int lgG, int s, int n, float scale, int mode,
     float coef, float* __restrict__ out, const float2* __restrict__ WM,
     const float2* __restrict__ WL) {
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(SMALL_T) void fft_rows_c2r(
     }
     __syncthreads();
     // untwist into Z (registers first: Z[k] needs X[M - k] of the same row)
-    constexpr int MAXB = SMALL_LDS / SMALL_T + 1;
+    constexpr int MAXB = SMALL_DATA / SMALL_T;
     float2 z[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(SMALL_T) void fft_rows_c2r(
         if (e < G * M) buf[(e >> lgM) * LS + pad(e & (M - 1))] = z[q];
     }
     __syncthreads();
-    stockham<true, SMALL_T>(buf, LS, G, lgM, WM);
+    stockham<true, SMALL_T, SMALL_DATA>(buf, LS, G, lgM, WM);
     for (int e = threadIdx.x; e < G * n; e += SMALL_T) {
         const int g = e / n, xo = e - g * n, row = row0 + g;
         if (row >= nrows) continue;
@@ -422,11 +422,11 @@ void launch_cols(hipStream_t st, int L, unsigned gx, int nb, const float2* in, f
                  int w1, const FftOp& op, const float2* W) {
     const int lgC = cols_lgC(L);
     if (cols_big(L))
-        hipLaunchKernelGGL((fft_cols<INV, BIG_T, BIG_LDS>), dim3(cols_grid(gx, nb)), dim3(BIG_T),
+        hipLaunchKernelGGL((fft_cols<INV, BIG_T, BIG_LDS, BIG_DATA, 8>), dim3(cols_grid(gx, nb)), dim3(BIG_T),
                            0, st, in, out, ilog2(L), strideA, strideB, nx, lgC, (int)gx,
                            (int)gx * nb, b0, r0, r1, w0, w1, op, W);
     else
-        hipLaunchKernelGGL((fft_cols<INV, SMALL_T, SMALL_LDS>), dim3(cols_grid(gx, nb)),
+        hipLaunchKernelGGL((fft_cols<INV, SMALL_T, SMALL_LDS, SMALL_DATA, 8>), dim3(cols_grid(gx, nb)),
                            dim3(SMALL_T), 0, st, in, out, ilog2(L), strideA, strideB, nx, lgC,
                            (int)gx, (int)gx * nb, b0, r0, r1, w0, w1, op, W);
 }
