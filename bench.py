@@ -265,7 +265,7 @@ def main():
     log(f"merge: {merge}")
     del recs
 
-    # ---------------- tile peak finding from the raw tile: filter-bank smoothing (hipFFT
+    # ---------------- tile peak finding from the raw tile: filter-bank smoothing (FFT
     # rfft once, window + irfft per filter) fused with the peak test (hnt_peak_find_tile);
     # reported beside the step, whose peak test reads the pre-smoothed fields
     tile_pf = None
@@ -289,7 +289,7 @@ def main():
         tile_pf = dict(ms=round(e0.elapsed_time(e1), 3), candidates=len(ft["i"]),
                        stage_ms={k: round(v[0], 3) for k, v in fprof.items() if v[1]},
                        filters=len(radii),
-                       note="one rfft, then per filter window x spectrum, irfft (hipFFT, "
+                       note="one rfft, then per filter window x spectrum, irfft (library FFT, "
                             "Float32) and the peak test; candidates differ from the step's "
                             "(torch-smoothed input) only by FFT rounding")
         del ft

@@ -541,18 +541,30 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         HNT_TRY(reserve(ctx, ctx->fb_lap, sizeof(float) * (size_t)nreal));
         HNT_TRY(reserve(ctx, ctx->fb_d2, sizeof(float) * (size_t)ncell));
     }
-    if (ctx->fft_n != n) {
-        release_fft(ctx);
-        hipfftHandle r2c = nullptr, c2r = nullptr;
-        HNT_FFT(hipfftPlan3d(&r2c, n, n, n, HIPFFT_R2C));
-        ctx->fft_r2c = r2c;
-        HNT_FFT(hipfftPlan3d(&c2r, n, n, n, HIPFFT_C2R));
-        ctx->fft_c2r = c2r;
-        ctx->fft_n = n;
+    // transforms: the library's own for power-of-two n (tuning fft = 0), else hipFFT, whose
+    // 32-bit 3-D plans are only valid below 2^31 elements
+    const bool own = ctx->fft_mode == 0 && fft_supported(n);
+    HNT_REQUIRE(own || nreal < (1LL << 31),
+                "hnt_peak_find_tile: a %d^3 tile needs the library FFT (power-of-two n, tuning "
+                "fft = 0)", n);
+    hipfftHandle r2c = nullptr, c2r = nullptr;
+    if (!own) {
+        if (ctx->fft_n != n) {
+            release_fft(ctx);
+            HNT_FFT(hipfftPlan3d(&r2c, n, n, n, HIPFFT_R2C));
+            ctx->fft_r2c = r2c;
+            HNT_FFT(hipfftPlan3d(&c2r, n, n, n, HIPFFT_C2R));
+            ctx->fft_c2r = c2r;
+            ctx->fft_n = n;
+        }
+        r2c = (hipfftHandle)ctx->fft_r2c;
+        c2r = (hipfftHandle)ctx->fft_c2r;
+        HNT_FFT(hipfftSetStream(r2c, st));
+        HNT_FFT(hipfftSetStream(c2r, st));
     }
-    hipfftHandle r2c = (hipfftHandle)ctx->fft_r2c, c2r = (hipfftHandle)ctx->fft_c2r;
-    HNT_FFT(hipfftSetStream(r2c, st));
-    HNT_FFT(hipfftSetStream(c2r, st));
+    const FftOp none;
+    float2* spec = (float2*)ctx->fb_spec.p;
+    float2* work = (float2*)ctx->fb_work.p;
     float* field = (float*)ctx->fb_field.p;
     float* lapf = lap ? (float*)ctx->fb_lap.p : nullptr;
     const float* ptrs[2] = {field, lapf};
@@ -561,11 +573,15 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
                            hipMemcpyHostToDevice, st));
     const float* const* dptrs = (const float* const*)ctx->fieldptrs.p;
 
-    // rfft of the tile (from a copy: the caller's delta stays as it was)
+    // rfft of the tile (hipFFT from a copy: the caller's delta stays as it was)
     stage_begin(ctx, HNT_STAGE_PF_SMOOTH);
-    HNT_HIP(hipMemcpyAsync(field, delta, sizeof(float) * (size_t)nreal, hipMemcpyDeviceToDevice,
-                           st));
-    HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)field, (hipfftComplex*)ctx->fb_spec.p));
+    if (own) {
+        HNT_TRY(fft_forward(ctx, delta, n, n, spec));
+    } else {
+        HNT_HIP(hipMemcpyAsync(field, delta, sizeof(float) * (size_t)nreal,
+                               hipMemcpyDeviceToDevice, st));
+        HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)field, (hipfftComplex*)spec));
+    }
     stage_end(ctx, HNT_STAGE_PF_SMOOTH);
     hipLaunchKernelGGL(pf_claim_init, dim3(2048), dim3(256), 0, st, (const int8_t*)mask, n,
                        nbuff, (uint8_t*)ctx->claim.p);
@@ -578,7 +594,10 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
                            (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
                            bank->Rf[f], bank->wsmooth, inv_n3);
-        HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fb_work.p, (hipfftReal*)field));
+        if (own)
+            HNT_TRY(fft_inverse(ctx, work, work, n, 0, n, none, 1.0f, 0, 0.0f, field));
+        else
+            HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)field));
         if (smoothed_out && smoothed_out[f])
             HNT_HIP(hipMemcpyAsync(smoothed_out[f], field, sizeof(float) * (size_t)nreal,
                                    hipMemcpyDeviceToDevice, st));
@@ -586,7 +605,10 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
             hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
                                (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
                                bank->Rf[f], 3, inv_n3);
-            HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)ctx->fb_work.p, (hipfftReal*)lapf));
+            if (own)
+                HNT_TRY(fft_inverse(ctx, work, work, n, 0, n, none, 1.0f, 0, 0.0f, lapf));
+            else
+                HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)lapf));
         }
         stage_end(ctx, HNT_STAGE_PF_SMOOTH);
         stage_begin(ctx, HNT_STAGE_PF_MARK);

@@ -399,7 +399,8 @@ hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int64_t nhalo,
 /* ---------------------------------------------------------------- tile field synthesis */
 /* The GPU methods that build a tile's input fields (src/MultiResolution.jl:881-1010 calls
  * them through the stubs of src/Huntian.jl:283-385).  Fields are device Float32 arrays,
- * column-major [i,j,k] (i fastest), as the reference's CuArrays.  Transforms use hipFFT;
+ * column-major [i,j,k] (i fastest), as the reference's CuArrays.  Transforms are the
+ * library's own for power-of-two sizes up to 2048 and hipFFT otherwise (see HNT_TUNE_FFT);
  * inverse transforms are normalised (1/n^3), as AbstractFFTs' irfft. */
 
 /* sqrt(P(k) dk^3 n2^3) tabulated at pk_table_n points uniform in ln k from ln(dk/2) to
