@@ -28,8 +28,9 @@ namespace {
 // workgroups per CU) for columns up to 1024 and for all rows; 1024 threads and 68 KB (four
 // 2048-point columns, two workgroups per CU) for 2048-point columns, where the small shape
 // would read 16-B pieces of each line.
-constexpr int SMALL_T = 512, SMALL_LDS = 4360, SMALL_DATA = 4096;  // 4 padded columns of 1024
-constexpr int BIG_T = 1024, BIG_LDS = 8712, BIG_DATA = 8192;       // 4 padded columns of 2048
+constexpr int SMALL_T = 512, SMALL_LDS = 4368, SMALL_DATA = 4096;  // 4 padded columns of 1024
+constexpr int BIG_T = 1024, BIG_LDS = 8720, BIG_DATA = 8192;       // 4 padded columns of 2048
+
 
 int ilog2(int v) {
     int l = 0;
@@ -41,6 +42,14 @@ int ilog2(int v) {
 // writes of the first Stockham stages (stride R complex) spread over the banks
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int padded_len(int L) { return L + (L >> 4) + 1; }
+// Column stride of the column passes: the padded length raised to LS = 16 / C (mod 16), so
+// that the C columns a 16-lane group touches when it loads or stores one row of each (c
+// fastest across lanes) fall in distinct 8-B bank slots.
+inline int cols_ls(int L, int C) {
+    const int base = padded_len(L);
+    const int target = (16 / C) & 15;
+    return base + ((target - base) % 16 + 16) % 16;
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -237,13 +246,13 @@ __device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, in
 template <bool INV, int NT, int NLDS, int DATA, int WAVES>
 __global__ __launch_bounds__(NT, WAVES) void fft_cols(
     const float2* in, float2* out, int lgL, int64_t strideA, int64_t strideB, int nx, int lgC,
-    int gx, int ntiles, int b0, int r0, int r1, int w0, int w1, FftOp op,
+    int gx, int ntiles, int LS, int b0, int r0, int r1, int w0, int w1, FftOp op,
     const float2* __restrict__ W) {
     __shared__ float2 buf[NLDS];
     const int per = (ntiles + 7) >> 3;
     const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (tile >= ntiles) return;
-    const int L = 1 << lgL, C = 1 << lgC, LS = padded_len(L);
+    const int L = 1 << lgL, C = 1 << lgC;
     const int x0 = (tile % gx) * C;
     const int b = b0 + tile / gx;
     const int64_t boff = (int64_t)b * strideB;
@@ -406,7 +415,7 @@ int cols_lgC(int L) {
     // as many padded columns as the shape's LDS holds, at most 8 (64-B segments)
     const int cap = cols_big(L) ? BIG_LDS : SMALL_LDS;
     int lg = 3;
-    while (lg > 0 && (1 << lg) * padded_len(L) > cap) --lg;
+    while (lg > 0 && (1 << lg) * cols_ls(L, 1 << lg) > cap) --lg;
     return lg;
 }
 
@@ -421,14 +430,15 @@ void launch_cols(hipStream_t st, int L, unsigned gx, int nb, const float2* in, f
                  int64_t strideA, int64_t strideB, int nx, int b0, int r0, int r1, int w0,
                  int w1, const FftOp& op, const float2* W) {
     const int lgC = cols_lgC(L);
+    const int LS = cols_ls(L, 1 << lgC);
     if (cols_big(L))
         hipLaunchKernelGGL((fft_cols<INV, BIG_T, BIG_LDS, BIG_DATA, 8>), dim3(cols_grid(gx, nb)), dim3(BIG_T),
                            0, st, in, out, ilog2(L), strideA, strideB, nx, lgC, (int)gx,
-                           (int)gx * nb, b0, r0, r1, w0, w1, op, W);
+                           (int)gx * nb, LS, b0, r0, r1, w0, w1, op, W);
     else
         hipLaunchKernelGGL((fft_cols<INV, SMALL_T, SMALL_LDS, SMALL_DATA, 8>), dim3(cols_grid(gx, nb)),
                            dim3(SMALL_T), 0, st, in, out, ilog2(L), strideA, strideB, nx, lgC,
-                           (int)gx, (int)gx * nb, b0, r0, r1, w0, w1, op, W);
+                           (int)gx, (int)gx * nb, LS, b0, r0, r1, w0, w1, op, W);
 }
 
 }  // namespace
