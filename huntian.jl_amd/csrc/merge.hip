@@ -16,10 +16,12 @@
 // adds, for halo j, first the caps cut by larger overlapping halos (in their rank order, as
 // its outer loop visits them), then its own caps against smaller halos in its cell-scan order
 // (cells by dz, dy, dx over +-ceil(2 r / cell); within a cell the hash's linked list, i.e.
-// descending catalog index).  Both orders are rebuilt here: the incoming caps are sorted by
-// (j, rank of i) and summed per halo, then each halo continues with its own caps in the
-// reference's cell order over a cell-sorted copy of the catalog (same grid as the reference's
-// hash: nc = clamp(round(cbrt(N)), 16, 1024) cells over the bounding box padded by 2 r_max).
+// descending catalog index).  Both orders are rebuilt with one sort: every overlapping pair
+// writes the cap cut from the smaller halo keyed by (halo, rank of the larger) and the
+// larger halo's own cap keyed by (halo, N + position of the smaller in a cell-sorted copy of
+// the catalog), whose order is the reference's cell-scan order (same grid as the reference's
+// hash: nc = clamp(round(cbrt(N)), 16, 1024) cells over the bounding box padded by 2 r_max);
+// one thread per halo then adds its run.
 //
 // Layout: after the rank and cell sorts every pass works on positions p of the cell-sorted
 // order, where halo idx[p] keeps (x, y, z, r) in one float4 and its rank beside it, so the
@@ -221,13 +223,15 @@ __device__ __forceinline__ void sphere_overlap(double d, double r1, double r2, d
 }
 
 // Overlapping survivor pairs (p larger by rank) of volume_reduction! (Exclusion.jl:163-180).
-// mode 0 counts pairs per p; mode 1 writes the cap each pair cuts from the smaller halo q
-// as (key = q << 32 | rank p, value v2); mode 2 adds p's own caps v1, in the reference's
-// visiting order, to dV[p] (which holds the incoming sum).
+// Every pair gives two caps: the one cut from the smaller halo q, keyed (q << 32 | rank p),
+// and p's own, keyed (p << 32 | N + q).  Sorted by key, a halo's caps come in the reference's
+// order: those cut by larger halos in their rank order, then its own in its cell-scan order
+// (cells by dz, dy, dx, descending catalog index within a cell = ascending position q).
+// mode 0 counts two entries per pair; mode 1 writes them from off[p].
 __global__ void merge_pairs(Grid g, int mode, const uint8_t* __restrict__ state,
                             int fortran_ties, int64_t* __restrict__ cnt,
                             const int64_t* __restrict__ off, uint64_t* __restrict__ keys,
-                            double* __restrict__ vals, double* __restrict__ dV) {
+                            double* __restrict__ vals) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.N) return;
     if (state[p] != 1) {
@@ -239,7 +243,6 @@ __global__ void merge_pairs(Grid g, int mode, const uint8_t* __restrict__ state,
     const int dcell = (int)ceil(2.0 * ri / g.cell);
     const int32_t rkp = g.prank[p];
     int64_t n = 0, w = mode == 1 ? off[p] : 0;
-    double acc = mode == 2 ? dV[p] : 0.0;
     for_neighbours(g, xi, yi, zi, dcell, [&](int64_t q) {
         if (g.prank[q] <= rkp || state[q] != 1) return;
         const float4 o = g.pos[q];
@@ -248,26 +251,23 @@ __global__ void merge_pairs(Grid g, int mode, const uint8_t* __restrict__ state,
         const double d = sqrt(ex * ex + ey * ey + ez * ez);
         if (!(d < ri + rj)) return;
         if (mode == 0) {
-            ++n;
+            n += 2;
             return;
         }
         double v1, v2;
         sphere_overlap(d, ri, rj, v1, v2);
         const double wgt = (fortran_ties && rj == ri) ? 2.0 : 1.0;
-        if (mode == 1) {
-            keys[w] = ((uint64_t)q << 32) | (uint32_t)rkp;
-            vals[w] = wgt * v2;
-            ++w;
-        } else {
-            acc += wgt * v1;
-        }
+        keys[w] = ((uint64_t)q << 32) | (uint32_t)rkp;
+        vals[w] = wgt * v2;
+        keys[w + 1] = ((uint64_t)p << 32) | (uint32_t)(g.N + q);
+        vals[w + 1] = wgt * v1;
+        w += 2;
     });
     if (mode == 0) cnt[p] = n;
-    if (mode == 2) dV[p] = acc;
 }
 
-// dV[q] = the caps cut from q by larger halos, summed in their rank order (pairs sorted by
-// (q, rank p); one thread per halo walks its run).
+// dV[j] = halo j's caps summed in the reference's order (entries sorted by key; one thread
+// per halo walks its run).
 __global__ void merge_incoming(const double* __restrict__ vals, const int64_t* __restrict__ start,
                                int64_t N, double* __restrict__ dV) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -530,8 +530,7 @@ extern "C" hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int6
     double* dV = (double*)B.dv.p;
     if (volume_reduction) {
         hipLaunchKernelGGL(merge_pairs, dim3(nblk(N)), dim3(MB), 0, st, G, 0,
-                           (const uint8_t*)state, fortran_ties, cnt, nullptr, nullptr, nullptr,
-                           nullptr);
+                           (const uint8_t*)state, fortran_ties, cnt, nullptr, nullptr, nullptr);
         HNT_HIP(hipMemsetAsync(cnt + N, 0, 8, st));
         {
             size_t t = ctx->cub_tmp.bytes;
@@ -541,7 +540,8 @@ extern "C" hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int6
         int64_t P = 0;
         HNT_HIP(hipMemcpyAsync(&P, off + N, 8, hipMemcpyDeviceToHost, st));
         HNT_HIP(hipStreamSynchronize(st));
-        HNT_REQUIRE(P < (1LL << 31), "hnt_merge_catalog: %lld overlapping pairs", (long long)P);
+        HNT_REQUIRE(P < (1LL << 31), "hnt_merge_catalog: %lld cap entries (2 per overlapping "
+                    "pair)", (long long)P);
         if (P > 0) {
             HNT_TRY(reserve(ctx, B.pk, (size_t)P * 8 * 2));
             HNT_TRY(reserve(ctx, B.pv, (size_t)P * 8 * 2));
@@ -551,7 +551,7 @@ extern "C" hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int6
             double* pv1 = pv0 + P;
             hipLaunchKernelGGL(merge_pairs, dim3(nblk(N)), dim3(MB), 0, st, G, 1,
                                (const uint8_t*)state, fortran_ties, nullptr,
-                               (const int64_t*)off, pk0, pv0, nullptr);
+                               (const int64_t*)off, pk0, pv0);
             size_t t = 0;
             HNT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t, pk0, pk1, pv0, pv1, (int)P, 0,
                                                        64, st));
@@ -566,9 +566,6 @@ extern "C" hnt_status hnt_merge_catalog(hnt_ctx* ctx, const float* records, int6
         } else {
             HNT_HIP(hipMemsetAsync(dV, 0, (size_t)N * 8, st));
         }
-        hipLaunchKernelGGL(merge_pairs, dim3(nblk(N)), dim3(MB), 0, st, G, 2,
-                           (const uint8_t*)state, fortran_ties, nullptr, nullptr, nullptr,
-                           nullptr, dV);
     }
 
     // ---- survivors, in catalog order, with their new radii ----
