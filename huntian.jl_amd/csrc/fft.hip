@@ -129,8 +129,28 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = col[pad(j + r * nb)];
             const int k = j & (Ns - 1);
-#pragma unroll
-            for (int r = 1; r < R; ++r) v[q][r] = ctw<INV>(v[q][r], W[(k * r) << tw_shift]);
+            if (k != 0) {
+                // W^k from the table, the higher powers by products (at most four roundings
+                // more than the table's): one table read per butterfly instead of R - 1, whose
+                // latency each stage waited for (computing W^k with sincospi was slower)
+                const float2 w1 = W[k << tw_shift];
+                if constexpr (R == 2) {
+                    v[q][1] = ctw<INV>(v[q][1], w1);
+                } else {
+                    const float2 w2 = ctw<false>(w1, w1);
+                    v[q][1] = ctw<INV>(v[q][1], w1);
+                    v[q][2] = ctw<INV>(v[q][2], w2);
+                    const float2 w3 = ctw<false>(w1, w2);
+                    v[q][3] = ctw<INV>(v[q][3], w3);
+                    if constexpr (R == 8) {
+                        const float2 w4 = ctw<false>(w2, w2);
+                        v[q][4] = ctw<INV>(v[q][4], w4);
+                        v[q][5] = ctw<INV>(v[q][5], ctw<false>(w1, w4));
+                        v[q][6] = ctw<INV>(v[q][6], ctw<false>(w2, w4));
+                        v[q][7] = ctw<INV>(v[q][7], ctw<false>(w3, w4));
+                    }
+                }
+            }
             dft<R, INV>(v[q]);
         }
     }

@@ -437,7 +437,6 @@ extern "C" hnt_status hnt_isolated_convolve(hnt_ctx* ctx, const float* noise, in
     HNT_TRY(plans_for(ctx, n2, r2c, c2r));
     HNT_TRY(padded_rfft(ctx, noise, n_input, n2, r2c));
     HNT_TRY(reserve(ctx, ctx->fs_work, sizeof(float2) * (size_t)half_spectrum(n2)));
-    const int64_t nh = half_spectrum(n2);
     for (int q = 0; q < nkernels; ++q) {
         const hnt_transfer& k = kernels[q];
         hipLaunchKernelGGL(fs_transfer, dim3(grid_rows((int64_t)(n2 / 2 + 1) * (n2 / 2 + 1))),
