@@ -261,9 +261,25 @@ def main():
         merge["reduction" if reduce else "exclusion"] = dict(
             ms=round(e0.elapsed_time(e1), 3), halos_in=int(recs.shape[0]),
             halos_out=int(merged.shape[0]))
-        del merged
     log(f"merge: {merge}")
     del recs
+
+    # ---------------- Eulerian positions and velocities of the merged catalog
+    # (hnt_finalize_eulerian, lightcone mode: each halo's z from its distance to an observer
+    # 2000 Mpc/h away), in place; warm call, then timed
+    fkw = dict(Om=cos.Om, OL=cos.OL, ievol=1, chi2z=cosmology.build_chi_to_z(cos, z_max=10.0))
+    obs = (0.0, 0.0, -2000.0)
+    fin_in = merged.clone()
+    api.finalize_eulerian(fin_in, growth, obs, out=fin_in, **fkw)
+    torch.cuda.synchronize()
+    e0.record()
+    api.finalize_eulerian(merged, growth, obs, out=merged, **fkw)
+    e1.record()
+    torch.cuda.synchronize()
+    finalize = dict(ms=round(e0.elapsed_time(e1), 3), halos=int(merged.shape[0]),
+                    note="hnt_finalize_eulerian in place, lightcone mode")
+    log(f"finalize: {finalize}")
+    del merged, fin_in
 
     # ---------------- tile peak finding from the raw tile: filter-bank smoothing (FFT
     # rfft once, window + irfft per filter) fused with the peak test (hnt_peak_find_tile);
@@ -417,6 +433,7 @@ def main():
                    "tile_peak_find": tile_pf,
                    "records": records,
                    "merge": merge,
+                   "finalize": finalize,
                    "field_synthesis": synth_t,
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,

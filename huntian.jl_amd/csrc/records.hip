@@ -11,6 +11,9 @@
 // where device and host pow() differ in the last place (the 2LPT factor).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <vector>
+
 #include "hnt_internal.h"
 
 namespace {
@@ -88,6 +91,41 @@ __device__ double growth_D(const double* atab, const double* Dtab, int ntab, dou
     }
     if (ai >= atab[ntab - 1]) return Dtab[ntab - 1] * ai;
     return Dtab[0] * ai;
+}
+
+// Dlinear_ab(ai)[2] = f = dlnD/dlna (Cosmology.jl:287-307): the same 4-point (2 at the table
+// ends) Lagrange weights as growth_D over the growth-rate column; end values outside the table.
+__device__ double growth_f(const double* atab, const double* ftab, int ntab, double ai) {
+    if (ai > atab[0] && ai < atab[ntab - 1]) {
+        const int kk = hunt(atab, ntab, ai);
+        int nlint, kkl;
+        if (kk < 2 || kk >= ntab) {
+            nlint = 2;
+            kkl = kk - 1 > 1 ? kk - 1 : 1;
+        } else {
+            nlint = 4;
+            kkl = kk - 2;
+        }
+        const double* xa = atab + (kkl - 1);
+        const double* v = ftab + (kkl - 1);
+        const double x = ai;
+        if (nlint == 4) {
+            const double a1 = (x - xa[1]) / (xa[0] - xa[1]) * (x - xa[2]) / (xa[0] - xa[2]) *
+                              (x - xa[3]) / (xa[0] - xa[3]);
+            const double a2 = (x - xa[0]) / (xa[1] - xa[0]) * (x - xa[2]) / (xa[1] - xa[2]) *
+                              (x - xa[3]) / (xa[1] - xa[3]);
+            const double a3 = (x - xa[0]) / (xa[2] - xa[0]) * (x - xa[1]) / (xa[2] - xa[1]) *
+                              (x - xa[3]) / (xa[2] - xa[3]);
+            const double a4 = (x - xa[0]) / (xa[3] - xa[0]) * (x - xa[1]) / (xa[3] - xa[1]) *
+                              (x - xa[2]) / (xa[3] - xa[2]);
+            return a1 * v[0] + a2 * v[1] + a3 * v[2] + a4 * v[3];
+        }
+        const double a1 = (x - xa[1]) / (xa[0] - xa[1]);
+        const double a2 = 1.0 - a1;
+        return a1 * v[0] + a2 * v[1];
+    }
+    if (ai >= atab[ntab - 1]) return ftab[ntab - 1];
+    return ftab[0];
 }
 
 // chi_to_z (Cosmology.jl:357-372): bisection in the comoving-distance table, then linear
@@ -204,6 +242,69 @@ __global__ __launch_bounds__(REC_BLOCK) void rec_write_kernel(RecArgs a,
     for (int64_t e = threadIdx.x; e < nwrite; e += REC_BLOCK) dst[e] = rows[e];
 }
 
+
+struct FinArgs {
+    const float* rec;
+    float* out;
+    int64_t N;
+    int nf;
+    const double* atab;
+    const double* ftab;
+    int ntab;
+    const double* chiz;
+    const double* chid;
+    int nchi;
+    double chi_zmax;
+    double Om, OL, ox, oy, oz, a_snap;
+    int ievol;
+};
+
+// finalize_eulerian (Merger.jl:123-148) per halo, in the reference's operation order
+// One block finalizes FIN_T records: they are loaded into LDS as one contiguous run
+// (coalesced), each thread rewrites its record there, and the run is stored back.  A block
+// reads all of its records before it writes any, and blocks own disjoint runs, so out may
+// equal rec.  The record stride (11 or 33 floats) is odd: no LDS bank conflicts.
+constexpr int FIN_T = 256;
+
+__global__ __launch_bounds__(FIN_T) void fin_kernel(FinArgs a) {
+    __shared__ float tile[FIN_T * 33];
+    const int64_t base = (int64_t)blockIdx.x * FIN_T;
+    const int cnt = (int)(a.N - base < FIN_T ? a.N - base : FIN_T);
+    const int tot = cnt * a.nf;
+    const float* src = a.rec + base * a.nf;
+    for (int q = threadIdx.x; q < tot; q += FIN_T) tile[q] = src[q];
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+        float* h = tile + threadIdx.x * a.nf;
+        const double d1x = h[3], d1y = h[4], d1z = h[5];
+        const double d2x = h[7], d2y = h[8], d2z = h[9];
+        const double ex = (double)h[0] + d1x + d2x, ey = (double)h[1] + d1y + d2y,
+                     ez = (double)h[2] + d1z + d2z;
+        double av = a.a_snap;
+        if (a.ievol == 1) {
+            const double rx = ex - a.ox, ry = ey - a.oy, rz = ez - a.oz;
+            const double r = sqrt(rx * rx + ry * ry + rz * rz);
+            av = r > 0.0 ? 1.0 / (1.0 + chi_to_z(a.chiz, a.chid, a.nchi, a.chi_zmax, r)) : 1.0;
+        }
+        const double ia = 1.0 / av;  // a^-3 = inv(a)^3
+        const double E = sqrt(a.Om * (ia * ia * ia) + a.OL);
+        const double f = growth_f(a.atab, a.ftab, a.ntab, av);
+        const double vfac = av * 100.0 * E * f;
+        h[0] = (float)ex;
+        h[1] = (float)ey;
+        h[2] = (float)ez;
+        h[3] = (float)(vfac * (d1x + 2.0 * d2x));
+        h[4] = (float)(vfac * (d1y + 2.0 * d2y));
+        h[5] = (float)(vfac * (d1z + 2.0 * d2z));
+        h[7] = 0.0f;
+        h[8] = 0.0f;
+        h[9] = 0.0f;
+    }
+    __syncthreads();
+    float* dst = a.out + base * a.nf;
+    for (int q = threadIdx.x; q < tot; q += FIN_T) dst[q] = tile[q];
+}
+
 }  // namespace
 
 extern "C" hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch,
@@ -308,4 +409,71 @@ extern "C" hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch,
         return HNT_ERR_CAPACITY;
     }
     return HNT_OK;
+}
+
+extern "C" hnt_status hnt_finalize_eulerian(hnt_ctx* ctx, const float* records, int64_t nhalo,
+                                            int32_t nfields, const hnt_cosmo_tables* tabs,
+                                            double Om, double OL, const double* obs,
+                                            int32_t ievol, double z_out, float* out) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(tabs && obs && (nhalo == 0 || (records && out)),
+                "hnt_finalize_eulerian: NULL argument");
+    HNT_REQUIRE(nfields == 11 || nfields == 33, "hnt_finalize_eulerian: records of %d floats "
+                "(11 or 33)", nfields);
+    // a launch holds fewer than 2^32 work items
+    HNT_REQUIRE(nhalo >= 0 && nhalo <= ((int64_t)1 << 32) - FIN_T,
+                "hnt_finalize_eulerian: %lld halos", (long long)nhalo);
+    HNT_REQUIRE(ievol == 0 || ievol == 1, "hnt_finalize_eulerian: ievol %d", ievol);
+    HNT_REQUIRE(tabs->ntab >= 4 && tabs->atab && tabs->dlnD_atab,
+                "hnt_finalize_eulerian: the growth tables (atab, dlnD_atab, >= 4 points) are "
+                "needed");
+    // the smallest scale factor reached: 1/(1 + z_max) of the chi -> z table, or of z_out;
+    // above the table's third node (the reference's lookup is defined there)
+    if (ievol == 1) {
+        HNT_REQUIRE(tabs->nchi >= 2 && tabs->chi2z_z && tabs->chi2z_chi,
+                    "hnt_finalize_eulerian: ievol = 1 needs the chi -> z table");
+        HNT_REQUIRE(1.0 / (1.0 + tabs->chi2z_zmax) > tabs->atab[2],
+                    "hnt_finalize_eulerian: z up to %g is beyond the growth table",
+                    tabs->chi2z_zmax);
+    } else {
+        HNT_REQUIRE(z_out > -1.0 && 1.0 / (1.0 + z_out) > tabs->atab[2],
+                    "hnt_finalize_eulerian: z_out %g is beyond the growth table", z_out);
+    }
+    if (nhalo == 0) return HNT_OK;
+    hipStream_t st = ctx->stream;
+    const int nt = tabs->ntab, nc = ievol == 1 ? tabs->nchi : 0;
+    std::vector<double> host((size_t)2 * nt + 2 * (size_t)nc);
+    std::copy(tabs->atab, tabs->atab + nt, host.begin());
+    std::copy(tabs->dlnD_atab, tabs->dlnD_atab + nt, host.begin() + nt);
+    if (nc) {
+        std::copy(tabs->chi2z_z, tabs->chi2z_z + nc, host.begin() + 2 * nt);
+        std::copy(tabs->chi2z_chi, tabs->chi2z_chi + nc, host.begin() + 2 * nt + nc);
+    }
+    HNT_TRY(reserve(ctx, ctx->rec_tabs, sizeof(double) * host.size()));
+    HNT_HIP(hipMemcpyAsync(ctx->rec_tabs.p, host.data(), sizeof(double) * host.size(),
+                           hipMemcpyHostToDevice, st));
+    const double* d = (const double*)ctx->rec_tabs.p;
+    FinArgs a;
+    a.rec = records;
+    a.out = out;
+    a.N = nhalo;
+    a.nf = nfields;
+    a.atab = d;
+    a.ftab = d + nt;
+    a.ntab = nt;
+    a.chiz = nc ? d + 2 * nt : nullptr;
+    a.chid = nc ? d + 2 * nt + nc : nullptr;
+    a.nchi = nc;
+    a.chi_zmax = tabs->chi2z_zmax;
+    a.Om = Om;
+    a.OL = OL;
+    a.ox = obs[0];
+    a.oy = obs[1];
+    a.oz = obs[2];
+    a.a_snap = 1.0 / (1.0 + z_out);
+    a.ievol = ievol;
+    hipLaunchKernelGGL(fin_kernel, dim3((unsigned)((nhalo + FIN_T - 1) / FIN_T)), dim3(FIN_T),
+                       0, st, a);
+    // the host tables are read by the copy above before this returns
+    return finish_call(ctx, "hnt_finalize_eulerian");
 }

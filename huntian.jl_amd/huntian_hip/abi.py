@@ -138,6 +138,7 @@ class CosmoTables(C.Structure):
         ("chi2z_z", _f64p),
         ("chi2z_chi", _f64p),
         ("chi2z_zmax", C.c_double),
+        ("dlnD_atab", _f64p),
     ]
 
 
@@ -246,6 +247,7 @@ EXPORTED = (
     "hnt_plan_launch",
     "hnt_plan_destroy",
     "hnt_pack_records",
+    "hnt_finalize_eulerian",
     "hnt_merge_catalog",
     "hnt_isolated_convolve",
     "hnt_compute_2lpt",
@@ -309,6 +311,9 @@ def load(path: str | None = None):
     lib.hnt_pack_records.argtypes = [vp, C.POINTER(Batch), C.POINTER(Results), _f32p, _f32p,
                                      C.POINTER(CosmoTables), C.POINTER(PackParams), _f32p,
                                      C.c_int64, C.POINTER(C.c_int64)]
+    lib.hnt_finalize_eulerian.argtypes = [vp, _f32p, C.c_int64, C.c_int32,
+                                          C.POINTER(CosmoTables), C.c_double, C.c_double,
+                                          _f64p, C.c_int32, C.c_double, _f32p]
     lib.hnt_merge_catalog.argtypes = [vp, _f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                       _f32p, C.POINTER(C.c_int64)]
     fpp = C.POINTER(_f32p)

@@ -547,6 +547,10 @@ def cosmo_tables_struct(growth, chi2z=None):
     else:
         t = abi.CosmoTables(len(at), at.ctypes.data_as(f64), Dt.ctypes.data_as(f64), 0,
                             None, None, 0.0)
+    if len(growth) > 2:
+        ft = np.ascontiguousarray(np.asarray(growth[2], np.float64))
+        keep.append(ft)
+        t.dlnD_atab = ft.ctypes.data_as(f64)
     return t, keep
 
 
@@ -598,6 +602,39 @@ def pack_records(pi, pj, pk, Rf, results: dict, FcRf, *, growth, chi2z=None, d2R
         abi.check(ctx.lib, st)
     del keep, tkeep
     return out[:min(int(cnt.value), int(capacity))]
+
+
+def finalize_eulerian(records, growth, obs=(0.0, 0.0, 0.0), *, Om, OL, ievol=1, z_out=0.0,
+                      chi2z=None, out=None):
+    """Eulerian positions and km/s velocities of a raw catalog on the device
+    (hnt_finalize_eulerian; Merger.finalize_eulerian, src/Merger/Merger.jl:107-166).
+
+    records: (N, 11) or (N, 33) float32 rows, device or host; growth: the three
+    cosmology.dlinear_tables columns; chi2z: cosmology.build_chi_to_z(cosmo, z_max=10.0)
+    for ievol = 1 (the reference's table).  out: a device float32 tensor of the records'
+    shape (records itself for an in-place update); a new one by default.  Returns out."""
+    torch = _require_gpu()
+    dev = records.device if isinstance(records, torch.Tensor) and records.is_cuda else \
+        torch.device("cuda", torch.cuda.current_device())
+    rec = torch.as_tensor(records, dtype=torch.float32).to(dev).contiguous()
+    if rec.ndim != 2 or rec.shape[1] not in (11, 33):
+        raise ValueError(f"records must be (N, 11) or (N, 33), got {tuple(rec.shape)}")
+    if len(growth) < 3:
+        raise ValueError("growth must hold (atab, D_atab, dlnD_dlnatab)")
+    ctx = context(dev.index or 0)
+    if out is None:
+        out = torch.empty_like(rec)
+    elif (not out.is_cuda or out.dtype != torch.float32 or out.shape != rec.shape
+          or not out.is_contiguous()):
+        raise ValueError("out must be a contiguous device float32 tensor of the records' shape")
+    t, keep = cosmo_tables_struct(growth, chi2z if int(ievol) == 1 else None)
+    o = (C.c_double * 3)(*[float(v) for v in obs])
+    abi.check(ctx.lib, ctx.lib.hnt_finalize_eulerian(
+        ctx.handle, abi.as_ptr(rec.data_ptr(), C.c_float), int(rec.shape[0]), int(rec.shape[1]),
+        C.byref(t), float(Om), float(OL), o, int(ievol), float(z_out),
+        abi.as_ptr(out.data_ptr(), C.c_float)))
+    del keep
+    return out
 
 
 def merge_catalog(records, *, volume_reduction: bool = False, fortran_ties: bool = False):

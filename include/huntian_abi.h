@@ -329,7 +329,9 @@ hnt_status hnt_plan_destroy(hnt_plan* plan);
  *  growth: Dlinear_tables (src/Cosmology/Cosmology.jl:161-284): ntab ascending scale factors
  *          atab and D(a)/a in D_atab, normalised to D(1) = 1;
  *  chi2z:  build_chi_to_z (:343-350): nchi redshifts (uniform from 0 to chi2z_zmax) and their
- *          comoving distances, Mpc/h; needed in lightcone mode only (nchi = 0 otherwise). */
+ *          comoving distances, Mpc/h; needed in lightcone mode only (nchi = 0 otherwise);
+ *  dlnD_atab: the growth rate dlnD/dlna at atab (Dlinear_tables' third column); needed by
+ *          hnt_finalize_eulerian only (NULL otherwise). */
 typedef struct {
     int32_t ntab;
     const double* atab;
@@ -338,6 +340,7 @@ typedef struct {
     const double* chi2z_z;
     const double* chi2z_chi;
     double chi2z_zmax;
+    const double* dlnD_atab;
 } hnt_cosmo_tables;
 
 /* What the record loop of the tile driver knows (src/MultiResolution.jl:1159-1219). */
@@ -373,6 +376,19 @@ hnt_status hnt_pack_records(hnt_ctx* ctx, const hnt_batch* batch, const hnt_resu
                             const float* FcRf, const float* d2Rf, const hnt_cosmo_tables* tabs,
                             const hnt_pack_params* pp, float* records, int64_t capacity,
                             int64_t* count);
+
+/* hnt_finalize_eulerian -- replaces Merger.finalize_eulerian (src/Merger/Merger.jl:107-166):
+ * a raw catalog (Lagrangian positions in x y z, 1LPT displacement in vx vy vz, 2LPT
+ * displacement in vx2 vy2 vz2, Mpc/h) to Eulerian positions x + d1 + d2 and peculiar
+ * velocities a 100 E(a) f(a) (d1 + 2 d2) km/s, E = sqrt(Om a^-3 + OL), f = Dlinear_ab(a)[2];
+ * a from the Eulerian distance to obs through the chi -> z table (ievol = 1; a = 1 at the
+ * observer) or 1 / (1 + z_out) (ievol = 0).  vx2 vy2 vz2 are set to 0, every other field is
+ * kept.  records, out: device [nhalo * nfields] (11 or 33), out may equal records.
+ * tabs needs atab and dlnD_atab (and the chi -> z table for ievol = 1). */
+hnt_status hnt_finalize_eulerian(hnt_ctx* ctx, const float* records, int64_t nhalo,
+                                 int32_t nfields, const hnt_cosmo_tables* tabs, double Om,
+                                 double OL, const double* obs, int32_t ievol, double z_out,
+                                 float* out);
 
 /* ---------------------------------------------------------------- catalog merge */
 

@@ -1055,6 +1055,41 @@ double hro_growth_D(const hnt_cosmo_tables* t, double ai) {
     return Dt[0] * ai;
 }
 
+/* Dlinear_ab(ai)[2] = f (Cosmology.jl:287-307): _lagrint_2col's second column with the same
+ * weights; the table's end values outside it. */
+double hro_growth_f(const hnt_cosmo_tables* t, double ai) {
+    const double* at = t->atab;
+    const double* ft = t->dlnD_atab;
+    int ntab = t->ntab;
+    if (ai > at[0] && ai < at[ntab - 1]) {
+        int kk = hunt_1(at, ntab, ai);
+        int nlint, kkl;
+        if (kk < 2 || kk >= ntab) {
+            nlint = 2;
+            kkl = kk - 1 > 1 ? kk - 1 : 1;
+        } else {
+            nlint = 4;
+            kkl = kk - 2;
+        }
+        if (kkl < 1) return NAN; /* the reference's index would fall before its table */
+        double xa1 = at[kkl - 1], xa2 = at[kkl], x = ai;
+        if (nlint == 4) {
+            double xa3 = at[kkl + 1], xa4 = at[kkl + 2];
+            double a1 = (x - xa2) / (xa1 - xa2) * (x - xa3) / (xa1 - xa3) * (x - xa4) / (xa1 - xa4);
+            double a2 = (x - xa1) / (xa2 - xa1) * (x - xa3) / (xa2 - xa3) * (x - xa4) / (xa2 - xa4);
+            double a3 = (x - xa1) / (xa3 - xa1) * (x - xa2) / (xa3 - xa2) * (x - xa4) / (xa3 - xa4);
+            double a4 = (x - xa1) / (xa4 - xa1) * (x - xa2) / (xa4 - xa2) * (x - xa3) / (xa4 - xa3);
+            return a1 * ft[kkl - 1] + a2 * ft[kkl] + a3 * ft[kkl + 1] + a4 * ft[kkl + 2];
+        }
+        double a1 = (x - xa2) / (xa1 - xa2);
+        double a2 = 1.0 - a1;
+        return a1 * ft[kkl - 1] + a2 * ft[kkl];
+    } else if (ai >= at[ntab - 1]) {
+        return ft[ntab - 1];
+    }
+    return ft[0];
+}
+
 /* chi_to_z (Cosmology.jl:357-372) */
 double hro_chi_to_z(const hnt_cosmo_tables* t, double chi_val) {
     const double* zs = t->chi2z_z;
@@ -1329,4 +1364,45 @@ int64_t hro_merge_catalog(const float* rec, int64_t N, int32_t nf, int32_t reduc
     }
     free(surv); free(newr);
     return st == 0 ? nout : st;
+}
+
+/* ------------------------------------------------------------------ finalize_eulerian */
+
+/* Merger.finalize_eulerian (src/Merger/Merger.jl:123-166): Eulerian position q + d1 + d2 and
+ * velocity a 100 E(a) f(a) (d1 + 2 d2), E = sqrt(Om a^-3 + OL) with a^-3 as (1/a)^3; the
+ * 2LPT fields zeroed, the rest kept.  a from the Eulerian distance to obs (ievol = 1) or
+ * 1/(1 + z_out). */
+void hro_finalize_eulerian(const float* rec, int64_t N, int32_t nf, const hnt_cosmo_tables* t,
+                           double Om, double OL, const double* obs, int32_t ievol, double z_out,
+                           float* out) {
+    double a_snap = 1.0 / (1.0 + z_out);
+    for (int64_t i = 0; i < N; ++i) {
+        const float* h = rec + i * nf;
+        float* o = out + i * nf;
+        double d1x = h[3], d1y = h[4], d1z = h[5];
+        double d2x = h[7], d2y = h[8], d2z = h[9];
+        double ex = (double)h[0] + d1x + d2x, ey = (double)h[1] + d1y + d2y,
+               ez = (double)h[2] + d1z + d2z;
+        double a = a_snap;
+        if (ievol == 1) {
+            double rx = ex - obs[0], ry = ey - obs[1], rz = ez - obs[2];
+            double r = sqrt(rx * rx + ry * ry + rz * rz);
+            a = r > 0.0 ? 1.0 / (1.0 + hro_chi_to_z(t, r)) : 1.0;
+        }
+        double ia = 1.0 / a;
+        double E = sqrt(Om * (ia * ia * ia) + OL);
+        double f = hro_growth_f(t, a);
+        double vfac = a * 100.0 * E * f;
+        float keep[33];
+        for (int q = 0; q < nf; ++q) keep[q] = h[q];
+        o[0] = (float)ex;
+        o[1] = (float)ey;
+        o[2] = (float)ez;
+        o[3] = (float)(vfac * (d1x + 2.0 * d2x));
+        o[4] = (float)(vfac * (d1y + 2.0 * d2y));
+        o[5] = (float)(vfac * (d1z + 2.0 * d2z));
+        o[6] = keep[6];
+        o[7] = o[8] = o[9] = 0.0f;
+        for (int q = 10; q < nf; ++q) o[q] = keep[q];
+    }
 }

@@ -85,6 +85,11 @@ def lib():
                                        C.POINTER(C.c_uint8), _f64p]
         L.hro_merge_passes.restype = C.c_int32
         L.hro_sphere_overlap.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _f64p]
+        L.hro_growth_f.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
+        L.hro_growth_f.restype = C.c_double
+        L.hro_finalize_eulerian.argtypes = [_f32p, C.c_int64, C.c_int32,
+                                            C.POINTER(abi.CosmoTables), C.c_double, C.c_double,
+                                            _f64p, C.c_int32, C.c_double, _f32p]
         L.hro_growth_D.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
         L.hro_growth_D.restype = C.c_double
         L.hro_chi_to_z.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
@@ -299,18 +304,43 @@ def analyse_peaks(flds: dict, pi, pj, pk, Rf, ir2min, t: dict, ct: dict, *, alat
 # ---------------------------------------------------------------- halo records
 
 def cosmo_tables(growth, chi2z=None):
-    """abi.CosmoTables over host arrays (kept alive in the returned list)."""
+    """abi.CosmoTables over host arrays (kept alive in the returned list); growth = (atab,
+    D_atab[, dlnD_dlnatab])."""
     at = np.ascontiguousarray(growth[0], np.float64)
     Dt = np.ascontiguousarray(growth[1], np.float64)
     keep = [at, Dt]
     if chi2z is None:
-        return abi.CosmoTables(len(at), _p(at, C.c_double), _p(Dt, C.c_double), 0, None, None,
-                               0.0), keep
-    zs = np.ascontiguousarray(chi2z[0], np.float64)
-    ch = np.ascontiguousarray(chi2z[1], np.float64)
-    keep += [zs, ch]
-    return abi.CosmoTables(len(at), _p(at, C.c_double), _p(Dt, C.c_double), len(zs),
-                           _p(zs, C.c_double), _p(ch, C.c_double), float(chi2z[2])), keep
+        t = abi.CosmoTables(len(at), _p(at, C.c_double), _p(Dt, C.c_double), 0, None, None, 0.0)
+    else:
+        zs = np.ascontiguousarray(chi2z[0], np.float64)
+        ch = np.ascontiguousarray(chi2z[1], np.float64)
+        keep += [zs, ch]
+        t = abi.CosmoTables(len(at), _p(at, C.c_double), _p(Dt, C.c_double), len(zs),
+                            _p(zs, C.c_double), _p(ch, C.c_double), float(chi2z[2]))
+    if len(growth) > 2:
+        ft = np.ascontiguousarray(growth[2], np.float64)
+        keep.append(ft)
+        t.dlnD_atab = _p(ft, C.c_double)
+    return t, keep
+
+
+def growth_f(growth, a):
+    """Dlinear_ab(a)[2], the growth rate dlnD/dlna."""
+    t, keep = cosmo_tables(growth)
+    return lib().hro_growth_f(C.byref(t), float(a))
+
+
+def finalize_eulerian(records, growth, obs=(0.0, 0.0, 0.0), *, Om, OL, ievol=1, z_out=0.0,
+                      chi2z=None):
+    """Merger.finalize_eulerian on an (N, 11 or 33) float32 record array."""
+    rec = np.ascontiguousarray(records, np.float32)
+    n, nf = rec.shape
+    out = np.zeros_like(rec)
+    t, keep = cosmo_tables(growth, chi2z if ievol == 1 else None)
+    o = np.ascontiguousarray(obs, np.float64)
+    lib().hro_finalize_eulerian(_p(rec, C.c_float), n, nf, C.byref(t), float(Om), float(OL),
+                                _p(o, C.c_double), int(ievol), float(z_out), _p(out, C.c_float))
+    return out
 
 
 def growth_D(growth, a):
