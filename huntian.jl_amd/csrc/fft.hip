@@ -25,11 +25,13 @@ namespace {
 
 // Workgroup shapes hold at most 8 points per thread (<= 64 VGPRs: 8 waves per SIMD, some
 // workgroups loading or storing while others transform): 512 threads and 35 KB of LDS (four
-// workgroups per CU) for columns up to 1024 and for all rows; 1024 threads and 68 KB (four
-// 2048-point columns, two workgroups per CU) for 2048-point columns, where the small shape
-// would read 16-B pieces of each line.
+// workgroups per CU) for columns up to 512 and for all rows; 1024 threads and 68 KB (two
+// workgroups per CU) for 1024-point columns (eight per workgroup, 64-B pieces of each line
+// instead of the small shape's 32 B: 7 % faster per transform) and 2048-point columns (four).
+// Sixteen 1024-point columns (whole 128-B lines) need 16 points per thread and 139 KB of
+// LDS, i.e. 4 waves per SIMD, and measured slower (9 % at 1024, 10 % at 2048).
 constexpr int SMALL_T = 512, SMALL_LDS = 4368, SMALL_DATA = 4096;  // 4 padded columns of 1024
-constexpr int BIG_T = 1024, BIG_LDS = 8720, BIG_DATA = 8192;       // 4 padded columns of 2048
+constexpr int BIG_T = 1024, BIG_LDS = 8720, BIG_DATA = 8192;       // 8 of 1024, 4 of 2048
 
 
 int ilog2(int v) {
@@ -429,7 +431,7 @@ unsigned cols_grid(unsigned gx, int nb) {
     return (unsigned)(((t + 7) / 8) * 8);
 }
 
-bool cols_big(int L) { return L >= 2048; }
+bool cols_big(int L) { return L >= 1024; }
 
 int cols_lgC(int L) {
     // as many padded columns as the shape's LDS holds, at most 8 (64-B segments)
