@@ -75,8 +75,8 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): this build has no "
-              "multi-GPU path; the per-tile work would shard by tiles, see DESIGN.md",
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the library has "
+              "no multi-GPU path; the per-tile work would shard by tiles, one process per GPU",
               file=sys.stderr)
         sys.exit(2)
 
@@ -170,7 +170,7 @@ def main():
             results = api.alloc_results(int(npk * 1.25) + 1024, dev)
         Rf = Rf_tab[pf["filt"].long()]
         ir2 = ir2_tab[pf["filt"].long()]
-        mask.zero_()  # the driver clears the mask between peak finding and shells
+        mask.zero_()  # the reference tile driver clears the mask between peak finding and shells
         r = api.shell_analyse(fields, pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct, alatt=alatt,
                               ZZon=1.0, fcrit=fcrit, nbuff=nbuff, variant=args.variant,
                               early_exit=True, mask=mask, results=results, order=args.order,

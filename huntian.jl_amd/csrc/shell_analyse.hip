@@ -1769,7 +1769,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     HNT_REQUIRE(params->order != HNT_ORDER_LEARNED || tables->nshells > 12,
                 "hnt_shell_analyse: HNT_ORDER_LEARNED needs more than 12 shells");
     if (params->rmax2rs != 0.0) {
-        set_error("hnt_shell_analyse: rmax2rs > 0 is not supported by this build");
+        set_error("hnt_shell_analyse: rmax2rs > 0 is not supported by this library");
         return HNT_ERR_UNSUPPORTED;
     }
     HNT_REQUIRE(params->alatt > 0.0, "hnt_shell_analyse: alatt must be > 0");

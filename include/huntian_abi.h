@@ -37,7 +37,7 @@ typedef enum {
     HNT_ERR_INVALID = 1,     /* bad argument (shape, NULL pointer, out-of-range index) */
     HNT_ERR_DEVICE = 2,      /* a HIP runtime call failed */
     HNT_ERR_CAPACITY = 3,    /* output capacity too small; hnt_peaks.total says how much is needed */
-    HNT_ERR_UNSUPPORTED = 4, /* option not implemented by this build */
+    HNT_ERR_UNSUPPORTED = 4, /* option not implemented by this library */
     HNT_ERR_ALLOC = 5        /* device or host allocation failed */
 } hnt_status;
 
@@ -232,7 +232,7 @@ typedef struct {
 typedef struct {
     double alatt;              /* lattice spacing (Mpc/h) */
     int32_t nbuff;             /* mask is only written on (nbuff+1 : n-nbuff)^3 */
-    double rmax2rs;            /* only 0 is supported (the BASELINE setting) */
+    double rmax2rs;            /* only 0 is supported (the setting of the benchmark workloads) */
     int32_t variant;           /* HNT_VARIANT_A / _B / _C */
     int32_t early_exit;        /* 1: stop gathering once no later shell can change a result */
     int32_t order;             /* wavefront schedule of the peaks, HNT_ORDER_* (0: crossing pass);

@@ -1,7 +1,7 @@
 """GPU hot path vs the FP64 oracle on the same seeded inputs.
 
 Bars: peak lists (indices, filter, value) and masks equal; collapse flags and halo counts
-equal; float results within 1e-10 relative of the oracle.  The north star's tolerance is
+equal; float results within 1e-10 relative of the oracle.  The required tolerance is
 1e-5 for RTHL / overdensity and 1e-4 for strain, e_v, p_v; the two sides run the same FP64
 operations in the same order and differ only where the device and host log10 / pow differ
 by an ulp, which reaches ~1e-12 after the RTHL interpolation and the eigen-solve."""
