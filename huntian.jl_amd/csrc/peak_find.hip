@@ -69,7 +69,8 @@ __device__ __forceinline__ float from_lane_above(float v) {
 template <bool TILE>
 __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __restrict__ fields,
                                                const float* __restrict__ fcrit, int nfilt,
-                                               int fbase, int n, int nbuff,
+                                               int fbase, int n, int nbuff, int fside,
+                                               int fshift,
                                                const int8_t* __restrict__ mask, int from_claim,
                                                uint8_t* __restrict__ claim,
                                                float* __restrict__ cellval,
@@ -85,10 +86,14 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
     // all its cells are outside and so marked taken below.
     const int io = i0 - 1 + lane;              // inner-cube i of this lane's column, >= -1
     const bool out_i = lane >= 1 && lane <= OI && io < m;
+    // The fields are fside^3 arrays whose index 0 is tile index fshift along each axis (the
+    // whole tile: fside = n, fshift = 0; the inner cube and its one-cell halo: fside = m + 2,
+    // fshift = nbuff - 1).  The reads stay in tile indices nbuff - 1 .. nbuff + m.
+    const int fb = nbuff - fshift;
     // column offset inside a plane; io is clamped to m (a halo column: values unused)
-    const int col_bytes = (min(io, m) + nbuff) * 4;
-    const int64_t s2 = (int64_t)n * n;
-    const int row_bytes = 4 * n, plane_bytes = (int)(4 * s2);  // n <= 4096: below 2^31
+    const int col_bytes = (min(io, m) + fb) * 4;
+    const int64_t s2 = (int64_t)n * n, fs2 = (int64_t)fside * fside;
+    const int row_bytes = 4 * fside, plane_bytes = (int)(4 * fs2);  // fside <= 4096
     const int64_t mm = m, mm2 = (int64_t)m * m;
 
     // Cells already taken (masked, or a peak of an earlier filter): bit 8*(kk%4)+jr of
@@ -119,14 +124,14 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
         // tested), as buffer loads off a per-plane resource: the lane's column offset is the
         // one vector register, the row offset is scalar.
         auto load = [&](int kp, float (&v)[TJ + 2]) {
-            const float* P = F + (int64_t)(min(kp, m) + nbuff) * s2;
+            const float* P = F + (int64_t)(min(kp, m) + fb) * fs2;
             const __amdgpu_buffer_rsrc_t rs =
                 __builtin_amdgcn_make_buffer_rsrc((void*)P, 0, plane_bytes, BUF_DWORD3);
 #pragma unroll
             for (int r = 0; r < TJ + 2; ++r)
                 v[r] = __builtin_bit_cast(
                     float, __builtin_amdgcn_raw_buffer_load_b32(
-                               rs, col_bytes, (min(j0 - 1 + r, m) + nbuff) * row_bytes, 0));
+                               rs, col_bytes, (min(j0 - 1 + r, m) + fb) * row_bytes, 0));
         };
         auto reduce = [&](const float (&v)[TJ + 2], float (&mx)[TJ]) {
             float h[TJ + 2];
@@ -189,9 +194,9 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
                             if (TILE && from_claim && claim[ci] != NONE) continue;
                             claim[ci] = (uint8_t)(fbase + f);
                             if (TILE && cellval) {
-                                const int64_t a = (int64_t)(io + nbuff) +
-                                                  (int64_t)(j0 + r + nbuff) * n +
-                                                  (k0 + kk + nbuff) * s2;
+                                const int64_t a = (int64_t)(io + fb) +
+                                                  (int64_t)(j0 + r + fb) * fside +
+                                                  (k0 + kk + fb) * fs2;
                                 cellval[ci] = F[a];
                                 if (celld2) celld2[ci] = lapfields[f][a];
                             }
@@ -495,7 +500,7 @@ extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
     stage_begin(ctx, HNT_STAGE_PF_MARK);
     HNT_HIP(hipMemsetAsync(ctx->claim.p, NONE, (size_t)ncell, st));
     hipLaunchKernelGGL(pf_mark<false>, mark_grid(m), dim3(PF_THREADS), 0, st, dfields,
-                       (const float*)ctx->fcrit.p, nfilt, 0, n, nbuff, (const int8_t*)mask, 0,
+                       (const float*)ctx->fcrit.p, nfilt, 0, n, nbuff, n, 0, (const int8_t*)mask, 0,
                        (uint8_t*)ctx->claim.p, nullptr, nullptr, nullptr);
     stage_end(ctx, HNT_STAGE_PF_MARK);
     return compact_claims(ctx, "hnt_peak_find", n, nbuff, nfilt, tmp_bytes, dfields, nullptr,
@@ -590,12 +595,17 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
     const int64_t nwin = (int64_t)(n / 2 + 1) * (n / 2 + 1) * (n / 2 + 1);
     const unsigned wblocks = (unsigned)std::min<int64_t>((nwin + 255) / 256, 16384);
     for (int f = 0; f < nfilt; ++f) {
+        // The peak test reads the inner cube and its one-cell halo, (m + 2)^3 cells from tile
+        // index nbuff - 1: the library's inverse transform writes only those (the same values
+        // as in the whole field) unless the caller wants the whole smoothed field.
+        const bool whole = !own || (smoothed_out && smoothed_out[f]);
+        const int fshift = whole ? 0 : nbuff - 1, fside = whole ? n : m + 2;
         stage_begin(ctx, HNT_STAGE_PF_SMOOTH);
         hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
                            (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
                            bank->Rf[f], bank->wsmooth, inv_n3);
         if (own)
-            HNT_TRY(fft_inverse(ctx, work, work, n, 0, n, none, 1.0f, 0, 0.0f, field));
+            HNT_TRY(fft_inverse(ctx, work, work, n, fshift, fside, none, 1.0f, 0, 0.0f, field));
         else
             HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)field));
         if (smoothed_out && smoothed_out[f])
@@ -606,14 +616,15 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
                                (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
                                bank->Rf[f], 3, inv_n3);
             if (own)
-                HNT_TRY(fft_inverse(ctx, work, work, n, 0, n, none, 1.0f, 0, 0.0f, lapf));
+                HNT_TRY(fft_inverse(ctx, work, work, n, fshift, fside, none, 1.0f, 0, 0.0f,
+                                    lapf));
             else
                 HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)lapf));
         }
         stage_end(ctx, HNT_STAGE_PF_SMOOTH);
         stage_begin(ctx, HNT_STAGE_PF_MARK);
         hipLaunchKernelGGL(pf_mark<true>, mark_grid(m), dim3(PF_THREADS), 0, st, dptrs,
-                           (const float*)ctx->fcrit.p + f, 1, f, n, nbuff,
+                           (const float*)ctx->fcrit.p + f, 1, f, n, nbuff, fside, fshift,
                            (const int8_t*)mask, 1, (uint8_t*)ctx->claim.p,
                            (float*)ctx->fb_val.p, lap ? dptrs + 1 : nullptr,
                            lap ? (float*)ctx->fb_d2.p : nullptr);

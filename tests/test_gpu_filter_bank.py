@@ -124,6 +124,32 @@ def test_tile_peaks_equal_oracle_on_its_fields(oracle, tile, nbuff, seed):
     assert np.array_equal(md3.cpu().numpy(), mref3)
 
 
+@pytest.mark.parametrize("nbuff", [1, 5, 16])
+def test_inner_cube_transforms_give_the_same_peaks(tile, nbuff):
+    """Without smoothed_out the inverse transforms write only the inner cube and its one-cell
+    halo; peaks, values, d2Rf and the mask equal those of the call that writes whole fields."""
+    import torch
+
+    delta, alatt = tile
+    n = delta.shape[0]
+    radii = [12.0, 7.0, 4.0, 2.5]
+    fcrits = [0.6 * float(np.sqrt(np.mean(smooth_ref(delta, alatt, R, 1) ** 2))) for R in radii]
+    mask0 = np.zeros((n, n, n), np.int8, order="F")
+    mask0[np.random.default_rng(nbuff).random((n, n, n)) < 0.02] = 1
+    res = []
+    for whole in (True, False):
+        md = _dev(mask0, torch.int8)
+        r = api.peak_find_tile(_dev(delta), radii, fcrits, nbuff, md, alatt=alatt,
+                               with_lapd=True, capacity=n ** 3,
+                               smoothed_out=_outs(n, len(radii)) if whole else None)
+        res.append((r, md))
+    (a, ma), (b, mb) = res
+    assert a["total"] == b["total"] > 50
+    for k in ("i", "j", "k", "filt", "delta", "d2Rf"):
+        assert torch.equal(a[k], b[k]), k
+    assert torch.equal(ma, mb)
+
+
 def test_d2Rf_is_the_k2_gaussian_at_each_peak(tile):
     """With the Laplacian output (ioutshear >= 1), d2Rf of a peak of filter f is the
     window-3 smoothing at radius Rf[f] sampled at the peak cell."""
