@@ -297,32 +297,70 @@ __device__ __forceinline__ void stencil(int g, int block, int M, float w[4], int
     for (int q = 0; q < 4; ++q) idx[q] = mod1(i - 1 + q, M) - 1;
 }
 
+// The stencils of the tile's cells along each axis (axis a, 0-based cell l: global cell
+// mod1(start_a + l, N)), computed once per call: Catmull-Rom weights w[a * nmesh + l] and
+// coarse indices idx[a * nmesh + l].  Every row of the tile shares the x stencils, every
+// plane the y ones.
+__global__ void fs_tricubic_axes(int M, int N, int block, int i0, int j0, int k0, int nmesh,
+                                 float4* __restrict__ w, int4* __restrict__ idx) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= 3 * nmesh) return;
+    const int axis = t / nmesh, l = t - axis * nmesh;
+    const int s = axis == 0 ? i0 : (axis == 1 ? j0 : k0);
+    float ww[4];
+    int ii[4];
+    stencil(mod1(s + l, N), block, M, ww, ii);
+    w[t] = make_float4(ww[0], ww[1], ww[2], ww[3]);
+    idx[t] = make_int4(ii[0], ii[1], ii[2], ii[3]);
+}
+
 // _tricubic_kernel! (ext/CUDAExt.jl:3083-3140): one wavefront per tile row (lj, lk), whose y
-// and z weights and coarse rows are the same for every lane; lanes over li.  The sum runs
-// over (z, y, x) with ((wx wy) wz) f as the reference does.
-__global__ void fs_tricubic(const float* __restrict__ coarse, int M, int N, int block, int i0,
-                            int j0, int k0, int nmesh, float* __restrict__ out) {
+// and z weights and coarse rows are the same for every lane.  Each lane takes a run of
+// consecutive cells of the row: cells of one coarse cell (N / M of them) share the x indices,
+// so the lane keeps the 4 x 4 x 4 coarse values in registers and reloads them only when the
+// x indices change.  The sum runs over (z, y, x) with ((wx wy) wz) f as the reference does.
+__global__ __launch_bounds__(FT) void fs_tricubic(const float* __restrict__ coarse, int M,
+                                                  int nmesh, const float4* __restrict__ sw,
+                                                  const int4* __restrict__ si,
+                                                  float* __restrict__ out) {
     const int lane = threadIdx.x & 63;
+    const int run = (nmesh + 63) >> 6;
+    const int l0 = min(lane * run, nmesh), l1 = min(l0 + run, nmesh);
     FOR_ROWS(nmesh * nmesh) {
-        const int lj = row % nmesh + 1, lk = row / nmesh + 1;
-        float wy[4], wz[4];
-        int ys[4], zs[4];
-        stencil(mod1(j0 + lj - 1, N), block, M, wy, ys);
-        stencil(mod1(k0 + lk - 1, N), block, M, wz, zs);
+        const int lj = row % nmesh, lk = row / nmesh;
+        const float4 wy4 = sw[nmesh + lj], wz4 = sw[2 * nmesh + lk];
+        const int4 ys4 = si[nmesh + lj], zs4 = si[2 * nmesh + lk];
+        const float wy[4] = {wy4.x, wy4.y, wy4.z, wy4.w}, wz[4] = {wz4.x, wz4.y, wz4.z, wz4.w};
+        const int ys[4] = {ys4.x, ys4.y, ys4.z, ys4.w}, zs[4] = {zs4.x, zs4.y, zs4.z, zs4.w};
         float* o = out + (int64_t)row * nmesh;
-        for (int li = lane + 1; li <= nmesh; li += 64) {
-            float wx[4];
-            int xs[4];
-            stencil(mod1(i0 + li - 1, N), block, M, wx, xs);
-            float val = 0.0f;
-            for (int c = 0; c < 4; ++c) {
-                const float* cz = coarse + (int64_t)M * M * zs[c];
-                for (int b = 0; b < 4; ++b) {
-                    const float* cy = cz + (int64_t)M * ys[b];
-                    for (int a = 0; a < 4; ++a) val += wx[a] * wy[b] * wz[c] * cy[xs[a]];
+        float cv[4][4][4];
+        int4 have = make_int4(-1, -1, -1, -1);
+        for (int li = l0; li < l1; ++li) {
+            const float4 wx4 = sw[li];
+            const int4 xs4 = si[li];
+            if (xs4.x != have.x || xs4.y != have.y || xs4.z != have.z || xs4.w != have.w) {
+                const int xs[4] = {xs4.x, xs4.y, xs4.z, xs4.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float* cz = coarse + (int64_t)M * M * zs[c];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const float* cy = cz + (int64_t)M * ys[b];
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) cv[c][b][a] = cy[xs[a]];
+                    }
                 }
+                have = xs4;
             }
-            o[li - 1] = val;
+            const float wx[4] = {wx4.x, wx4.y, wx4.z, wx4.w};
+            float val = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) val += wx[a] * wy[b] * wz[c] * cv[c][b][a];
+            o[li] = val;
         }
     }
 }
@@ -661,7 +699,13 @@ extern "C" hnt_status hnt_interpolate_to_tile(hnt_ctx* ctx, const float* coarse,
     const int64_t k0 = (int64_t)(kt - 1) * nsub + 1 - nbuff;
     HNT_REQUIRE(std::max(i0, std::max(j0, k0)) + nmesh < (1LL << 31),
                 "hnt_interpolate_to_tile: tile start beyond Int32");
-    hipLaunchKernelGGL(fs_tricubic, dim3(grid_rows((int64_t)nmesh * nmesh)), dim3(FT), 0, ctx->stream, coarse,
-                       M, N, block, (int)i0, (int)j0, (int)k0, nmesh, tile);
+    // the three axes' stencils: 3 nmesh x (float4 weights + int4 indices)
+    HNT_TRY(reserve(ctx, ctx->fs_tab, (size_t)3 * nmesh * 32));
+    float4* sw = (float4*)ctx->fs_tab.p;
+    int4* si = (int4*)(sw + 3 * (size_t)nmesh);
+    hipLaunchKernelGGL(fs_tricubic_axes, dim3((unsigned)((3 * nmesh + 255) / 256)), dim3(256), 0,
+                       ctx->stream, M, N, block, (int)i0, (int)j0, (int)k0, nmesh, sw, si);
+    hipLaunchKernelGGL(fs_tricubic, dim3(grid_rows((int64_t)nmesh * nmesh)), dim3(FT), 0,
+                       ctx->stream, coarse, M, nmesh, (const float4*)sw, (const int4*)si, tile);
     return finish_call(ctx, "hnt_interpolate_to_tile");
 }
