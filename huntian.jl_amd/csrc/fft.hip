@@ -202,6 +202,13 @@ __device__ __forceinline__ double pick(int d, double kx, double ky, double kz) {
 // kernels compute it (_transfer_kernel!, _periodic_kernel!, _short_pot_kernel).
 __device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, int iy, int iz) {
     const int L = op.L, half = L / 2;
+    if (op.kind == FFT_OP_WINDOW) {
+        const int ay = iy <= half ? iy : L - iy, az = iz <= half ? iz : L - iz;
+        const float w = op.amp[ix + (int64_t)(half + 1) * (ay + (int64_t)(half + 1) * az)];
+        v.x *= w;
+        v.y *= w;
+        return make_float2(v.x * op.post, v.y * op.post);
+    }
     if (op.kind == FFT_OP_TRANSFER) {
         const double kx = (double)ix * op.dk;
         const double ky = (double)(iy < half ? iy : iy - L) * op.dk;

@@ -101,7 +101,7 @@ struct hnt_ctx {
     hnt::MlpScratch mlp;
     hnt::DevBuf tiles, tilemeta, slots, gtiles;  // multi-tile shell analysis
     // tile peak finding: spectrum, windowed spectrum, smoothed fields, per-cell peak values
-    hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2;
+    hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2, fb_wtab;
     hnt::DevBuf rec_tabs, rec_keep, rec_slot;  // halo records: tables, flags, slots
     hnt::MergeBufs mg;                         // catalog merge
     // tile field synthesis: hipFFT plan pairs by size {n, r2c, c2r}, real and two complex
@@ -142,14 +142,21 @@ void release_field_plans(hnt_ctx* ctx);  // FFT plans of the field-synthesis ent
 // Power-of-two 3-D real FFTs (fft.hip).  Operators applied to the spectrum as the inverse
 // transform first reads it (the reference's _transfer_kernel!, _periodic_kernel!,
 // _short_pot_kernel).
-enum { FFT_OP_NONE = 0, FFT_OP_TRANSFER = 1, FFT_OP_PERIODIC = 2, FFT_OP_SHORTPOT = 3 };
+enum {
+    FFT_OP_NONE = 0,
+    FFT_OP_TRANSFER = 1,
+    FFT_OP_PERIODIC = 2,
+    FFT_OP_SHORTPOT = 3,
+    FFT_OP_WINDOW = 4  // v * amp[(ix, |iy|, |iz|)], then * post (the filter bank's window)
+};
 struct FftOp {
     int kind = FFT_OP_NONE;
     int L = 0;                 // transform length
     double dk = 0.0;
-    const float* amp = nullptr;  // sqrt P per (ix, |iy|, |iz|), (L/2 + 1)^3, for TRANSFER
+    const float* amp = nullptr;  // per (ix, |iy|, |iz|), (L/2 + 1)^3: sqrt P, or the window
     int id = 0, d1 = 0, d2 = 0;
     double rs2 = 0.0;
+    float post = 1.0f;
 };
 bool fft_supported(int L);
 hnt_status fft_twiddles(hnt_ctx* ctx, int L, const float2** out);

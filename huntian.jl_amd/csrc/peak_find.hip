@@ -321,6 +321,42 @@ __global__ __launch_bounds__(256) void pf_claim_init(const int8_t* __restrict__ 
 // (kx, |ky|, |kz|) and applies it to the up to four elements with those magnitudes.  The
 // inverse transform's 1/n^3 is applied here too (for a power-of-two n it is exact, so it
 // equals scaling afterwards).
+// The Float32 window of one filter at k^2 > 0 (FP64, then rounded).
+__device__ __forceinline__ float window32(double k2, double Rf, int wsmooth) {
+    const double k = sqrt(k2), kR = k * Rf;
+    double w;
+    if (wsmooth == 0) {
+        w = exp(-kR * kR * 0.125);
+    } else if (wsmooth == 3) {
+        w = exp(-kR * kR * 0.125) * k2;
+    } else {
+        double sn, cs;
+        sincos(kR, &sn, &cs);
+        w = 3.0 * (sn - kR * cs) / (kR * kR * kR);
+    }
+    return (float)w;
+}
+
+// The window of one filter as a table over (kx, |ky|, |kz|): w[ix + nk (ay + nk az)],
+// nk = n/2 + 1, with k from those magnitudes as fb_window computes it, and 1 at k = 0 (where
+// fb_window leaves the spectrum as it is: v * 1 = v).  The library's inverse transform reads
+// it as it first reads the spectrum (FFT_OP_WINDOW), so the windowed spectrum is never
+// written: 0.54 GB of table instead of 8.6 GB of spectrum traffic per filter at 1024^3.
+__global__ __launch_bounds__(256) void fb_wtable(int n, double dk, double Rf, int wsmooth,
+                                                 float* __restrict__ w) {
+    const int nk = n / 2 + 1;
+    const int64_t total = (int64_t)nk * nk * nk;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(e % nk);
+        const int64_t rem = e / nk;
+        const int ay = (int)(rem % nk), az = (int)(rem / nk);
+        const double kx = (double)ix * dk, ky = (double)ay * dk, kz = (double)az * dk;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        w[e] = k2 != 0.0 ? window32(k2, Rf, wsmooth) : 1.0f;
+    }
+}
+
 __global__ __launch_bounds__(256) void fb_window(const float2* __restrict__ spec,
                                                  float2* __restrict__ out, int n, double dk,
                                                  double Rf, int wsmooth, float inv_n3) {
@@ -336,18 +372,7 @@ __global__ __launch_bounds__(256) void fb_window(const float2* __restrict__ spec
         float w32 = 1.0f;
         bool apply = false;
         if (k2 != 0.0) {
-            const double k = sqrt(k2), kR = k * Rf;
-            double w;
-            if (wsmooth == 0) {
-                w = exp(-kR * kR * 0.125);
-            } else if (wsmooth == 3) {
-                w = exp(-kR * kR * 0.125) * k2;
-            } else {
-                double sn, cs;
-                sincos(kR, &sn, &cs);
-                w = 3.0 * (sn - kR * cs) / (kR * kR * kR);
-            }
-            w32 = (float)w;
+            w32 = window32(k2, Rf, wsmooth);
             apply = true;
         }
         // rows iy = ay and n - ay (one row when ay is 0, or n/2 for an even n), likewise iz
@@ -546,12 +571,14 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         HNT_TRY(reserve(ctx, ctx->fb_lap, sizeof(float) * (size_t)nreal));
         HNT_TRY(reserve(ctx, ctx->fb_d2, sizeof(float) * (size_t)ncell));
     }
+    const int64_t nwtab = (int64_t)(n / 2 + 1) * (n / 2 + 1) * (n / 2 + 1);
     // transforms: the library's own for power-of-two n (tuning fft = 0), else hipFFT, whose
     // 32-bit 3-D plans are only valid below 2^31 elements
     const bool own = ctx->fft_mode == 0 && fft_supported(n);
     HNT_REQUIRE(own || nreal < (1LL << 31),
                 "hnt_peak_find_tile: a %d^3 tile needs the library FFT (power-of-two n, tuning "
                 "fft = 0)", n);
+    if (own) HNT_TRY(reserve(ctx, ctx->fb_wtab, sizeof(float) * (size_t)nwtab));
     hipfftHandle r2c = nullptr, c2r = nullptr;
     if (!own) {
         if (ctx->fft_n != n) {
@@ -567,7 +594,6 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         HNT_FFT(hipfftSetStream(r2c, st));
         HNT_FFT(hipfftSetStream(c2r, st));
     }
-    const FftOp none;
     float2* spec = (float2*)ctx->fb_spec.p;
     float2* work = (float2*)ctx->fb_work.p;
     float* field = (float*)ctx->fb_field.p;
@@ -594,32 +620,47 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
     const float inv_n3 = (float)(1.0 / (double)nreal);
     const int64_t nwin = (int64_t)(n / 2 + 1) * (n / 2 + 1) * (n / 2 + 1);
     const unsigned wblocks = (unsigned)std::min<int64_t>((nwin + 255) / 256, 16384);
+    const unsigned tblocks = (unsigned)std::min<int64_t>((nwtab + 255) / 256, 16384);
     for (int f = 0; f < nfilt; ++f) {
         // The peak test reads the inner cube and its one-cell halo, (m + 2)^3 cells from tile
         // index nbuff - 1: the library's inverse transform writes only those (the same values
         // as in the whole field) unless the caller wants the whole smoothed field.
         const bool whole = !own || (smoothed_out && smoothed_out[f]);
         const int fshift = whole ? 0 : nbuff - 1, fside = whole ? n : m + 2;
+        // library path: the window table of this filter, then the inverse transform of the
+        // spectrum times the table (and 1/n^3) into `dst`
+        auto window_inverse = [&](int wsmooth, float* dst) -> hnt_status {
+            float* wtab = (float*)ctx->fb_wtab.p;
+            hipLaunchKernelGGL(fb_wtable, dim3(tblocks), dim3(256), 0, st, n, dk, bank->Rf[f],
+                               wsmooth, wtab);
+            FftOp wop;
+            wop.kind = FFT_OP_WINDOW;
+            wop.L = n;
+            wop.amp = wtab;
+            wop.post = inv_n3;
+            return fft_inverse(ctx, spec, work, n, fshift, fside, wop, 1.0f, 0, 0.0f, dst);
+        };
         stage_begin(ctx, HNT_STAGE_PF_SMOOTH);
-        hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
-                           (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
-                           bank->Rf[f], bank->wsmooth, inv_n3);
-        if (own)
-            HNT_TRY(fft_inverse(ctx, work, work, n, fshift, fside, none, 1.0f, 0, 0.0f, field));
-        else
+        if (own) {
+            HNT_TRY(window_inverse(bank->wsmooth, field));
+        } else {
+            hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
+                               (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
+                               bank->Rf[f], bank->wsmooth, inv_n3);
             HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)field));
+        }
         if (smoothed_out && smoothed_out[f])
             HNT_HIP(hipMemcpyAsync(smoothed_out[f], field, sizeof(float) * (size_t)nreal,
                                    hipMemcpyDeviceToDevice, st));
         if (lap) {
-            hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
-                               (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n, dk,
-                               bank->Rf[f], 3, inv_n3);
-            if (own)
-                HNT_TRY(fft_inverse(ctx, work, work, n, fshift, fside, none, 1.0f, 0, 0.0f,
-                                    lapf));
-            else
+            if (own) {
+                HNT_TRY(window_inverse(3, lapf));
+            } else {
+                hipLaunchKernelGGL(fb_window, dim3(wblocks), dim3(256), 0, st,
+                                   (const float2*)ctx->fb_spec.p, (float2*)ctx->fb_work.p, n,
+                                   dk, bank->Rf[f], 3, inv_n3);
                 HNT_FFT(hipfftExecC2R(c2r, (hipfftComplex*)work, (hipfftReal*)lapf));
+            }
         }
         stage_end(ctx, HNT_STAGE_PF_SMOOTH);
         stage_begin(ctx, HNT_STAGE_PF_MARK);
