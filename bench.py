@@ -305,9 +305,10 @@ def main():
         tile_pf = dict(ms=round(e0.elapsed_time(e1), 3), candidates=len(ft["i"]),
                        stage_ms={k: round(v[0], 3) for k, v in fprof.items() if v[1]},
                        filters=len(radii),
-                       note="one rfft, then per filter window x spectrum, irfft (library FFT, "
-                            "Float32) and the peak test; candidates differ from the step's "
-                            "(torch-smoothed input) only by FFT rounding")
+                       note="one rfft, then per filter a window table and an irfft of the "
+                            "spectrum times the table to the inner cube and its halo (library "
+                            "FFT, Float32) and the peak test; candidates differ from the "
+                            "step's (torch-smoothed input) only by FFT rounding")
         del ft
         log(f"tile peak finding: {tile_pf}")
 
