@@ -334,7 +334,7 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_r2c(
         // (zk - conj zm) / 2i
         const float2 od = make_float2((zk.y + zm.y) * 0.5f, -(zk.x - zm.x) * 0.5f);
         const float2 t = ctw<false>(od, WL[k]);
-        spec[k + (int64_t)nk * (iy + (int64_t)L * iz)] = cadd(ev, t);
+        spec[k + (int64_t)fft_pitch(L) * (iy + (int64_t)L * iz)] = cadd(ev, t);
     }
 }
 
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r(
         float2 v = make_float2(0.0f, 0.0f);
         if (row < nrows) {
             const int iy = s + row % n, iz = s + row / n;
-            v = spec[k + (int64_t)nk * (iy + (int64_t)L * iz)];
+            v = spec[k + (int64_t)fft_pitch(L) * (iy + (int64_t)L * iz)];
         }
         buf[g * LS + pad(k)] = v;
     }
@@ -510,10 +510,11 @@ hnt_status fft_forward(hnt_ctx* ctx, const float* f, int w, int L, float2* spec)
     const int lgC = cols_lgC(L);
     const unsigned gx = (unsigned)((nk + (1 << lgC) - 1) >> lgC);
     // y: planes iz < w, rows iy < w are filled
-    launch_cols<false>(st, L, gx, w, spec, spec, (int64_t)nk, (int64_t)nk * L, nk, 0, 0, w, 0,
+    const int64_t P = fft_pitch(L);
+    launch_cols<false>(st, L, gx, w, spec, spec, P, P * L, nk, 0, 0, w, 0,
                        L, none, WL);
     // z: every (ix, iy) column, planes iz < w are filled
-    launch_cols<false>(st, L, gx, L, spec, spec, (int64_t)nk * L, (int64_t)nk, nk, 0, 0, w, 0,
+    launch_cols<false>(st, L, gx, L, spec, spec, P * L, P, nk, 0, 0, w, 0,
                        L, none, WL);
     return HNT_OK;
 }
@@ -529,10 +530,11 @@ hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, in
     const unsigned gx = (unsigned)((nk + (1 << lgC) - 1) >> lgC);
     FftOp none;
     // z: every (ix, iy) column, the operator applied as read; planes iz in [s, s + n) kept
-    launch_cols<true>(st, L, gx, L, spec, work, (int64_t)nk * L, (int64_t)nk, nk, 0, 0, L, s,
+    const int64_t P = fft_pitch(L);
+    launch_cols<true>(st, L, gx, L, spec, work, P * L, P, nk, 0, 0, L, s,
                       s + n, op, WL);
     // y: planes iz in [s, s + n), rows iy in [s, s + n) kept
-    launch_cols<true>(st, L, gx, n, work, work, (int64_t)nk, (int64_t)nk * L, nk, s, 0, L, s,
+    launch_cols<true>(st, L, gx, n, work, work, P, P * L, nk, s, 0, L, s,
                       s + n, none, WL);
     const int lgG = rows_lgG(M);
     const int64_t rows = (int64_t)n * n;

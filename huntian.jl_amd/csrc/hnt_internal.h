@@ -159,6 +159,10 @@ struct FftOp {
     float post = 1.0f;
 };
 bool fft_supported(int L);
+// Row pitch (complex elements) of the library's half spectra: L/2 + 1 rounded up to 16, so
+// that every row starts on a 128-B line and the column passes' 64-B pieces pair up in lines.
+// A spectrum holds fft_pitch(L) x L x L elements; columns from L/2 + 1 on are never used.
+__host__ __device__ constexpr int fft_pitch(int L) { return ((L / 2 + 1) + 15) & ~15; }
 hnt_status fft_twiddles(hnt_ctx* ctx, int L, const float2** out);
 // spec ((L/2+1) x L x L) = R2C of the w^3 field f zero-padded to L^3
 hnt_status fft_forward(hnt_ctx* ctx, const float* f, int w, int L, float2* spec);

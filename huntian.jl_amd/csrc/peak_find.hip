@@ -556,7 +556,8 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
     if (m <= 0) return HNT_OK;
     hipStream_t st = ctx->stream;
     const int64_t ncell = (int64_t)m * m * m;
-    const int64_t nreal = (int64_t)n * n * n, ncplx = (int64_t)(n / 2 + 1) * n * n;
+    // spectrum buffers: the library's padded rows (hipFFT's n/2 + 1 fit in them)
+    const int64_t nreal = (int64_t)n * n * n, ncplx = (int64_t)fft_pitch(n) * n * n;
     const bool lap = d2Rf != nullptr;
 
     size_t tmp_bytes = 0;
