@@ -285,14 +285,23 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols(
     const int x0 = (tile % gx) * C;
     const int b = b0 + tile / gx;
     const int64_t boff = (int64_t)b * strideB;
-    for (int e = threadIdx.x; e < C * L; e += NT) {
+    // two elements per step, both loads issued before either is used
+    for (int e = threadIdx.x; e < C * L; e += 2 * NT) {
+        const int e1 = e + NT;
         const int c = e & (C - 1), a = e >> lgC, x = x0 + c;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (x < nx && a >= r0 && a < r1) {
-            v = in[x + a * strideA + boff];
-            if (op.kind != FFT_OP_NONE) v = apply_op(op, v, x, b, a);
+        const int c1 = e1 & (C - 1), a1 = e1 >> lgC, x1 = x0 + c1;
+        const bool has1 = e1 < C * L;
+        const bool in0 = x < nx && a >= r0 && a < r1;
+        const bool in1 = has1 && x1 < nx && a1 >= r0 && a1 < r1;
+        float2 v = make_float2(0.0f, 0.0f), v1 = make_float2(0.0f, 0.0f);
+        if (in0) v = in[x + a * strideA + boff];
+        if (in1) v1 = in[x1 + a1 * strideA + boff];
+        if (op.kind != FFT_OP_NONE) {
+            if (in0) v = apply_op(op, v, x, b, a);
+            if (in1) v1 = apply_op(op, v1, x1, b, a1);
         }
         buf[c * LS + pad(a)] = v;
+        if (has1) buf[c1 * LS + pad(a1)] = v1;
     }
     __syncthreads();
     stockham<INV, NT, DATA>(buf, LS, C, lgL, W);
@@ -312,15 +321,25 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_r2c(
     const int M = 1 << lgM, G = 1 << lgG, LS = padded_len(M), L = 2 * M, nk = M + 1;
     const int row0 = (int)blockIdx.x * G;
     const int nrows = w * w;
-    for (int e = threadIdx.x; e < G * M; e += SMALL_T) {
-        const int g = e >> lgM, m = e & (M - 1), row = row0 + g;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (row < nrows) {
-            const float* src = f + (int64_t)row * w;
-            const int x = 2 * m;
-            v = make_float2(x < w ? src[x] : 0.0f, x + 1 < w ? src[x + 1] : 0.0f);
+    // two elements per step, both loads issued before either is used
+    for (int e0 = threadIdx.x; e0 < G * M; e0 += 2 * SMALL_T) {
+        float2 v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * SMALL_T;
+            const int g = e >> lgM, m = e & (M - 1), row = row0 + g;
+            v[u] = make_float2(0.0f, 0.0f);
+            if (e < G * M && row < nrows) {
+                const float* src = f + (int64_t)row * w;
+                const int x = 2 * m;
+                v[u] = make_float2(x < w ? src[x] : 0.0f, x + 1 < w ? src[x + 1] : 0.0f);
+            }
         }
-        buf[g * LS + pad(m)] = v;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * SMALL_T;
+            if (e < G * M) buf[(e >> lgM) * LS + pad(e & (M - 1))] = v[u];
+        }
     }
     __syncthreads();
     stockham<false, SMALL_T, SMALL_DATA>(buf, LS, G, lgM, WM);
@@ -351,14 +370,27 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r(
     const int M = 1 << lgM, G = 1 << lgG, LS = padded_len(M), L = 2 * M, nk = M + 1;
     const int row0 = (int)blockIdx.x * G;
     const int nrows = n * n;
-    for (int e = threadIdx.x; e < G * nk; e += SMALL_T) {
-        const int g = e / nk, k = e - g * nk, row = row0 + g;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (row < nrows) {
-            const int iy = s + row % n, iz = s + row / n;
-            v = spec[k + (int64_t)fft_pitch(L) * (iy + (int64_t)L * iz)];
+    // two elements per step, both loads issued before either is used
+    for (int e0 = threadIdx.x; e0 < G * nk; e0 += 2 * SMALL_T) {
+        float2 v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * SMALL_T;
+            const int g = e / nk, k = e - g * nk, row = row0 + g;
+            v[u] = make_float2(0.0f, 0.0f);
+            if (e < G * nk && row < nrows) {
+                const int iy = s + row % n, iz = s + row / n;
+                v[u] = spec[k + (int64_t)fft_pitch(L) * (iy + (int64_t)L * iz)];
+            }
         }
-        buf[g * LS + pad(k)] = v;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * SMALL_T;
+            if (e < G * nk) {
+                const int g = e / nk;
+                buf[g * LS + pad(e - g * nk)] = v[u];
+            }
+        }
     }
     __syncthreads();
     // untwist into Z (registers first: Z[k] needs X[M - k] of the same row)
