@@ -296,7 +296,26 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols(
         float2 v = make_float2(0.0f, 0.0f), v1 = make_float2(0.0f, 0.0f);
         if (in0) v = in[x + a * strideA + boff];
         if (in1) v1 = in[x1 + a1 * strideA + boff];
-        if (op.kind != FFT_OP_NONE) {
+        if (op.kind == FFT_OP_WINDOW) {
+            // the window table's entries loaded with the spectrum, then apply_op's products
+            // index ix + (half + 1) (ay + (half + 1) az), (ix, iy, iz) = (x, b, a), as apply_op
+            const int half = L / 2, ay = b <= half ? b : L - b;
+            const int64_t h1 = half + 1, rowb = h1 * ay;
+            const int az = a <= half ? a : L - a, az1 = a1 <= half ? a1 : L - a1;
+            float w = 1.0f, w1 = 1.0f;
+            if (in0) w = op.amp[x + rowb + h1 * h1 * az];
+            if (in1) w1 = op.amp[x1 + rowb + h1 * h1 * az1];
+            if (in0) {
+                v.x *= w;
+                v.y *= w;
+                v = make_float2(v.x * op.post, v.y * op.post);
+            }
+            if (in1) {
+                v1.x *= w1;
+                v1.y *= w1;
+                v1 = make_float2(v1.x * op.post, v1.y * op.post);
+            }
+        } else if (op.kind != FFT_OP_NONE) {
             if (in0) v = apply_op(op, v, x, b, a);
             if (in1) v1 = apply_op(op, v1, x1, b, a1);
         }
