@@ -200,11 +200,20 @@ __device__ __forceinline__ double pick(int d, double kx, double ky, double kz) {
 
 // Operator on element (ix, iy, iz) of the half spectrum of an L^3 field, as the reference
 // kernels compute it (_transfer_kernel!, _periodic_kernel!, _short_pot_kernel).
-__device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, int iy, int iz) {
+// Entry of element (ix, iy, iz) in a (L/2 + 1)^3 table over (ix, |iy|, |iz|).
+__device__ __forceinline__ int64_t table_index(int L, int ix, int iy, int iz) {
+    const int half = L / 2;
+    const int ay = iy <= half ? iy : L - iy, az = iz <= half ? iz : L - iz;
+    return ix + (int64_t)(half + 1) * (ay + (int64_t)(half + 1) * az);
+}
+
+// `tab` is op.amp[table_index(...)] for WINDOW and TRANSFER (loaded by the caller together
+// with the spectrum element), unused otherwise.
+__device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, int iy, int iz,
+                                           float tab) {
     const int L = op.L, half = L / 2;
     if (op.kind == FFT_OP_WINDOW) {
-        const int ay = iy <= half ? iy : L - iy, az = iz <= half ? iz : L - iz;
-        const float w = op.amp[ix + (int64_t)(half + 1) * (ay + (int64_t)(half + 1) * az)];
+        const float w = tab;
         v.x *= w;
         v.y *= w;
         return make_float2(v.x * op.post, v.y * op.post);
@@ -215,9 +224,7 @@ __device__ __forceinline__ float2 apply_op(const FftOp& op, float2 v, int ix, in
         const double kz = (double)(iz < half ? iz : iz - L) * op.dk;
         const double k2 = kx * kx + ky * ky + kz * kz;
         if (k2 == 0.0) return make_float2(0.0f, 0.0f);
-        const int ay = iy <= half ? iy : L - iy, az = iz <= half ? iz : L - iz;
-        const double sqrt_pk =
-            (double)op.amp[ix + (int64_t)(half + 1) * (ay + (int64_t)(half + 1) * az)];
+        const double sqrt_pk = (double)tab;
         if (op.id == 0)
             return make_float2((float)((double)v.x * sqrt_pk), (float)((double)v.y * sqrt_pk));
         if (op.id == 1 || op.id == 2) {
@@ -296,28 +303,15 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols(
         float2 v = make_float2(0.0f, 0.0f), v1 = make_float2(0.0f, 0.0f);
         if (in0) v = in[x + a * strideA + boff];
         if (in1) v1 = in[x1 + a1 * strideA + boff];
-        if (op.kind == FFT_OP_WINDOW) {
-            // the window table's entries loaded with the spectrum, then apply_op's products
-            // index ix + (half + 1) (ay + (half + 1) az), (ix, iy, iz) = (x, b, a), as apply_op
-            const int half = L / 2, ay = b <= half ? b : L - b;
-            const int64_t h1 = half + 1, rowb = h1 * ay;
-            const int az = a <= half ? a : L - a, az1 = a1 <= half ? a1 : L - a1;
-            float w = 1.0f, w1 = 1.0f;
-            if (in0) w = op.amp[x + rowb + h1 * h1 * az];
-            if (in1) w1 = op.amp[x1 + rowb + h1 * h1 * az1];
-            if (in0) {
-                v.x *= w;
-                v.y *= w;
-                v = make_float2(v.x * op.post, v.y * op.post);
-            }
-            if (in1) {
-                v1.x *= w1;
-                v1.y *= w1;
-                v1 = make_float2(v1.x * op.post, v1.y * op.post);
-            }
-        } else if (op.kind != FFT_OP_NONE) {
-            if (in0) v = apply_op(op, v, x, b, a);
-            if (in1) v1 = apply_op(op, v1, x1, b, a1);
+        // the operator's table entries (window, sqrt P) loaded together with the spectrum
+        float t = 0.0f, t1 = 0.0f;
+        if (op.kind == FFT_OP_WINDOW || op.kind == FFT_OP_TRANSFER) {
+            if (in0) t = op.amp[table_index(L, x, b, a)];
+            if (in1) t1 = op.amp[table_index(L, x1, b, a1)];
+        }
+        if (op.kind != FFT_OP_NONE) {
+            if (in0) v = apply_op(op, v, x, b, a, t);
+            if (in1) v1 = apply_op(op, v1, x1, b, a1, t1);
         }
         buf[c * LS + pad(a)] = v;
         if (has1) buf[c1 * LS + pad(a1)] = v1;
