@@ -28,18 +28,14 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 METRIC = "peaks resolved/s at 1024^3 tile: learned-bracket vs plain shell scan, same GPU"
 
-CONFIGS = {
-    # name: n, box, seed, filters (rmin, rmax, count), nbuff, nhunt
-    "tile1024": dict(n=1024, box=2048.0, seed=2024, rmin=1.5, rmax=34.0, nfilt=24, nbuff=64,
-                     nhunt=30),
-    "tile512": dict(n=512, box=1024.0, seed=7, rmin=1.5, rmax=20.0, nfilt=16, nbuff=32,
-                    nhunt=20),
-    "tile128": dict(n=128, box=256.0, seed=12345, rmin=1.5, rmax=20.0, nfilt=16, nbuff=16,
-                    nhunt=15),
-    # 64 tiles, sides drawn from {96, 128, 192, 256}, buffers 16..48, seeds 100..163
-    "batched64": dict(ntiles=64, sides=(96, 128, 192, 256), nbuff=(16, 48), seeds=(100, 163),
-                      rmin=1.5, rmax=20.0, nfilt=16, nhunt=15, alatt=2.0),
-}
+
+def _configs():
+    from huntian_hip import synth
+
+    return synth.WORKLOADS
+
+
+CONFIGS = _configs()
 
 
 def log(msg):
@@ -513,10 +509,10 @@ def run_batched(args, cfg):
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(0)
-    rng = np.random.default_rng(cfg["seeds"][0])
-    nt = cfg["ntiles"]
-    sides = rng.choice(cfg["sides"], size=nt)
-    nbuffs = rng.integers(cfg["nbuff"][0], cfg["nbuff"][1] + 1, size=nt)
+    specs = synth.batched_specs(args.config)
+    nt = len(specs)
+    sides = [s for s, _, _ in specs]
+    nbuffs = [b for _, b, _ in specs]
     radii = synth.filter_radii(cfg["rmin"], cfg["rmax"], cfg["nfilt"])
     alatt, nhunt = cfg["alatt"], cfg["nhunt"]
     ct = read_homeltab(WEBSKY_TABLE)
@@ -528,8 +524,7 @@ def run_batched(args, cfg):
     ctx = api.context(0)
     tiles, counts = [], []
     t0 = time.time()
-    for q in range(nt):
-        n, nb, seed = int(sides[q]), int(nbuffs[q]), cfg["seeds"][0] + q
+    for n, nb, seed in specs:
         box = alatt * n
         fields, dk = synth.fields_torch(n, box, seed, dev)
         sm = [synth.smooth_tophat_torch(dk, n, box, R) for R in radii]

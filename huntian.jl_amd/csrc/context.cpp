@@ -44,11 +44,33 @@ void release(DevBuf& b) {
     b.bytes = 0;
 }
 
+hnt_status reserve_in(hnt_ctx* ctx, DevBuf& b, size_t bytes, Arena* ar) {
+    if (!ar) return reserve(ctx, b, bytes);
+    const size_t off = (ar->used + 255) & ~(size_t)255;
+    if (off > ar->cap || bytes > ar->cap - off) {
+        set_error("fixed scratch of %zu bytes exhausted: %zu more bytes needed at offset %zu",
+                  ar->cap, bytes, off);
+        return HNT_ERR_CAPACITY;
+    }
+    b.p = ar->base + off;
+    b.bytes = bytes;
+    ar->used = off + bytes;
+    return HNT_OK;
+}
+
 void release_work(SAWork& w) {
+    if (w.arena) {  // carved from a fixed region: nothing to free
+        w = SAWork();
+        return;
+    }
     DevBuf* bufs[] = {&w.recs, &w.keys_a, &w.keys_b, &w.vals_a, &w.vals_b, &w.cub_tmp, &w.slots,
                       &w.gtiles, &w.tiles, &w.tilemeta, &w.stops, &w.feats, &w.prof, &w.queue,
-                      &w.flag};
+                      &w.flag, &w.tables.cells, &w.tables.start, &w.tables.count,
+                      &w.tables.r2, &w.tables.rad, &w.tables.kw, &w.tables.kw_idx,
+                      &w.ctab.table, &w.akk};
     for (DevBuf* b : bufs) release(*b);
+    w.tables = TablesDev();
+    w.ctab = CTabDev();
 }
 
 hnt_status begin_call(hnt_ctx* ctx) {
@@ -110,14 +132,17 @@ hnt_status finish_call(hnt_ctx* ctx, const char* what) {
 // Shell tables: validated on the host (the library does not trust the caller's grouping),
 // then uploaded once and reused while the caller passes identical tables.
 hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t) {
+    return upload_tables_to(ctx, ctx->tables, t, nullptr);
+}
+
+hnt_status upload_tables_to(hnt_ctx* ctx, TablesDev& T, const hnt_shell_tables* t, Arena* ar) {
     HNT_REQUIRE(t && t->di && t->dj && t->dk && t->shell_start && t->shell_count && t->shell_r2,
                 "shell tables: NULL array");
     HNT_REQUIRE(t->nshells >= 3 && t->ncells > 0 && t->ncells < (1LL << 31),
                 "shell tables: need at least 3 shells (rmax >= 2), got %d", t->nshells);
     const int32_t ns = t->nshells;
     const int64_t nc = t->ncells;
-    TablesDev& T = ctx->tables;
-    if (T.rmax == t->rmax && T.nshells == ns && T.ncells == nc &&
+    if (!ar && T.rmax == t->rmax && T.nshells == ns && T.ncells == nc &&
         T.host.size() == (size_t)(3 * nc + 3 * ns)) {
         const int32_t* H = T.host.data();
         const size_t b = sizeof(int32_t);
@@ -187,13 +212,13 @@ hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t) {
         }
     }
     if (kw.empty()) kw.push_back(0.0);
-    HNT_TRY(reserve(ctx, T.cells, cells4.size() * sizeof(int32_t)));
-    HNT_TRY(reserve(ctx, T.start, (size_t)ns * sizeof(int32_t)));
-    HNT_TRY(reserve(ctx, T.count, (size_t)ns * sizeof(int32_t)));
-    HNT_TRY(reserve(ctx, T.r2, (size_t)ns * sizeof(int32_t)));
-    HNT_TRY(reserve(ctx, T.rad, rad.size() * sizeof(double)));
-    HNT_TRY(reserve(ctx, T.kw, kw.size() * sizeof(double)));
-    HNT_TRY(reserve(ctx, T.kw_idx, kidx.size() * sizeof(int32_t)));
+    HNT_TRY(reserve_in(ctx, T.cells, cells4.size() * sizeof(int32_t), ar));
+    HNT_TRY(reserve_in(ctx, T.start, (size_t)ns * sizeof(int32_t), ar));
+    HNT_TRY(reserve_in(ctx, T.count, (size_t)ns * sizeof(int32_t), ar));
+    HNT_TRY(reserve_in(ctx, T.r2, (size_t)ns * sizeof(int32_t), ar));
+    HNT_TRY(reserve_in(ctx, T.rad, rad.size() * sizeof(double), ar));
+    HNT_TRY(reserve_in(ctx, T.kw, kw.size() * sizeof(double), ar));
+    HNT_TRY(reserve_in(ctx, T.kw_idx, kidx.size() * sizeof(int32_t), ar));
     HNT_HIP(hipMemcpyAsync(T.cells.p, cells4.data(), cells4.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, ctx->stream));
     HNT_HIP(hipMemcpyAsync(T.start.p, t->shell_start, ns * sizeof(int32_t), hipMemcpyHostToDevice,
@@ -217,14 +242,17 @@ hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t) {
 }
 
 hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c) {
+    return upload_ctab_to(ctx, ctx->ctab, c, nullptr);
+}
+
+hnt_status upload_ctab_to(hnt_ctx* ctx, CTabDev& D, const hnt_ctab* c, Arena* ar) {
     HNT_REQUIRE(c && c->table, "collapse table: NULL");
     HNT_REQUIRE(c->nx >= 1 && c->ny >= 1 && c->nz >= 1, "collapse table: bad shape");
     size_t n = (size_t)c->nx * c->ny * c->nz;
-    CTabDev& D = ctx->ctab;
-    if (D.nx == c->nx && D.ny == c->ny && D.nz == c->nz && D.host.size() == n &&
+    if (!ar && D.nx == c->nx && D.ny == c->ny && D.nz == c->nz && D.host.size() == n &&
         std::memcmp(D.host.data(), c->table, n * sizeof(float)) == 0)
         return HNT_OK;
-    HNT_TRY(reserve(ctx, D.table, n * sizeof(float)));
+    HNT_TRY(reserve_in(ctx, D.table, n * sizeof(float), ar));
     HNT_HIP(hipMemcpyAsync(D.table.p, c->table, n * sizeof(float), hipMemcpyHostToDevice,
                            ctx->stream));
     HNT_HIP(hipStreamSynchronize(ctx->stream));
@@ -236,8 +264,10 @@ hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c) {
 }
 
 // SPH kernel table of RadialShell._build_atab4 (RadialShell.jl:125-142), FP64.
-hnt_status ensure_akk(hnt_ctx* ctx) {
-    if (ctx->akk.p) return HNT_OK;
+hnt_status ensure_akk(hnt_ctx* ctx) { return ensure_akk_to(ctx, ctx->akk, nullptr); }
+
+hnt_status ensure_akk_to(hnt_ctx* ctx, DevBuf& akk_buf, Arena* ar) {
+    if (!ar && akk_buf.p) return HNT_OK;
     double akk[1101];
     const double ca = 3.0 / (2.0 * kPi);
     const double cb = 1.0 / (4.0 * kPi);
@@ -257,8 +287,8 @@ hnt_status ensure_akk(hnt_ctx* ctx) {
         akk[i - 1] = v;
     }
     akk[1100] = 0.0;
-    HNT_TRY(reserve(ctx, ctx->akk, sizeof(akk)));
-    HNT_HIP(hipMemcpyAsync(ctx->akk.p, akk, sizeof(akk), hipMemcpyHostToDevice, ctx->stream));
+    HNT_TRY(reserve_in(ctx, akk_buf, sizeof(akk), ar));
+    HNT_HIP(hipMemcpyAsync(akk_buf.p, akk, sizeof(akk), hipMemcpyHostToDevice, ctx->stream));
     HNT_HIP(hipStreamSynchronize(ctx->stream));
     return HNT_OK;
 }

@@ -58,14 +58,6 @@ struct TileDev {
     int64_t base;  // global id of the tile's first peak
 };
 
-// Device buffers of one shell analysis (the context holds one for direct calls, each plan
-// its own): cell records, sort keys, lane slots, tile descriptors, profile slots, queues.
-struct SAWork {
-    DevBuf recs, keys_a, keys_b, vals_a, vals_b, cub_tmp, slots, gtiles, tiles, tilemeta, stops,
-        feats, prof, queue, flag;
-    std::vector<TileDev> tile_host;
-    std::vector<int64_t> meta_host;
-};
 
 // scratch of hnt_merge_catalog: sort keys and permutations, ranks, the cell hash,
 // containment edges, decision states, overlap pairs, volume deficits, new radii
@@ -84,6 +76,30 @@ struct CTabDev {
     int32_t nx = 0, ny = 0, nz = 0;
     std::vector<float> host;
     DevBuf table;
+};
+
+// Bump allocator over a fixed device region: buffers carved from it are never freed, the
+// region is reused from its start by the next call that uses it.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+// Device buffers of one shell analysis (the context holds one for direct calls, each plan
+// its own): cell records, sort keys, lane slots, tile descriptors, profile slots, queues.
+// With own_tables the shell tables, collapse table and SPH kernel table the kernels read
+// are copies held here (a plan's replays must not depend on what the context uploads later);
+// otherwise they are the context's.  With an arena every buffer is carved from it.
+struct SAWork {
+    DevBuf recs, keys_a, keys_b, vals_a, vals_b, cub_tmp, slots, gtiles, tiles, tilemeta, stops,
+        feats, prof, queue, flag;
+    std::vector<TileDev> tile_host;
+    std::vector<int64_t> meta_host;
+    bool own_tables = false;
+    TablesDev tables;
+    CTabDev ctab;
+    DevBuf akk;
+    Arena* arena = nullptr;
 };
 
 }  // namespace hnt
@@ -172,9 +188,16 @@ hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, in
                        const FftOp& op, float scale, int mode, float coef, float* out);
 hnt_status fft_amp_table(hnt_ctx* ctx, int L, double dk, const float* tab, int ntab,
                          float log_k_min, float d_log_k, float* amp);
+// reserve(), or a carve of `bytes` from `ar` when it is not NULL (HNT_ERR_CAPACITY if the
+// arena is full)
+hnt_status reserve_in(hnt_ctx* ctx, DevBuf& b, size_t bytes, Arena* ar);
+// Uploads into the context's copies, or into T / D / akk (carved from ar if not NULL)
 hnt_status upload_tables(hnt_ctx* ctx, const hnt_shell_tables* t);
 hnt_status upload_ctab(hnt_ctx* ctx, const hnt_ctab* c);
 hnt_status ensure_akk(hnt_ctx* ctx);
+hnt_status upload_tables_to(hnt_ctx* ctx, TablesDev& T, const hnt_shell_tables* t, Arena* ar);
+hnt_status upload_ctab_to(hnt_ctx* ctx, CTabDev& D, const hnt_ctab* c, Arena* ar);
+hnt_status ensure_akk_to(hnt_ctx* ctx, DevBuf& akk, Arena* ar);
 hnt_status begin_call(hnt_ctx* ctx);
 hnt_status finish_call(hnt_ctx* ctx, const char* what);
 hnt_status mlp_predict(hnt_ctx* ctx, const hnt_mlp* net, const float* x, int64_t B, float* out);

@@ -1785,15 +1785,18 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     }
     HNT_REQUIRE(ntot < (1LL << 31), "hnt_shell_analyse: %lld peaks in one call",
                 (long long)ntot);
-    HNT_TRY(upload_tables(ctx, tables));
-    HNT_TRY(upload_ctab(ctx, ctab));
-    HNT_TRY(ensure_akk(ctx));
+    TablesDev& TT = W.own_tables ? W.tables : ctx->tables;
+    CTabDev& CT = W.own_tables ? W.ctab : ctx->ctab;
+    DevBuf& AKK = W.own_tables ? W.akk : ctx->akk;
+    HNT_TRY(upload_tables_to(ctx, TT, tables, W.arena));
+    HNT_TRY(upload_ctab_to(ctx, CT, ctab, W.arena));
+    HNT_TRY(ensure_akk_to(ctx, AKK, W.arena));
     if (ntot == 0) return HNT_OK;
     hipStream_t st = ctx->stream;
 
     // ---- validate the peak cells and ZZon on the device before any gather ----
     stage_begin(ctx, HNT_STAGE_SA_PREP);
-    HNT_TRY(reserve(ctx, W.flag, 64));
+    HNT_TRY(reserve_in(ctx, W.flag, 64, W.arena));
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)W.flag.p, 1,
                        0ull);
     for (int t = 0; t < ntiles; ++t) {
@@ -1812,7 +1815,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     }
 
     // ---- repack every tile's fields into 32-byte cell records ----
-    HNT_TRY(reserve(ctx, W.recs, (size_t)ncell * 32));
+    HNT_TRY(reserve_in(ctx, W.recs, (size_t)ncell * 32, W.arena));
     std::vector<TileDev>& TD = W.tile_host;
     TD.assign((size_t)ntiles, TileDev{});
     std::vector<RepackJob> jobs;
@@ -1852,8 +1855,8 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
         }
     }
     launch_repacks(st, jobs);
-    HNT_TRY(reserve(ctx, W.tiles, TD.size() * sizeof(TileDev)));
-    HNT_TRY(reserve(ctx, W.tilemeta, meta.size() * sizeof(int64_t)));
+    HNT_TRY(reserve_in(ctx, W.tiles, TD.size() * sizeof(TileDev), W.arena));
+    HNT_TRY(reserve_in(ctx, W.tilemeta, meta.size() * sizeof(int64_t), W.arena));
     W.meta_host = meta;
     HNT_HIP(hipMemcpyAsync(W.tiles.p, TD.data(), TD.size() * sizeof(TileDev),
                            hipMemcpyHostToDevice, st));
@@ -1863,18 +1866,18 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     const int64_t* tile_pad = tile_base + ntiles;
 
     // ---- first order: (tile, ir2min, Morton) ----
-    HNT_TRY(reserve(ctx, W.keys_a, (size_t)ntot * 8));
-    HNT_TRY(reserve(ctx, W.keys_b, (size_t)ntot * 8));
-    HNT_TRY(reserve(ctx, W.vals_a, (size_t)ntot * 4));
-    HNT_TRY(reserve(ctx, W.vals_b, (size_t)ntot * 4));
-    HNT_TRY(reserve(ctx, W.slots, (size_t)ngroups * 64 * 4));
-    HNT_TRY(reserve(ctx, W.gtiles, (size_t)ngroups * 4));
+    HNT_TRY(reserve_in(ctx, W.keys_a, (size_t)ntot * 8, W.arena));
+    HNT_TRY(reserve_in(ctx, W.keys_b, (size_t)ntot * 8, W.arena));
+    HNT_TRY(reserve_in(ctx, W.vals_a, (size_t)ntot * 4, W.arena));
+    HNT_TRY(reserve_in(ctx, W.vals_b, (size_t)ntot * 4, W.arena));
+    HNT_TRY(reserve_in(ctx, W.slots, (size_t)ngroups * 64 * 4, W.arena));
+    HNT_TRY(reserve_in(ctx, W.gtiles, (size_t)ngroups * 4, W.arena));
     int32_t* slot = (int32_t*)W.slots.p;
     size_t tmp_bytes = 0;
     HNT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint64_t*)W.keys_a.p,
                                                (uint64_t*)W.keys_b.p, (int32_t*)W.vals_a.p,
                                                (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st));
-    HNT_TRY(reserve(ctx, W.cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16));
+    HNT_TRY(reserve_in(ctx, W.cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16, W.arena));
     auto order_by = [&](const int32_t* const* prim, int L) -> hnt_status {
         for (int t = 0; t < ntiles; ++t) {
             const hnt_batch& b = batches[t];
@@ -1910,18 +1913,18 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     A.slot = slot;
     A.gtile = (const int32_t*)W.gtiles.p;
     A.ngroups = ngroups;
-    A.cells = (const int4*)ctx->tables.cells.p;
-    A.shell_start = (const int32_t*)ctx->tables.start.p;
-    A.shell_count = (const int32_t*)ctx->tables.count.p;
-    A.shell_r2 = (const int32_t*)ctx->tables.r2.p;
-    A.rad = (const double*)ctx->tables.rad.p;
-    A.kw = (const double*)ctx->tables.kw.p;
-    A.kidx = (const int32_t*)ctx->tables.kw_idx.p;
+    A.cells = (const int4*)TT.cells.p;
+    A.shell_start = (const int32_t*)TT.start.p;
+    A.shell_count = (const int32_t*)TT.count.p;
+    A.shell_r2 = (const int32_t*)TT.r2.p;
+    A.rad = (const double*)TT.rad.p;
+    A.kw = (const double*)TT.kw.p;
+    A.kidx = (const int32_t*)TT.kw_idx.p;
     A.NS = tables->nshells;
     A.rmax = tables->rmax;
-    A.ct = CTabView{(const float*)ctx->ctab.table.p, ctab->nx, ctab->ny, ctab->nz, ctab->x1,
+    A.ct = CTabView{(const float*)CT.table.p, ctab->nx, ctab->ny, ctab->nz, ctab->x1,
                     ctab->x2, ctab->y1, ctab->y2, ctab->z1, ctab->z2, ctab->out_val};
-    A.akk = (const double*)ctx->akk.p;
+    A.akk = (const double*)AKK.p;
     A.alatt = params->alatt;
     // hlatt = 1 (RadialShell.jl:296-300)
     A.aRnor = 3.0 * 1.0 / params->alatt;
@@ -1944,7 +1947,17 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     if (ctx->waves_per_cu > 0 && ctx->waves_per_cu < per_cu) per_cu = ctx->waves_per_cu;
     A.slot_doubles = (int64_t)(A.NS + 1) * PW;
     const size_t slot_bytes = (size_t)A.slot_doubles * 8;
-    const size_t budget = (size_t)24 << 30;  // profile scratch cap
+    HNT_TRY(reserve_in(ctx, W.queue, 8 * sizeof(unsigned long long), W.arena));
+    A.qctr = (unsigned long long*)W.queue.p;
+    const bool sched = A.early && params->order != HNT_ORDER_PLAIN;
+    if (sched && W.arena) HNT_TRY(reserve_in(ctx, W.stops, (size_t)ntot * 4, W.arena));
+    // profile scratch: 24 GiB at most, or what is left of a fixed region (fewer resident
+    // wavefronts then, same results)
+    size_t budget = (size_t)24 << 30;
+    if (W.arena) {
+        const size_t used = (W.arena->used + 255) & ~(size_t)255;
+        budget = used < W.arena->cap ? W.arena->cap - used : 0;
+    }
     int64_t nblocks = (int64_t)per_cu * ctx->num_cus;
     int64_t cap_blocks = (int64_t)(budget / (slot_bytes * 64));
     if (nblocks > cap_blocks) nblocks = cap_blocks;
@@ -1952,10 +1965,8 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     if (nblocks > want) nblocks = want;
     nblocks = (nblocks / 8) * 8;
     if (nblocks < 8) nblocks = 8;
-    HNT_TRY(reserve(ctx, W.prof, (size_t)nblocks * 64 * slot_bytes));
+    HNT_TRY(reserve_in(ctx, W.prof, (size_t)nblocks * 64 * slot_bytes, W.arena));
     A.prof = (double*)W.prof.p;
-    HNT_TRY(reserve(ctx, W.queue, 8 * sizeof(unsigned long long)));
-    A.qctr = (unsigned long long*)W.queue.p;
     for (int x = 0; x < 8; ++x) {
         A.qbeg[x] = ngroups * x / 8;
         A.qend[x] = ngroups * (x + 1) / 8;
@@ -1964,17 +1975,16 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     stage_end(ctx, HNT_STAGE_SA_PREP);
 
     // ---- schedule: order the peaks by their (exact or predicted) gather extent ----
-    if (A.early && params->order != HNT_ORDER_PLAIN) {
+    if (sched) {
         stage_begin(ctx, HNT_STAGE_SA_CROSS);
-        HNT_TRY(reserve(ctx, W.stops, (size_t)ntot * 4));
+        if (!W.arena) HNT_TRY(reserve_in(ctx, W.stops, (size_t)ntot * 4, W.arena));
         int32_t* stops = (int32_t*)W.stops.p;
         if (params->order == HNT_ORDER_CROSSING) {
             const int64_t G8 = (ngroups + 7) / 8;
             hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A, G8,
                                stops);
         } else {
-            HNT_TRY(reserve(ctx, W.feats,
-                            (size_t)ntot * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4));
+            HNT_TRY(reserve_in(ctx, W.feats, (size_t)ntot * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4, W.arena));
             float* feats = (float*)W.feats.p;
             float* pred = feats + ntot * HNT_MLP_INPUTS;
             const int64_t G8 = (ngroups + 7) / 8;
@@ -2085,6 +2095,7 @@ extern "C" hnt_status hnt_plan_create(hnt_ctx* ctx, int32_t ntiles, const hnt_fi
     *out = nullptr;
     hnt_plan* P = new hnt_plan;
     P->ctx = ctx;
+    P->W.own_tables = true;  // replays read the plan's copies of the tables
     const hnt_status st = plan_build(P, ntiles, fields, batches, tables, ctab, params, results,
                                      masks);
     if (st != HNT_OK) {

@@ -361,3 +361,50 @@ def smooth_tophat_torch(delta_k, n, boxsize, R):
         w = torch.where(x > 0, w, torch.ones_like(w))
         out[sl] = delta_k[sl] * w
     return torch.fft.irfftn(out, s=(n, n, n)).permute(2, 1, 0)
+
+
+# ---------------------------------------------------------------- benchmark workloads
+
+# name: n, box (Mpc/h), Threefry seed, filter bank (rmin, rmax, count; log-spaced, largest
+# first), buffer cells, hunt radius in cells.  Peak threshold 1.5 for every filter, collapse
+# at z = 0 with HomelTab_websky.dat, alatt = box / n = 2 Mpc/h throughout.
+WORKLOADS = {
+    "tile1024": dict(n=1024, box=2048.0, seed=2024, rmin=1.5, rmax=34.0, nfilt=24, nbuff=64,
+                     nhunt=30),
+    "tile512": dict(n=512, box=1024.0, seed=7, rmin=1.5, rmax=20.0, nfilt=16, nbuff=32,
+                    nhunt=20),
+    "tile128": dict(n=128, box=256.0, seed=12345, rmin=1.5, rmax=20.0, nfilt=16, nbuff=16,
+                    nhunt=15),
+    # 64 tiles, sides drawn from {96, 128, 192, 256}, buffers 16..48, seeds 100..163
+    "batched64": dict(ntiles=64, sides=(96, 128, 192, 256), nbuff=(16, 48), seeds=(100, 163),
+                      rmin=1.5, rmax=20.0, nfilt=16, nhunt=15, alatt=2.0),
+}
+
+
+def workload_tile(name: str, device, smoothed: bool = True, seed: int | None = None):
+    """Inputs of one tile workload, synthesised on the GPU: the fields (delta, psi_1, psi_2)
+    and, with smoothed, the filter bank's smoothed deltas (largest radius first)."""
+    import torch
+
+    w = dict(WORKLOADS[name])
+    if seed is not None:
+        w["seed"] = seed
+    n, box = w["n"], w["box"]
+    fields, dk = fields_torch(n, box, w["seed"], device)
+    radii = filter_radii(w["rmin"], w["rmax"], w["nfilt"])
+    sm = [smooth_tophat_torch(dk, n, box, R) for R in radii] if smoothed else []
+    del dk
+    torch.cuda.synchronize(device)
+    w.update(fields=fields, smoothed=sm, radii=radii, alatt=box / n)
+    return w
+
+
+def batched_specs(name: str = "batched64"):
+    """(side, nbuff, seed) of every tile of a batched workload: sides and buffers drawn with
+    numpy's default generator seeded by the first tile seed, seeds consecutive."""
+    w = WORKLOADS[name]
+    rng = np.random.default_rng(w["seeds"][0])
+    nt = w["ntiles"]
+    sides = rng.choice(w["sides"], size=nt)
+    nbuffs = rng.integers(w["nbuff"][0], w["nbuff"][1] + 1, size=nt)
+    return [(int(sides[q]), int(nbuffs[q]), w["seeds"][0] + q) for q in range(nt)]

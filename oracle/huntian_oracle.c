@@ -371,6 +371,75 @@ int64_t hro_peak_find_multi(const float* const* fields, const float* fcrit, int3
     return total;
 }
 
+/* The same filter loop with each filter's scan split over k-planes between threads.  A
+ * cell's test reads the mask only at that cell and a scan visits each cell once, so the
+ * planes are independent within a filter: each thread counts the peaks of its planes, the
+ * counts give every plane its offset in the (k, j, i) order of the serial scan, and a second
+ * pass writes the peaks and masks those that fit under cap.  Same outputs as
+ * hro_peak_find_multi. */
+static int peak_at(const float* delta, const int8_t* mask, int64_t c, int64_t s1, int64_t s2,
+                   float fc) {
+    if (mask[c] == 1) return 0;
+    float ff = delta[c];
+    if (ff < fc) return 0;
+    for (int kk = -1; kk <= 1; ++kk)
+        for (int jj = -1; jj <= 1; ++jj)
+            for (int ii = -1; ii <= 1; ++ii)
+                if (ff < delta[c + ii + jj * s1 + kk * s2]) return 0;
+    return 1;
+}
+
+int64_t hro_peak_find_multi_par(const float* const* fields, const float* fcrit, int32_t nfilt,
+                                int32_t n, int32_t nbuff, int8_t* mask, int64_t cap,
+                                int32_t* pi, int32_t* pj, int32_t* pk, int32_t* filt,
+                                float* pdelta, int32_t nthreads) {
+    const int64_t s1 = n, s2 = (int64_t)n * n;
+    const int32_t k0 = nbuff + 1, k1 = n - nbuff;
+    if (k1 < k0) return 0;
+    const int32_t np = k1 - k0 + 1;
+    int64_t* off = (int64_t*)calloc((size_t)np + 1, sizeof(int64_t));
+    if (!off) return -1;
+    if (nthreads < 1) nthreads = 1;
+    int64_t total = 0;
+    for (int32_t f = 0; f < nfilt; ++f) {
+        const float* delta = fields[f];
+        const float fc = fcrit[f];
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+#endif
+        for (int32_t q = 0; q < np; ++q) {
+            const int32_t k = k0 + q;
+            int64_t cnt = 0;
+            for (int32_t j = nbuff + 1; j <= n - nbuff; ++j)
+                for (int32_t i = nbuff + 1; i <= n - nbuff; ++i)
+                    cnt += peak_at(delta, mask, (i - 1) + (j - 1) * s1 + (k - 1) * s2, s1, s2,
+                                   fc);
+            off[q + 1] = cnt;
+        }
+        off[0] = total;
+        for (int32_t q = 0; q < np; ++q) off[q + 1] += off[q];
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+#endif
+        for (int32_t q = 0; q < np; ++q) {
+            const int32_t k = k0 + q;
+            int64_t o = off[q];
+            if (o >= cap) continue;
+            for (int32_t j = nbuff + 1; j <= n - nbuff && o < cap; ++j)
+                for (int32_t i = nbuff + 1; i <= n - nbuff && o < cap; ++i) {
+                    const int64_t c = (i - 1) + (j - 1) * s1 + (k - 1) * s2;
+                    if (!peak_at(delta, mask, c, s1, s2, fc)) continue;
+                    pi[o] = i; pj[o] = j; pk[o] = k; filt[o] = f; pdelta[o] = delta[c];
+                    mask[c] = 1;
+                    ++o;
+                }
+        }
+        total = off[np];
+    }
+    free(off);
+    return total;
+}
+
 /* ------------------------------------------------------------------ shell gather */
 
 void hro_shell_gather_full(const hnt_fields* f, int64_t npeaks, const int32_t* pi,

@@ -45,6 +45,10 @@ int64_t hro_find_peaks(const float* delta, int8_t* mask, int32_t n1, int32_t n2,
 int64_t hro_peak_find_multi(const float* const* fields, const float* fcrit, int32_t nfilt,
                             int32_t n, int32_t nbuff, int8_t* mask, int64_t cap,
                             int32_t* pi, int32_t* pj, int32_t* pk, int32_t* filt, float* pdelta);
+int64_t hro_peak_find_multi_par(const float* const* fields, const float* fcrit, int32_t nfilt,
+                                int32_t n, int32_t nbuff, int8_t* mask, int64_t cap,
+                                int32_t* pi, int32_t* pj, int32_t* pk, int32_t* filt,
+                                float* pdelta, int32_t nthreads);
 
 /* Per-shell sums of every shell for every peak (the brute-force loop of
  * test/test_shell_gpu.jl:241-300, in the FP64 arithmetic of analyse_peak_gpu). */

@@ -66,6 +66,8 @@ def lib():
                                           C.c_int32, _i8p, C.c_int64, _i32p, _i32p, _i32p,
                                           _i32p, _f32p]
         L.hro_peak_find_multi.restype = C.c_int64
+        L.hro_peak_find_multi_par.argtypes = L.hro_peak_find_multi.argtypes + [C.c_int32]
+        L.hro_peak_find_multi_par.restype = C.c_int64
         L.hro_shell_gather_full.argtypes = [C.POINTER(abi.Fields), C.c_int64, _i32p, _i32p,
                                             _i32p, C.POINTER(abi.ShellTables)] + [_f64p, _i32p] + [_f64p] * 6
         L.hro_analyse_peaks.argtypes = [C.POINTER(abi.Fields), C.POINTER(abi.Batch),
@@ -194,7 +196,9 @@ def find_peaks(delta, mask, nbuff, fcrit, max_peaks=2_000_000):
     return pi[:np_], pj[:np_], pk[:np_], pd[:np_]
 
 
-def peak_find_multi(fields, fcrits, nbuff, mask, cap=None):
+def peak_find_multi(fields, fcrits, nbuff, mask, cap=None, nthreads=1):
+    """The filter loop of find_peaks scans, mask carried.  nthreads > 1 splits each filter's
+    scan over k-planes (same output, hro_peak_find_multi_par)."""
     fs = [np.asfortranarray(f, dtype=np.float32) for f in fields]
     n = fs[0].shape[0]
     if cap is None:
@@ -203,10 +207,14 @@ def peak_find_multi(fields, fcrits, nbuff, mask, cap=None):
     fc = np.asarray(fcrits, np.float32)
     out = {k: np.zeros(cap, np.int32) for k in ("i", "j", "k", "filt")}
     out["delta"] = np.zeros(cap, np.float32)
-    tot = lib().hro_peak_find_multi(arr, _p(fc, C.c_float), len(fs), n, nbuff,
-                                    _p(mask, C.c_int8), cap, _p(out["i"], C.c_int32),
-                                    _p(out["j"], C.c_int32), _p(out["k"], C.c_int32),
-                                    _p(out["filt"], C.c_int32), _p(out["delta"], C.c_float))
+    args = (arr, _p(fc, C.c_float), len(fs), n, nbuff, _p(mask, C.c_int8), cap,
+            _p(out["i"], C.c_int32), _p(out["j"], C.c_int32), _p(out["k"], C.c_int32),
+            _p(out["filt"], C.c_int32), _p(out["delta"], C.c_float))
+    if nthreads > 1:
+        tot = lib().hro_peak_find_multi_par(*args, int(nthreads))
+        assert tot >= 0, "oracle peak_find_multi_par: out of host memory"
+    else:
+        tot = lib().hro_peak_find_multi(*args)
     cnt = min(tot, cap)
     return {k: v[:cnt] for k, v in out.items()}, tot
 

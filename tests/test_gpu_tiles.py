@@ -107,3 +107,33 @@ def test_plan_replays_match_eager_calls(oracle, websky):
         same(a, b)
     assert not torch.equal(plan.results[0]["RTHL"], eager[0]["RTHL"])
     plan.close()
+
+
+def test_plan_keeps_its_tables_after_other_calls(oracle, websky):
+    """A plan reads its own copies of the shell, collapse and SPH kernel tables: eager calls
+    on the same context with another hunt radius and another collapse table (which replace
+    the context's uploads, reallocating them) leave later replays exact."""
+    import torch
+    from fieldgen import nontrivial_ctab
+    from huntian_hip.homeltab import make_ctab
+
+    nbuff, nhunt = 11, 10
+    tiles = [_tile(n, s, nbuff, oracle) for n, s in [(32, 211), (40, 212)]]
+    stab = api.build_shell_tables(nhunt)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    kw = dict(alatt=2.0, ZZon=1.0, fcrit=fcrit, nbuff=nbuff)
+    plan = api.ShellPlan(tiles, stab, websky, **kw)
+    ref = [{k: v.clone() for k, v in r.items()} for r in plan.results]
+    # other calls in between: larger and smaller shell tables, another collapse table
+    other_ct = nontrivial_ctab(make_ctab)
+    for rmax, ct in ((14, other_ct), (6, websky), (9, other_ct)):
+        api.shell_analyse_tiles(tiles, api.build_shell_tables(rmax), ct, **kw)
+    for r in plan.results:
+        for v in r.values():
+            v.fill_(-3)
+    plan.launch()
+    torch.cuda.synchronize()
+    for a, b in zip(plan.results, ref):
+        for k in a:
+            assert torch.equal(torch.nan_to_num(a[k], nan=7.0), torch.nan_to_num(b[k], nan=7.0)), k
+    plan.close()

@@ -111,3 +111,26 @@ def test_multi_filter_vs_brute_force(oracle, seed):
     for (a, b, c, _, _) in ref:
         exp_mask[a - 1, b - 1, c - 1] = 1
     assert np.array_equal(mask, exp_mask)
+
+
+@pytest.mark.parametrize("cap", [None, 150])
+def test_threaded_scan_equals_serial_scan(oracle, cap):
+    """The k-plane-threaded filter loop (used on the large tiles) returns the serial scan's
+    peaks, total and mask, also under a capacity cut inside a filter (the cut peaks are
+    counted, neither written nor masked, and a later filter may count their cells again)."""
+    rng = np.random.default_rng(9)
+    n, nbuff = 40, 3
+    fields = [np.asfortranarray(rng.standard_normal((n, n, n)).astype(np.float32) * s)
+              for s in (1.0, 1.2, 0.9, 1.1)]
+    fields[2][10:14, 10:14, 10:14] = 4.0  # plateau ties
+    fields[3][rng.random((n, n, n)) < 0.003] = np.nan
+    fc = [1.0, 0.8, 0.5, 0.7]
+    mask0 = M(n)
+    mask0[rng.random((n, n, n)) < 0.04] = 1
+    m_ser, m_par = mask0.copy(order="F"), mask0.copy(order="F")
+    ser, tot_s = oracle.peak_find_multi(fields, fc, nbuff, m_ser, cap=cap)
+    par, tot_p = oracle.peak_find_multi(fields, fc, nbuff, m_par, cap=cap, nthreads=4)
+    assert tot_s == tot_p and len(ser["i"]) > 100
+    for k in ser:
+        assert np.array_equal(ser[k], par[k], equal_nan=True), k
+    assert np.array_equal(m_ser, m_par)
