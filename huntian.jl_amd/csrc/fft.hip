@@ -522,7 +522,7 @@ bool fft_supported(int L) { return L >= 8 && L <= 2048 && (L & (L - 1)) == 0; }
 // Forward twiddles W[k] = exp(-2 pi i k / L), k < L, computed in double, kept per L.
 hnt_status fft_twiddles(hnt_ctx* ctx, int L, const float2** out) {
     for (hnt_ctx::FftTw& t : ctx->fft_tw)
-        if (t.L == L) {
+        if (t.L == L && t.buf.p) {
             *out = (const float2*)t.buf.p;
             return HNT_OK;
         }
@@ -531,12 +531,19 @@ hnt_status fft_twiddles(hnt_ctx* ctx, int L, const float2** out) {
         const double a = -2.0 * M_PI * (double)k / (double)L;
         h[k] = make_float2((float)cos(a), (float)sin(a));
     }
-    ctx->fft_tw.push_back({L, DevBuf()});
-    DevBuf& b = ctx->fft_tw.back().buf;
+    // filled in a local buffer and cached only once it holds the table, so a failed
+    // allocation or copy leaves no entry behind
+    DevBuf b;
     HNT_TRY(reserve(ctx, b, sizeof(float2) * (size_t)L));
-    HNT_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(float2) * (size_t)L, hipMemcpyHostToDevice,
-                           ctx->stream));
-    HNT_HIP(hipStreamSynchronize(ctx->stream));
+    hipError_t e = hipMemcpyAsync(b.p, h.data(), sizeof(float2) * (size_t)L,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        release(b);
+        set_error("fft_twiddles: %s", hipGetErrorString(e));
+        return HNT_ERR_DEVICE;
+    }
+    ctx->fft_tw.push_back({L, b});
     *out = (const float2*)b.p;
     return HNT_OK;
 }

@@ -15,6 +15,23 @@ namespace hnt {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// x^-3 for a Float64 x as Julia evaluates the literal power `x^-3` (Base.literal_pow of an
+// AbstractFloat forwards to x^n with an Int64 n; n = -3 takes the compensated
+// square-and-multiply of its integer-power routine): r = 1/x with its residual
+// rlo = -(x r - 1) r, then r^3 carried as a double-double (two_mul products with their
+// fma-exact low parts) and summed once at the end.  The reference runs on hosts with FMA,
+// where its muladd steps are fused; fma() here does the same on host and device.
+HNT_HD double pow_m3(double x) {
+    const double r = 1.0 / x;
+    const double rlo = isfinite(x) ? -fma(x, r, -1.0) * r : 0.0;
+    // y = r (ylo = rlo); then x2 = r * r with its low part and the cross term 2 r rlo
+    const double y = r, ylo = rlo;
+    const double x2 = r * r;
+    const double x2lo = fma(r, r, -x2) + r * 2.0 * rlo;
+    if (!isfinite(x2)) return x2 * y;
+    return fma(x2, y, fma(y, x2lo, x2 * ylo));
+}
+
 // hRinteg: integral of the SPH kernel (RadialShell.jl:95-121).
 HNT_HD double hRinteg(double u0, double u12) {
     double x0 = fabs(u0);

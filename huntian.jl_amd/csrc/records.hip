@@ -158,7 +158,8 @@ __device__ PeakPos peak_position(const RecArgs& a, int64_t p) {
 
 // 1 + z of peak p, or 0 if its record is dropped (no collapse, or beyond z_max).
 __device__ double record_ZZon(const RecArgs& a, int64_t p, const PeakPos& q) {
-    if (!((float)a.res.RTHL[p] > 0.0f)) return 0.0;
+    // `r.RTHL[idx] <= 0 && continue` (MultiResolution.jl:1169): a NaN RTHL keeps its record
+    if ((float)a.res.RTHL[p] <= 0.0f) return 0.0;
     if (!a.pp.ievol) return a.pp.ZZon;
     // peak_redshift (Cosmology.jl:376-385)
     const double dx = q.x - a.pp.obs[0], dy = q.y - a.pp.obs[1], dz = q.z - a.pp.obs[2];
@@ -286,8 +287,7 @@ __global__ __launch_bounds__(FIN_T) void fin_kernel(FinArgs a) {
             const double r = sqrt(rx * rx + ry * ry + rz * rz);
             av = r > 0.0 ? 1.0 / (1.0 + chi_to_z(a.chiz, a.chid, a.nchi, a.chi_zmax, r)) : 1.0;
         }
-        const double ia = 1.0 / av;  // a^-3 = inv(a)^3
-        const double E = sqrt(a.Om * (ia * ia * ia) + a.OL);
+        const double E = sqrt(a.Om * pow_m3(av) + a.OL);  // Julia's a^-3
         const double f = growth_f(a.atab, a.ftab, a.ntab, av);
         const double vfac = av * 100.0 * E * f;
         h[0] = (float)ex;

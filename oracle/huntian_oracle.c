@@ -1437,8 +1437,24 @@ int64_t hro_merge_catalog(const float* rec, int64_t N, int32_t nf, int32_t reduc
 
 /* ------------------------------------------------------------------ finalize_eulerian */
 
+/* Julia's `x^-3` for a Float64 x: Base.literal_pow forwards to x^n (Int64 n), whose
+ * integer-power routine (base/math.jl, power by squaring) takes r = 1/x with the residual
+ * rlo = -fma(x, r, -1) r and carries the product r * r * r as a double-double: y = r
+ * (low part rlo); r^2 by two_mul with its low part plus the cross term 2 r rlo; the result
+ * fma(r^2, y, fma(y, lo(r^2), r^2 * lo(y))).  muladd is fused on FMA hosts, so fma here. */
+double hro_pow_m3(double x) {
+    double r = 1.0 / x;
+    double rlo = isfinite(x) ? -fma(x, r, -1.0) * r : 0.0;
+    double y = r, ylo = rlo;
+    double x2 = r * r;
+    double x2lo = fma(r, r, -x2) + r * 2.0 * rlo;
+    if (!isfinite(x2)) return x2 * y;
+    return fma(x2, y, fma(y, x2lo, x2 * ylo));
+}
+
 /* Merger.finalize_eulerian (src/Merger/Merger.jl:123-166): Eulerian position q + d1 + d2 and
- * velocity a 100 E(a) f(a) (d1 + 2 d2), E = sqrt(Om a^-3 + OL) with a^-3 as (1/a)^3; the
+ * velocity a 100 E(a) f(a) (d1 + 2 d2), E = sqrt(Om a^-3 + OL) with a^-3 as Julia's literal
+ * power evaluates it (hro_pow_m3); the
  * 2LPT fields zeroed, the rest kept.  a from the Eulerian distance to obs (ievol = 1) or
  * 1/(1 + z_out). */
 void hro_finalize_eulerian(const float* rec, int64_t N, int32_t nf, const hnt_cosmo_tables* t,
@@ -1458,8 +1474,7 @@ void hro_finalize_eulerian(const float* rec, int64_t N, int32_t nf, const hnt_co
             double r = sqrt(rx * rx + ry * ry + rz * rz);
             a = r > 0.0 ? 1.0 / (1.0 + hro_chi_to_z(t, r)) : 1.0;
         }
-        double ia = 1.0 / a;
-        double E = sqrt(Om * (ia * ia * ia) + OL);
+        double E = sqrt(Om * hro_pow_m3(a) + OL);
         double f = hro_growth_f(t, a);
         double vfac = a * 100.0 * E * f;
         float keep[33];

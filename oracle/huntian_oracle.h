@@ -76,6 +76,7 @@ int64_t hro_pack_records(int64_t npeaks, const int32_t* pi, const int32_t* pj,
                          const hnt_pack_params* pp, float* out, int64_t capacity);
 double hro_growth_D(const hnt_cosmo_tables* t, double a);   /* Dlinear_ab(a)[1] */
 double hro_growth_f(const hnt_cosmo_tables* t, double a);   /* Dlinear_ab(a)[2] */
+double hro_pow_m3(double x);
 void hro_finalize_eulerian(const float* rec, int64_t nhalo, int32_t nf, const hnt_cosmo_tables* t,
                            double Om, double OL, const double* obs, int32_t ievol, double z_out,
                            float* out);

@@ -96,6 +96,8 @@ def lib():
         L.hro_growth_D.restype = C.c_double
         L.hro_chi_to_z.argtypes = [C.POINTER(abi.CosmoTables), C.c_double]
         L.hro_chi_to_z.restype = C.c_double
+        L.hro_pow_m3.argtypes = [C.c_double]
+        L.hro_pow_m3.restype = C.c_double
         _lib = L
     return _lib
 

@@ -123,3 +123,30 @@ def test_einstein_de_sitter_rate_is_one(oracle):
                                  z_out=3.0)[0]
     # a 100 E f = 100 a a^-3/2 = 100 / sqrt(a), a = 1/4
     assert o[3] == pytest.approx(200.0 * (D1[0] + 2 * D2[0]), rel=1e-6)
+
+
+def test_julia_negative_cube_is_compensated(oracle):
+    """E(a) = sqrt(Om a^-3 + OL) (Merger.jl:139) with a^-3 evaluated as Julia's literal power
+    does: a compensated square-and-multiply whose result is the correctly rounded a^-3 except
+    in rare near-ties, where (1/a)^3 is off by an ulp or two in a large share of cases.
+    Pinned by exact rational arithmetic (Python's float(Fraction) rounds correctly)."""
+    from fractions import Fraction
+
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(0.05, 1.0, 3000), rng.uniform(1.0, 40.0, 500)])
+    exact_hits = naive_hits = 0
+    for x in xs:
+        exact = float(Fraction(float(x)) ** -3)
+        got = L.hro_pow_m3(float(x))
+        r = 1.0 / float(x)
+        naive = r * r * r
+        assert abs(got - exact) <= np.spacing(exact), (x, got, exact)
+        exact_hits += got == exact
+        naive_hits += naive == exact
+    assert exact_hits >= 0.995 * len(xs), exact_hits
+    assert naive_hits <= 0.9 * len(xs), naive_hits  # the test tells the two apart
+    for x, want in ((1.0, 1.0), (0.5, 8.0), (2.0, 0.125), (0.25, 64.0)):
+        assert L.hro_pow_m3(x) == want
+    assert L.hro_pow_m3(float("inf")) == 0.0
+    assert L.hro_pow_m3(0.0) == float("inf")

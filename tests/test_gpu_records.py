@@ -68,3 +68,29 @@ def test_records_capacity_and_errors(oracle, tile, tables):
     with pytest.raises(HuntianError):  # z beyond the growth table's interpolation range
         api.pack_records(pf["i"], pf["j"], pf["k"], tile["Rf"], r, pf["delta"],
                          growth=growth, **dict(kw, ZZon=5000.0))
+
+
+def test_nan_radius_keeps_its_record(oracle, tile, tables):
+    """The record loop skips a peak on `RTHL <= 0` (MultiResolution.jl:1169), which is false
+    for NaN: a NaN radius gives a record (with NaN fields), as in the oracle's loop."""
+    import torch
+
+    growth, _ = tables
+    pf = tile["pf"]
+    r = {k: v.clone() for k, v in tile["r"].items()}
+    nan_at = torch.nonzero(r["RTHL"] <= 0)[:3, 0]
+    r["RTHL"][nan_at] = float("nan")
+    kw = dict(alatt=tile["alatt"], n=tile["n"], ZZon=1.0, ilpt=2, Om=COSMO.Om, OL=COSMO.OL,
+              ioutshear=1)
+    got = api.pack_records(pf["i"], pf["j"], pf["k"], tile["Rf"], r, pf["delta"],
+                           growth=growth, d2Rf=tile["d2"], **kw)
+    host = {k: v.cpu().numpy() for k, v in r.items()}
+    ref, nh = oracle.pack_records(pf["i"].cpu().numpy(), pf["j"].cpu().numpy(),
+                                  pf["k"].cpu().numpy(), tile["Rf"].cpu().numpy(), host,
+                                  pf["delta"].cpu().numpy(), growth=growth,
+                                  d2Rf=tile["d2"].cpu().numpy(), **kw)
+    g = got.cpu().numpy()
+    assert nh == int((host["RTHL"] > 0).sum()) + 3 == g.shape[0]
+    exact = [q for q in range(g.shape[1]) if q not in (7, 8, 9)]
+    np.testing.assert_array_equal(g[:, exact], ref[:, exact])
+    assert int(np.isnan(g[:, 6]).sum()) == 3
