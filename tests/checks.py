@@ -28,6 +28,8 @@ def rel_errors(gpu, ref, keys=FLOAT_KEYS):
     against the rest of their vector are compared absolutely.  NaN patterns must agree."""
     both = (gpu["RTHL"] > 0) & (ref["RTHL"] > 0)
     worst = {}
+    if not both.any():
+        return {k: 0.0 for k in keys}
     for k in keys:
         a = gpu[k][both].reshape(int(both.sum()), -1)
         b = ref[k][both].reshape(int(both.sum()), -1)
@@ -50,6 +52,8 @@ def bar_errors(gpu, ref):
     p_v relative to themselves or, near 0, absolutely below 1e-3 (p ranges over [-e, e])."""
     both = (gpu["RTHL"] > 0) & (ref["RTHL"] > 0)
     out = {}
+    if not both.any():
+        return {**{k: 0.0 for k in BARS}, **{k + "_over_bar": 0 for k in BARS}}
     for k in BARS:
         a = gpu[k][both].reshape(int(both.sum()), -1)
         b = ref[k][both].reshape(int(both.sum()), -1)
