@@ -331,6 +331,10 @@ hnt_status hnt_ctx_create(int device, hnt_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     if (c->num_cus <= 0) c->num_cus = 256;
+    // the fixed scratch of hnt_shell_analyse_noalloc: its address on this device, once per
+    // device; a failure here only disables that entry point (it reports the error)
+    (void)noalloc_scratch_init(device);
+    clear_error();
     *out = c;
     return HNT_OK;
 }

@@ -198,6 +198,9 @@ hnt_status ensure_akk(hnt_ctx* ctx);
 hnt_status upload_tables_to(hnt_ctx* ctx, TablesDev& T, const hnt_shell_tables* t, Arena* ar);
 hnt_status upload_ctab_to(hnt_ctx* ctx, CTabDev& D, const hnt_ctab* c, Arena* ar);
 hnt_status ensure_akk_to(hnt_ctx* ctx, DevBuf& akk, Arena* ar);
+// The fixed scratch of hnt_shell_analyse_noalloc: its device address on ctx's device,
+// looked up once per device (shell_analyse.hip)
+hnt_status noalloc_scratch_init(int device);
 hnt_status begin_call(hnt_ctx* ctx);
 hnt_status finish_call(hnt_ctx* ctx, const char* what);
 hnt_status mlp_predict(hnt_ctx* ctx, const hnt_mlp* net, const float* x, int64_t B, float* out);

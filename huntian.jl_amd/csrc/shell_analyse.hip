@@ -31,6 +31,8 @@
 //    empty.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "hnt_features.h"
 #include "hnt_internal.h"
 
@@ -2120,4 +2122,90 @@ extern "C" hnt_status hnt_plan_destroy(hnt_plan* plan) {
     (void)hipStreamSynchronize(plan->ctx->stream);
     plan_free(plan);
     return HNT_OK;
+}
+
+// ------------------------------------------------------------------ fixed-scratch entry point
+// hnt_shell_analyse_noalloc runs hnt_shell_analyse with every device buffer -- cell records,
+// sort keys, lane slots, tile descriptor, queues, stop shells, profile slots, the shell,
+// collapse and SPH kernel tables -- carved from `scratch`, a static device array each GPU
+// holds from the library's load to the end of the process.  The host only ever uses the
+// array's device address from hipGetSymbolAddress (per device, looked up by
+// noalloc_scratch_init from hnt_ctx_create); the array's name on the host is not a device
+// address.
+
+namespace {
+
+constexpr size_t kNoallocScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) char scratch[kNoallocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+}  // namespace
+
+namespace hnt {
+
+// With `device` current: the address of that device's copy of `scratch`, once per device.
+hnt_status noalloc_scratch_init(int device) {
+    HNT_REQUIRE(device >= 0 && device < kMaxDevices, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return HNT_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        set_error("hipGetSymbolAddress(scratch) on device %d: %s", device,
+                  hipGetErrorString(g_scratch_err[device]));
+        return HNT_ERR_DEVICE;
+    }
+    g_scratch_ptr[device] = p;
+    return HNT_OK;
+}
+
+}  // namespace hnt
+
+extern "C" hnt_status hnt_shell_analyse_noalloc(hnt_ctx* ctx, const hnt_fields* fields,
+                                                const hnt_batch* batch,
+                                                const hnt_shell_tables* tables,
+                                                const hnt_ctab* ctab, const hnt_params* params,
+                                                hnt_results* results, int8_t* mask) {
+    HNT_TRY(begin_call(ctx));
+    HNT_REQUIRE(params, "hnt_shell_analyse_noalloc: NULL params");
+    if (params->order == HNT_ORDER_LEARNED) {
+        set_error("hnt_shell_analyse_noalloc: the learned schedule needs the network's "
+                  "scratch; use HNT_ORDER_CROSSING or HNT_ORDER_PLAIN");
+        return HNT_ERR_UNSUPPORTED;
+    }
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) {
+        // the context was created before the lookup succeeded: try once more, device current
+        HNT_TRY(noalloc_scratch_init(ctx->device));
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch_ptr[ctx->device];
+    }
+    Arena ar;
+    ar.base = (char*)base;
+    ar.cap = kNoallocScratchBytes;
+    SAWork W;
+    W.arena = &ar;
+    W.own_tables = true;
+    const hnt_status st =
+        analyse_tiles(ctx, W, 1, fields, batch, tables, ctab, params, results, &mask, nullptr);
+    if (st == HNT_ERR_CAPACITY) {
+        char msg[1024];
+        hnt_last_error(msg, sizeof(msg));
+        set_error("hnt_shell_analyse_noalloc: the call needs more than the library's fixed "
+                  "%zu MiB of scratch (%s); use hnt_shell_analyse", kNoallocScratchBytes >> 20,
+                  msg);
+        return st;
+    }
+    HNT_TRY(st);
+    return finish_call(ctx, "hnt_shell_analyse_noalloc");
 }

@@ -242,6 +242,7 @@ EXPORTED = (
     "hnt_peak_find",
     "hnt_peak_find_tile",
     "hnt_shell_analyse",
+    "hnt_shell_analyse_noalloc",
     "hnt_shell_analyse_tiles",
     "hnt_plan_create",
     "hnt_plan_launch",
@@ -328,6 +329,7 @@ def load(path: str | None = None):
     lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                       C.POINTER(ShellTables), C.POINTER(CTab),
                                       C.POINTER(Params), C.POINTER(Results), _i8p]
+    lib.hnt_shell_analyse_noalloc.argtypes = lib.hnt_shell_analyse.argtypes
     lib.hnt_shell_gather_full.argtypes = [vp, C.POINTER(Fields), C.c_int64, _i32p, _i32p, _i32p,
                                           C.POINTER(ShellTables), _f64p, _i32p, _f64p, _f64p,
                                           _f64p, _f64p, _f64p, _f64p]

@@ -291,6 +291,27 @@ hnt_status hnt_shell_analyse(hnt_ctx* ctx, const hnt_fields* fields, const hnt_b
                              const hnt_params* params, hnt_results* results, int8_t* mask);
 
 /*
+ * hnt_shell_analyse_noalloc -- no counterpart in the reference.  hnt_shell_analyse for
+ * callers that cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): same arguments, same results bit for bit, but it
+ * calls no hipMalloc / hipFree and keeps nothing in the context.  All of its device scratch
+ * (cell records 32 B per cell, sort keys and lane slots ~30 B per peak, the shell, collapse
+ * and SPH kernel tables, profile slots 192 B per shell per resident lane) lives in one
+ * fixed 256 MiB device array that each GPU the library runs on holds until the process
+ * ends; fewer wavefronts stay resident when the profile slots do not fit beside the rest.
+ * A call that needs more (e.g. a 128^3 tile with hunt radius 15 fits, a 256^3 tile does
+ * not: its records alone are 512 MiB) returns HNT_ERR_CAPACITY with the sizes in
+ * hnt_last_error.  Every call on one GPU shares that array: calls that overlap on a GPU
+ * (several host threads, or several contexts' streams) race and must be serialised by the
+ * caller.  The learned schedule (HNT_ORDER_LEARNED) is not available here
+ * (HNT_ERR_UNSUPPORTED).
+ */
+hnt_status hnt_shell_analyse_noalloc(hnt_ctx* ctx, const hnt_fields* fields,
+                                     const hnt_batch* batch, const hnt_shell_tables* tables,
+                                     const hnt_ctab* ctab, const hnt_params* params,
+                                     hnt_results* results, int8_t* mask);
+
+/*
  * hnt_shell_analyse_tiles -- hnt_shell_analyse over many tiles in one persistent launch (no
  * counterpart in the reference, whose drivers call analyse_peaks_gpu_cuda_multirf once per
  * tile, src/MultiResolution.jl:532-644).  fields, batches, results: arrays of ntiles
