@@ -66,6 +66,7 @@ struct SAArgs {
     double* prof;
     int64_t slot_doubles;
     const int32_t* stop_hint;  // exact stop shell per global peak id (crossing pass) or null
+    const float* pred;         // bracket network outputs per global peak id (learned order)
     unsigned long long* qctr;
     int64_t qbeg[8], qend[8];
 };
@@ -488,9 +489,13 @@ __device__ void write_no_collapse(const TileDev& T, int p) {
 
 // The radial logic of one peak from its profile slot P (RadialShell.jl:449-865 for A,
 // ShellAnalysisGPU.jl:231-535 for B; C = B + zvir_half of A).
+// walk_lo: the walk's first phase runs from the crossing down to this shell (the learned
+// bracket's lo - 2); a walk that passes it without collapsing goes on to shell 1, the full
+// walk.  *lo_true (if not NULL) receives the shell holding RTHL, 0 without a collapse.
 __device__ void post_process(const SAArgs& a, const TileDev& T, int p, const double* __restrict__ P, int ci, int cj,
                              int ck, double ZZon, double fcrit, int ir2min, int mrf,
-                             int mlow_rf, int mupp_rf) {
+                             int mlow_rf, int mupp_rf, int walk_lo = 1, int* lo_true = nullptr) {
+    if (lo_true) *lo_true = 0;
     const double* __restrict__ rad = a.rad;
     const int NS = a.NS;
     const int m = NS - 1;
@@ -551,6 +556,9 @@ __device__ void post_process(const SAArgs& a, const TileDev& T, int p, const dou
         return;
     }
     bool collapsed = false;
+    // phase 1: mp = m0 - 1 .. walk_lo; phase 2 (the fallback), only if phase 1 found no
+    // collapse: walk_lo - 1 .. 1 -- one loop, the same steps in the same order
+    (void)walk_lo;
     for (int mp = m0 - 1; mp >= 1; --mp) {
         const double rupp_mp = rad[mp] + 2.0;
         double rlow_mp = rad[mp] - 2.0;
@@ -608,6 +616,7 @@ __device__ void post_process(const SAArgs& a, const TileDev& T, int p, const dou
         write_no_collapse(T, p);
         return;
     }
+    if (lo_true) *lo_true = RTHL == rad[m0 - 1] ? m0 - 1 : m0;
     const double RTHL3 = cube(RTHL);
     const double RTHL5 = RTHL3 * RTHL * RTHL;
     const double rad3p = cube(rad[m0 - 1]);
@@ -753,6 +762,14 @@ __device__ void post_process(const SAArgs& a, const TileDev& T, int p, const dou
     R.gradpkrf[o3] = krf.g0; R.gradpkrf[o3 + 1] = krf.g1; R.gradpkrf[o3 + 2] = krf.g2;
 }
 
+// A shell index from a network output log(1 + s), rounded, clamped to [1, m].
+__device__ __forceinline__ int bracket_shell(float y, int m) {
+    float s = expf(y) - 1.0f;
+    if (!(s >= 1.0f)) s = 1.0f;  // NaN too
+    if (s > (float)m) s = (float)m;
+    return (int)(s + 0.5f);
+}
+
 template <class S>
 __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__ P) {
     const int lane = threadIdx.x;
@@ -877,7 +894,22 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     if (!valid) return;
     if (T.res.nshells_used) T.res.nshells_used[p] = stop;
     if (T.res.crossing_shell) T.res.crossing_shell[p] = scross;
-    post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
+    if (!a.pred) {
+        post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
+        if (T.res.bracket) T.res.bracket[p] = 0;
+        return;
+    }
+    // learned bracket [lo, hi]: the gather verified when it ended within hi + 2, the walk
+    // when it collapsed at or above lo - 2
+    const float* y = a.pred + (T.base + p) * HNT_MLP_OUTPUTS;
+    const int lo = bracket_shell(y[0], m), hi = bracket_shell(y[1], m);
+    const int walk_lo = lo - 2 > 1 ? lo - 2 : 1;
+    int lo_true = 0;
+    post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf, walk_lo,
+                 &lo_true);
+    if (T.res.bracket)
+        T.res.bracket[p] = (stop <= hi + 2 ? 1 : 0) | (lo_true >= walk_lo && lo_true > 0 ? 2 : 0) |
+                           ((y[2] > 0.0f) == (lo_true > 0) ? 4 : 0);
 }
 
 // ------------------------------------------------------------------ ganged gather
@@ -1640,7 +1672,8 @@ __global__ __launch_bounds__(64) void features_sched_kernel(SAArgs a, int64_t G8
                   T.Rf[p] / a.alatt, load, feats + (T.base + p) * HNT_MLP_INPUTS);
 }
 
-// Predicted gather extent (network output 1 = log(1 + shells)), clamped to [1, m].
+// The bracket's hi, the predicted gather extent (network output 1 = log(1 + shells)),
+// clamped to [1, m].
 __global__ void pred_stop_kernel(const float* __restrict__ pred, int64_t np, int m,
                                  int32_t* __restrict__ stop) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1761,8 +1794,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     HNT_REQUIRE(fields && batches && tables && ctab && params && results,
                 "hnt_shell_analyse: NULL argument");
     HNT_REQUIRE(params->variant >= 0 && params->variant <= 2, "hnt_shell_analyse: bad variant");
-    HNT_REQUIRE(params->order >= HNT_ORDER_CROSSING && params->order <= HNT_ORDER_LEARNED,
+    HNT_REQUIRE(params->order >= HNT_ORDER_CROSSING && params->order <= HNT_ORDER_GIVEN,
                 "hnt_shell_analyse: bad order %d", params->order);
+    HNT_REQUIRE(params->order != HNT_ORDER_GIVEN || params->extent,
+                "hnt_shell_analyse: HNT_ORDER_GIVEN needs hnt_params.extent");
     HNT_REQUIRE(params->precision == HNT_PREC_FP64 || params->precision == HNT_PREC_FP32_KAHAN,
                 "hnt_shell_analyse: bad precision %d", params->precision);
     HNT_REQUIRE(params->order != HNT_ORDER_LEARNED ||
@@ -1985,6 +2020,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
             const int64_t G8 = (ngroups + 7) / 8;
             hipLaunchKernelGGL(crossing_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, A, G8,
                                stops);
+        } else if (params->order == HNT_ORDER_GIVEN) {
+            // the caller's extents order the peaks; no pass of the library's own
+            HNT_HIP(hipMemcpyAsync(stops, params->extent, (size_t)ntot * 4,
+                                   hipMemcpyDeviceToDevice, st));
         } else {
             HNT_TRY(reserve_in(ctx, W.feats, (size_t)ntot * (HNT_MLP_INPUTS + HNT_MLP_OUTPUTS) * 4, W.arena));
             float* feats = (float*)W.feats.p;
@@ -1995,6 +2034,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
             HNT_TRY(mlp_predict(ctx, params->bracket, feats, ntot, pred));
             hipLaunchKernelGGL(pred_stop_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256),
                                0, st, pred, ntot, A.NS - 1, stops);
+            A.pred = pred;  // the main kernel checks each peak against its bracket
         }
         for (int t = 0; t < ntiles; ++t) prim[t] = stops + meta[t];
         if (params->order == HNT_ORDER_CROSSING) A.stop_hint = stops;

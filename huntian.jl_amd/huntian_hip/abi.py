@@ -162,9 +162,9 @@ class PackParams(C.Structure):
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS = 18, 128, 3
 MLP_NPARAM = 18 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 3 + 3
-HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED = 0, 1, 2
+HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED, HNT_ORDER_GIVEN = 0, 1, 2, 3
 ORDERS = {"crossing": HNT_ORDER_CROSSING, "plain": HNT_ORDER_PLAIN,
-          "learned": HNT_ORDER_LEARNED}
+          "learned": HNT_ORDER_LEARNED, "given": HNT_ORDER_GIVEN}
 PRECISIONS = {"fp64": 0, "fp32_kahan": 1}
 TUNING = {"brick_log2": 0, "waves_per_cu": 1, "gang": 2, "fft": 3}
 
@@ -189,12 +189,13 @@ class Params(C.Structure):
         ("order", C.c_int32),
         ("precision", C.c_int32),
         ("bracket", C.POINTER(Mlp)),
+        ("extent", _i32p),
     ]
 
 
 RESULT_SCALARS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half")
 RESULT_VEC3 = ("eigs", "Sbar", "Sbar2", "gradpk", "gradpkf", "gradpkrf")
-RESULT_INT = ("nshells_used", "crossing_shell")
+RESULT_INT = ("nshells_used", "crossing_shell", "bracket")
 
 
 class Results(C.Structure):
@@ -215,6 +216,7 @@ class Results(C.Structure):
         ("zvir_half", _f64p),
         ("nshells_used", _i32p),
         ("crossing_shell", _i32p),
+        ("bracket", _i32p),
     ]
 
 

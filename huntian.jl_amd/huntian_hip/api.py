@@ -362,7 +362,7 @@ def alloc_results(npk: int, device="cuda"):
               for k in abi.RESULT_VEC3})
     r["strain"] = torch.empty((max(npk, 1), 3, 3), dtype=torch.float64, device=device)
     for k in abi.RESULT_INT:
-        r[k] = torch.empty(max(npk, 1), dtype=torch.int32, device=device)
+        r[k] = torch.zeros(max(npk, 1), dtype=torch.int32, device=device)
     return r
 
 
@@ -395,27 +395,31 @@ def _batch_struct(pi, pj, pk, Rf, ir2min, ZZon, fcrit, ZZon_pp, fcrit_pp, dev):
     return b, [pi_, pj_, pk_, Rf_, ir_, zz, fc]
 
 
-def _params(alatt, nbuff, variant, early_exit, order, bracket, precision):
+def _params(alatt, nbuff, variant, early_exit, order, bracket, precision, extent=None):
     net = None
     if order == "learned":
         if bracket is None:
             raise ValueError("order='learned' needs a bracket network")
         net = bracket.struct()
+    if order == "given" and extent is None:
+        raise ValueError("order='given' needs the extents (extent=...)")
     return abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)),
                       abi.ORDERS[order], abi.PRECISIONS[precision],
-                      C.pointer(net) if net is not None else None)
+                      C.pointer(net) if net is not None else None,
+                      abi.as_ptr(_ptr(extent), C.c_int32))
 
 
 def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct: dict, *,
                   alatt: float, ZZon: float, fcrit: float, nbuff: int = 0, variant="C",
                   early_exit: bool = True, mask=None, ZZon_pp=None, fcrit_pp=None,
                   results=None, order: str = "crossing", bracket=None,
-                  precision: str = "fp64"):
+                  precision: str = "fp64", extent=None):
     """hnt_shell_analyse over one batch of peaks (device arrays or numpy).  Returns a dict of
     device tensors: scalars [np], vectors [np, 3], strain [np, 3, 3] ([p, L, K]).
 
-    order: wavefront schedule ("crossing", "plain" or "learned" with `bracket`, a
-    huntian_hip.bracket.BracketNet); it changes the speed, never a result.
+    order: wavefront schedule ("crossing", "plain", "learned" with `bracket`, a
+    huntian_hip.bracket.BracketNet, or "given" with `extent`, a device int32 [np] of gather
+    extents); it changes the speed, never a result.
     precision: per-shell sums in "fp64" (the reference CPU arithmetic) or "fp32_kahan"."""
     torch = _require_gpu()
     ctx = context()
@@ -423,7 +427,8 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     npk = int(len(pi))
     dev = torch.device("cuda", ctx.device)
     b, keep_b = _batch_struct(pi, pj, pk, Rf, ir2min, ZZon, fcrit, ZZon_pp, fcrit_pp, dev)
-    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
+    ext = _dev1(extent, torch.int32, dev) if extent is not None else None
+    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision, ext)
     r = results if results is not None else alloc_results(npk, dev)
     rs = _results_struct(r)
     cs, keep_t = _ctab_struct(ct)
@@ -432,7 +437,7 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     st = ctx.lib.hnt_shell_analyse(ctx.handle, C.byref(fs), C.byref(b), C.byref(ts),
                                    C.byref(cs), C.byref(prm), C.byref(rs), m)
     abi.check(ctx.lib, st)
-    del keep_f, keep_t, keep_b
+    del keep_f, keep_t, keep_b, ext
     if npk == 0:
         return {k: v[:0] for k, v in r.items()}
     return r

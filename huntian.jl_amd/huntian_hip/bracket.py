@@ -1,14 +1,18 @@
-"""The learned shell-depth bracket: features, network, training (all on the GPU).
+"""The learned RTHL bracket: features, network, training (all on the GPU).
 
 No counterpart in the reference.  The network (18 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 on
 the matrix cores with FP32 accumulation) reads 18 features of a peak's first 12 shells and
-predicts log(1 + crossing shell), log(1 + gather extent in shells) and a collapse logit.
-hnt_shell_analyse(order="learned") sorts peaks by the predicted extent so that a
-wavefront's lanes finish their gathers together; each lane still stops by the exact
-early-exit rule, so the prediction never changes a result.
+proposes an RTHL bracket and a collapse probability: log(1 + lo), lo the shell holding RTHL;
+log(1 + hi), hi the last shell the exact search reads (the gather extent); the collapse
+logit.  hnt_shell_analyse(order="learned") orders the peaks by hi so that a wavefront's
+lanes finish their gathers together, then runs the exact search inside the bracket: the
+gather until the exact stop rule holds, the walk from the crossing down to lo - 2 first; a
+peak whose gather runs past hi + 2 or whose walk passes lo - 2 without collapsing fails
+verification and is finished by the full scan.  The plain method stays the judge: results
+never depend on the prediction.  hnt_results.bracket records each peak's verification.
 
-Training targets come from the plain method itself: crossing_shell, nshells_used and
-RTHL > 0 of hnt_shell_analyse.
+Training targets come from the plain method itself: the shell holding RTHL (1 without a
+collapse), nshells_used and RTHL > 0 of hnt_shell_analyse.
 """
 from __future__ import annotations
 
@@ -56,14 +60,25 @@ def init_params(seed: int = 0) -> np.ndarray:
     return flatten(d)
 
 
-def targets(result: dict, npk: int | None = None):
-    """(log1p crossing shell, log1p gather extent, collapsed) from a plain-method result
-    dict of device tensors -> device float32 [npk, 3]."""
+def rthl_shell(RTHL, tables):
+    """The shell holding each RTHL (first shell s with rad[s] >= RTHL, 1-based), 1 where there
+    is no collapse; RTHL a device float64 tensor in lattice units."""
+    import torch
+
+    rad = torch.sqrt(torch.as_tensor(np.asarray(tables.shell_r2, np.float64),
+                                     device=RTHL.device))
+    s = torch.searchsorted(rad, RTHL.contiguous(), right=False) + 1
+    return torch.where(RTHL > 0, s, torch.ones_like(s))
+
+
+def targets(result: dict, tables, npk: int | None = None):
+    """(log1p RTHL shell, log1p gather extent, collapsed) from a plain-method result dict of
+    device tensors -> device float32 [npk, 3]."""
     import torch
 
     n = npk if npk is not None else len(result["RTHL"])
     t = torch.empty((n, NO), dtype=torch.float32, device=result["RTHL"].device)
-    t[:, 0] = torch.log1p(result["crossing_shell"][:n].float())
+    t[:, 0] = torch.log1p(rthl_shell(result["RTHL"][:n], tables).float())
     t[:, 1] = torch.log1p(result["nshells_used"][:n].float())
     t[:, 2] = (result["RTHL"][:n] > 0).float()
     return t

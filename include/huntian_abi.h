@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define HNT_ABI_VERSION 1
+#define HNT_ABI_VERSION 2
 
 typedef enum {
     HNT_OK = 0,
@@ -240,18 +240,30 @@ typedef struct {
     int32_t precision;         /* per-shell sums: HNT_PREC_FP64 (0, the reference CPU arithmetic)
                                   or HNT_PREC_FP32_KAHAN */
     const struct hnt_mlp* bracket;  /* network for HNT_ORDER_LEARNED, else unused */
+    const int32_t* extent;     /* HNT_ORDER_GIVEN: device [total peaks of the call, tiles in
+                                  order] gather extent (last shell) per peak, e.g. the
+                                  nshells_used of an earlier call; else unused */
 } hnt_params;
 
 /* Per-shell sums of the gather.  FP64: FP64 sums of the Float32 terms, as the reference CPU
  * path.  FP32_KAHAN: FP32 sums with Kahan compensation; the radial logic stays FP64. */
 enum { HNT_PREC_FP64 = 0, HNT_PREC_FP32_KAHAN = 1 };
 
-/* Peak schedules of hnt_shell_analyse (early_exit = 1).
+/* Peak schedules of hnt_shell_analyse (early_exit = 1).  None changes a result.
  *  CROSSING: a delta-only pass finds each peak's crossing shell, hence its exact gather
  *            extent; wavefronts hold peaks of equal extent.
  *  PLAIN:    (ir2min, Morton code) order, no extra pass.
- *  LEARNED:  the bracket network predicts the extent from the first 12 shells. */
-enum { HNT_ORDER_CROSSING = 0, HNT_ORDER_PLAIN = 1, HNT_ORDER_LEARNED = 2 };
+ *  LEARNED:  the bracket network reads 18 features of the first 12 shells and proposes
+ *            an RTHL bracket [lo, hi] (lo: the shell holding RTHL, hi: the last shell the
+ *            exact search reads, i.e. the gather extent) and a collapse logit; peaks are
+ *            ordered by hi.  The exact search then runs inside the bracket: every lane
+ *            gathers until the exact stop rule holds, and the walk takes the shells from
+ *            the crossing down to max(lo - 2, 1) first; a gather that runs past hi + 2 or
+ *            a walk that passes lo - 2 without collapsing fails verification and goes on as
+ *            the full scan (hnt_results.bracket records which).
+ *  GIVEN:    peaks ordered by the extents in hnt_params.extent (no extra pass): with the
+ *            exact extents of an earlier call, the best order any bracket can give. */
+enum { HNT_ORDER_CROSSING = 0, HNT_ORDER_PLAIN = 1, HNT_ORDER_LEARNED = 2, HNT_ORDER_GIVEN = 3 };
 
 /* Per-peak results, device arrays of Float64 (caller-allocated, npeaks entries each;
  * vector fields hold 3 consecutive values per peak, strain 9 in column-major order).
@@ -275,6 +287,11 @@ typedef struct {
     int32_t* nshells_used;     /* optional (NULL): shells gathered per peak, incl. the centre */
     int32_t* crossing_shell;   /* optional (NULL): first shell (1-based) past ir2min whose Fbar
                                   is below fcrit, 0 if there is none within the hunt radius */
+    int32_t* bracket;          /* optional (NULL), with one lane per peak (HNT_TUNE_GANG 1):
+                                  HNT_ORDER_LEARNED: bit 0 the gather ended within hi + 2,
+                                  bit 1 the peak collapsed and the shell holding RTHL is
+                                  >= lo - 2 (both set: the bracket verified), bit 2 the
+                                  collapse logit predicted the collapse flag; other orders: 0 */
 } hnt_results;
 
 /*
@@ -546,9 +563,11 @@ hnt_status hnt_kernel_strain(hnt_ctx* ctx, const double* rad, const double* G,
 /* No counterpart in the reference.  An MLP 18 -> 128 -> 128 -> 128 -> 3 with SiLU, run in
  * BF16 on the matrix cores with FP32 accumulation.  Inputs (hnt_bracket_features): mean
  * delta of shells 1..12, (e, p) of the symmetrised psi1 gradient over shells 2..6 and
- * 2..12, mean delta over shells 1..12, Rf / alatt.  Outputs: log(1 + crossing shell),
- * log(1 + gather extent in shells), collapse logit.  Parameters: one FP32 vector, row-major
- * W1 [18x128], b1, W2 [128x128], b2, W3 [128x128], b3, W4 [128x3], b4. */
+ * 2..12, mean delta over shells 1..12, Rf / alatt.  Outputs, the RTHL bracket and the
+ * collapse logit: log(1 + lo), lo the shell holding RTHL (RTHL in (rad[lo - 1], rad[lo]],
+ * 1 for no collapse); log(1 + hi), hi the gather extent in shells; the collapse logit.
+ * Parameters: one FP32 vector, row-major W1 [18x128], b1, W2 [128x128], b2, W3 [128x128],
+ * b3, W4 [128x3], b4. */
 #define HNT_MLP_INPUTS 18
 #define HNT_MLP_HIDDEN 128
 #define HNT_MLP_OUTPUTS 3

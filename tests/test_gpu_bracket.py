@@ -140,4 +140,86 @@ def test_schedules_do_not_change_results(oracle, websky, cosmo48, order):
         out[o]["mask"] = mask.cpu().numpy()
     assert (out["crossing"]["RTHL"] > 0).sum() > 10
     for k in out["crossing"]:
-        assert np.array_equal(out["crossing"][k], out[order][k], equal_nan=True), k
+        if k != "bracket":
+            assert np.array_equal(out["crossing"][k], out[order][k], equal_nan=True), k
+    if order == "learned":
+        # the verification flags follow from the network's own outputs
+        x = bracket.features(f, pk["i"], pk["j"], pk["k"], Rf, stab, 2.0)
+        y = net.forward(x).cpu().numpy()
+        m = stab.nshells - 1
+        dec = lambda v: np.clip(np.floor(np.maximum(np.exp(v) - 1.0, 1.0) + 0.5), 1, m)
+        hi = dec(y[:, 1])
+        fl = out[order]["bracket"]
+        bit0 = (out["crossing"]["nshells_used"] <= hi + 2)
+        bit2 = (y[:, 2] > 0) == (out["crossing"]["RTHL"] > 0)
+        assert np.mean((fl & 1).astype(bool) == bit0) > 0.99
+        assert np.mean(((fl & 4) > 0) == bit2) > 0.99
+        assert np.all(fl[out["crossing"]["RTHL"] <= 0] & 2 == 0)
+    else:
+        assert np.all(out[order]["bracket"] == 0)
+
+
+def test_given_extents_order_like_the_crossing_pass(oracle, websky, cosmo48):
+    """order='given' with the crossing run's own extents (the perfect bracket): same results,
+    no pass of the library's own."""
+    import torch
+
+    f, radii, sm = cosmo48
+    nbuff = 7
+    pk, tot = oracle.peak_find_multi(sm, [1.5] * len(radii), nbuff,
+                                     np.zeros((48, 48, 48), np.int8, order="F"))
+    Rf = np.array([radii[q] for q in pk["filt"]])
+    ir2 = np.array([api.ir2min_of(r, 2.0) for r in Rf], np.int32)
+    stab = api.build_shell_tables(14)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    kw = dict(alatt=2.0, ZZon=1.0, fcrit=fcrit, nbuff=nbuff)
+    r0 = api.shell_analyse(f, pk["i"], pk["j"], pk["k"], Rf, ir2, stab, websky, **kw)
+    ext = r0["nshells_used"].clone()
+    r1 = api.shell_analyse(f, pk["i"], pk["j"], pk["k"], Rf, ir2, stab, websky, order="given",
+                           extent=ext, **kw)
+    for k in r0:
+        assert torch.equal(torch.nan_to_num(r0[k], nan=7.0), torch.nan_to_num(r1[k], nan=7.0)), k
+    with pytest.raises(ValueError):
+        api.shell_analyse(f, pk["i"], pk["j"], pk["k"], Rf, ir2, stab, websky, order="given",
+                          **kw)
+
+
+@pytest.mark.timeout(300)
+def test_trained_bracket_on_its_evaluation_tile(websky):
+    """The committed network (trained on the 128^3 workload's tiles, seeds 1-8) on the seed-99
+    tile of that workload: the learned schedule gives exactly the crossing schedule's
+    results; the share of peaks whose bracket verified is printed."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    w = synth.workload_tile("tile128", dev, seed=99)
+    n, nbuff, nhunt, alatt = w["n"], w["nbuff"], w["nhunt"], w["alatt"]
+    mask = api.fortran_empty((n, n, n), torch.int8, dev)
+    mask.zero_()
+    pf = api.peak_find(w["smoothed"], [1.5] * len(w["radii"]), nbuff, mask,
+                       capacity=(n - 2 * nbuff) ** 3)
+    f = pf["filt"].long()
+    Rf = torch.tensor(w["radii"], dtype=torch.float64, device=dev)[f]
+    ir2 = torch.tensor([api.ir2min_of(R, alatt) for R in w["radii"]], dtype=torch.int32,
+                       device=dev)[f]
+    stab = api.build_shell_tables(nhunt)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    net = bracket.BracketNet.load(bracket.DEFAULT_WEIGHTS, device=dev)
+    out = {}
+    for o in ("crossing", "learned"):
+        mask.zero_()
+        r = api.shell_analyse(w["fields"], pf["i"], pf["j"], pf["k"], Rf, ir2, stab, websky,
+                              alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff, mask=mask,
+                              order=o, bracket=net)
+        out[o] = ({k: v.clone() for k, v in r.items()}, mask.clone())
+    for k in out["crossing"][0]:
+        if k != "bracket":
+            assert torch.equal(torch.nan_to_num(out["crossing"][0][k], nan=7.0),
+                               torch.nan_to_num(out["learned"][0][k], nan=7.0)), k
+    assert torch.equal(out["crossing"][1], out["learned"][1])
+    fl = out["learned"][0]["bracket"].cpu().numpy()
+    halo = out["crossing"][0]["RTHL"].cpu().numpy() > 0
+    print(f"seed 99, {len(fl)} candidates, {halo.sum()} halos: gather verified "
+          f"{(fl & 1).astype(bool).mean():.3f}, walk verified {((fl & 2) > 0)[halo].mean():.3f} "
+          f"of halos, collapse logit right {((fl & 4) > 0).mean():.3f}", flush=True)
+    assert len(fl) > 5000 and (fl & 1).astype(bool).mean() > 0.5
