@@ -66,9 +66,13 @@ def main():
                     help="skip timing tile peak finding from the raw tile (smoothing included)")
     ap.add_argument("--no-field-synthesis", action="store_true",
                     help="skip timing the tile field synthesis")
+    ap.add_argument("--no-subworkloads", action="store_true",
+                    help="skip the 512^3 tile, the 64 batched tiles and the learned-bracket "
+                         "tile reported beside the headline")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain 4 GiB device copy (3 times), the reference kernel "
-                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+                    help="only run a read-only reduction of a 4 GiB array and a plain 4 GiB "
+                         "device copy (3 times each), the reference kernels for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE and TCC_EA0_RDREQ_* counts")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the library has "
@@ -85,10 +89,14 @@ def main():
     if args.copy_calibration:
         a = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
+        tot = 0.0
+        for _ in range(3):
+            tot += float(a.sum())  # read-only: every byte of a once
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_per_dispatch": a.numel() * 4}), flush=True)
+        print(json.dumps({"calibration_bytes_read_per_pass": a.numel() * 4,
+                          "sum": tot}), flush=True)
         return
 
     cfg = CONFIGS[args.config]
@@ -194,32 +202,12 @@ def main():
     ms_step = total_ms / args.steps
     log(f"timed {args.steps} steps: {ms_step:.2f} ms/step, {npk} candidates; stages {prof}")
 
-    # ---------------- schedule ablation: shell analysis alone under each schedule
+    # ---------------- schedule ablation: shell analysis alone under each schedule, every
+    # result field compared with the step's (the bench's default schedule)
     ablation = {}
     if not args.no_ablation:
-        Rf = Rf_tab[pf["filt"].long()]
-        ir2 = ir2_tab[pf["filt"].long()]
-        for o in ("crossing", "plain", "learned"):
-            if o == "learned" and net is None:
-                continue
-            res2 = api.alloc_results(npk, dev)
-            for rep in range(2):  # first call warms the schedule's buffers
-                mask.zero_()
-                torch.cuda.synchronize()
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-                r2 = api.shell_analyse(fields, pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct,
-                                       alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff,
-                                       variant=args.variant, early_exit=True, mask=mask,
-                                       results=res2, order=o, bracket=net)
-                e1.record()
-                torch.cuda.synchronize()
-            same = bool(torch.equal(r2["RTHL"][:npk], r["RTHL"][:npk]) and
-                        torch.equal(r2["nshells_used"][:npk], r["nshells_used"][:npk]))
-            ablation[o] = dict(ms=round(e0.elapsed_time(e1), 3), same_results=same)
-            del r2, res2
-            log(f"schedule {o}: {ablation[o]}")
+        ablation = schedule_ablation(args, ctx, fields, pf, r, npk, Rf_tab, ir2_tab, stab, ct,
+                                     alatt, fcrit, nbuff, mask, net, dev)
 
     # ---------------- halo records of the step's results (hnt_pack_records: 33-float
     # ExtHaloRecord with the 2LPT terms, z = 0), timed once after a warm call
@@ -331,7 +319,9 @@ def main():
     pf_bytes = float(len(radii)) * m ** 3 * 4 + 2.0 * m ** 3  # fields + mask read + claims
     stage_ms = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
     t_pf = stage_ms["pf_mark"] + stage_ms["pf_compact"]
-    t_sa = stage_ms["sa_prep"] + stage_ms["sa_kernel"]
+    # shell analysis = validation, repack, first sort, crossing pass and work-ordered sort,
+    # gather + radial logic
+    t_sa = stage_ms["sa_prep"] + stage_ms.get("sa_cross", 0.0) + stage_ms["sa_kernel"]
     kern = {
         "shell_analyse_kernel": dict(ms=stage_ms["sa_kernel"], bytes=sa_bytes),
         "pf_mark": dict(ms=stage_ms["pf_mark"], bytes=float(len(radii)) * m ** 3 * 4 + m ** 3),
@@ -356,40 +346,12 @@ def main():
                 traffic_note=traffic_note, kernel=dom, kernel_ms=round(kern[dom]["ms"], 3),
                 algorithmic_bytes=kern[dom]["bytes"])
 
-    # ---------------- CPU baseline: oracle, variant A (the reference CPU path), sample
-    cpu = None
+    # ---------------- CPU baseline: oracle, variant A (the reference CPU path), sample; then
+    # the oracle in the GPU's variant on the same sample, checked against the GPU's results
+    cpu, oracle_check = None, None
     if args.cpu_seconds > 0:
-        from oracle import oracle as O
-
-        cores = min(16, len(os.sched_getaffinity(0)))
-        host = {k: v.cpu().numpy() for k, v in fields.items()}
-        pi = pf["i"].cpu().numpy()
-        pj = pf["j"].cpu().numpy()
-        pk_ = pf["k"].cpu().numpy()
-        Rf_h = np.array(radii)[pf["filt"].cpu().numpy()]
-        ir2_h = np.array([api.ir2min_of(R, alatt) for R in radii], np.int32)[pf["filt"].cpu().numpy()]
-        t_oracle = O.shell_tables(nhunt)
-        rng = np.random.default_rng(0)
-        order = rng.permutation(npk)
-        done, elapsed, nsamp = 0, 0.0, max(cores * 4, 64)
-        while elapsed < args.cpu_seconds and done < npk:
-            sel = order[done:done + nsamp]
-            t1 = time.time()
-            O.analyse_peaks(host, pi[sel], pj[sel], pk_[sel], Rf_h[sel], ir2_h[sel], t_oracle, ct,
-                            alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff, variant="A",
-                            early_exit=False, nthreads=cores)
-            dt = time.time() - t1
-            elapsed += dt
-            done += len(sel)
-            rate = len(sel) / max(dt, 1e-9)
-            nsamp = int(min(npk - done, max(16, rate * (args.cpu_seconds - elapsed))))
-            if nsamp <= 0:
-                break
-        cpu = dict(value=round(done / elapsed, 2), unit="peaks/s", cores=cores, kind="port",
-                   sample=f"{done} of {npk} candidates drawn uniformly (seed 0); FP64 oracle of "
-                          "RadialShell.analyse_peak (variant A, full hunt radius, no early "
-                          f"exit), OpenMP over peaks; {elapsed:.1f} s")
-        log(f"cpu baseline: {cpu}")
+        cpu, oracle_check = cpu_baseline_and_check(args, fields, pf, r, npk, radii, alatt,
+                                                   nhunt, ct, fcrit, nbuff)
 
     # ---------------- tile field synthesis of one tile of this size (the step before the
     # path), after the step's buffers are freed
@@ -403,6 +365,21 @@ def main():
         torch.cuda.empty_cache()
         synth_t = time_field_synthesis(n, box, nbuff, dev)
         log(f"field synthesis: {synth_t}")
+
+    # ---------------- the other workloads, each on its own after the headline's buffers are
+    # released: the 512^3 tile, the 64 uneven tiles (per-tile calls, one launch, hipGraph
+    # plan), and the learned bracket on its evaluation tile (128^3, seed 99)
+    subs = None
+    if not args.no_subworkloads and args.config == "tile1024":
+        fields.clear()
+        smoothed.clear()
+        mask = r = pf = results = None
+        ctx.release_scratch()
+        torch.cuda.empty_cache()
+        subs = {"tile512": tile_result(args, "tile512"),
+                "batched64": batched_result(args, "batched64"),
+                "tile128_seed99_learned_bracket": tile_result(args, "tile128", seed=99,
+                                                              net=net, ablate=True)}
 
     out = {
         "metric": METRIC,
@@ -432,6 +409,8 @@ def main():
                    "merge": merge,
                    "finalize": finalize,
                    "field_synthesis": synth_t,
+                   "sub_workloads": subs,
+                   "oracle_check": oracle_check,
                    "shell_only_peaks_per_s": round(npk / (t_sa * 1e-3), 1) if t_sa > 0 else None,
                    "mean_cells_gathered": round(float(cells.mean()), 1) if npk else 0.0,
                    "mean_cells_reference_extent": round(float(ref_cells.mean()), 1)
@@ -443,6 +422,180 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+RESULT_FIELDS = ("RTHL", "Fbarx", "e_v", "p_v", "strain", "eigs", "Srb", "Sbar", "Sbar2",
+                 "gradpk", "gradpkf", "gradpkrf", "d2F", "zvir_half", "nshells_used",
+                 "crossing_shell")
+
+
+def same_results(a, b, npk):
+    """Every result field of two shell analyses equal bit for bit (NaN where both are NaN)."""
+    import torch
+
+    for k in RESULT_FIELDS:
+        x, y = a[k][:npk], b[k][:npk]
+        if x.dtype.is_floating_point:
+            x, y = torch.nan_to_num(x, nan=7.0), torch.nan_to_num(y, nan=7.0)
+        if not torch.equal(x, y):
+            return False
+    return True
+
+
+def schedule_ablation(args, ctx, fields, pf, r, npk, Rf_tab, ir2_tab, stab, ct, alatt, fcrit,
+                      nbuff, mask, net, dev):
+    """Shell analysis alone (validation, repack, schedule, gather + radial logic) under each
+    schedule, on the step's candidates: crossing (delta-only pass for the exact extents),
+    plain (ir2min, Morton), learned (the bracket network; its verification rates), perfect
+    (the exact extents of the step given as the bracket: the best any bracket can do here,
+    no pass of its own).  Each timed on a warm second call."""
+    import torch
+
+    from huntian_hip import api
+
+    Rf = Rf_tab[pf["filt"].long()]
+    ir2 = ir2_tab[pf["filt"].long()]
+    extent = r["nshells_used"][:npk].clone()
+    out = {}
+    for o in ("crossing", "plain", "learned", "perfect"):
+        if o == "learned" and net is None:
+            continue
+        res2 = api.alloc_results(npk, dev)
+        kw = dict(order="given", extent=extent) if o == "perfect" else dict(order=o, bracket=net)
+        for rep in range(2):  # first call warms the schedule's buffers
+            mask.zero_()
+            ctx.set_profiling(True)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            r2 = api.shell_analyse(fields, pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct,
+                                   alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff,
+                                   variant=args.variant, early_exit=True, mask=mask,
+                                   results=res2, **kw)
+            e1.record()
+            torch.cuda.synchronize()
+            prof = ctx.profile()
+            ctx.set_profiling(False)
+        ent = dict(ms=round(e0.elapsed_time(e1), 3), same_results=same_results(r2, r, npk),
+                   stage_ms={k: round(v[0], 3) for k, v in prof.items() if v[1]})
+        if o == "learned":
+            fl = r2["bracket"][:npk]
+            halo = r["RTHL"][:npk] > 0
+            ent["bracket"] = dict(
+                gather_verified=round(float(((fl & 1) > 0).float().mean()), 4),
+                walk_verified_of_halos=round(float(((fl & 2) > 0)[halo].float().mean()), 4),
+                both_verified_of_halos=round(float(((fl & 3) == 3)[halo].float().mean()), 4),
+                collapse_logit_right=round(float(((fl & 4) > 0).float().mean()), 4))
+        out[o] = ent
+        del r2, res2
+        log(f"schedule {o}: {ent}")
+    if "learned" in out and "plain" in out:
+        out["learned_over_plain_speedup"] = round(out["plain"]["ms"] / out["learned"]["ms"], 3)
+    if "perfect" in out:
+        out["perfect_over_plain_speedup"] = round(out["plain"]["ms"] / out["perfect"]["ms"], 3)
+    out["note"] = ("shell analysis alone on the step's candidates, same results required; "
+                   "plain = exact early exit in (ir2min, Morton) order; perfect = the step's "
+                   "own extents given as the bracket")
+    return out
+
+
+def host_info():
+    """Cores this process may use, nproc, the machine's CPU count and model."""
+    import subprocess
+
+    info = dict(affinity=len(os.sched_getaffinity(0)), cpu_count=os.cpu_count())
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                           timeout=10).stdout.strip())
+    except Exception:
+        info["nproc"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return info
+
+
+def cpu_baseline_and_check(args, fields, pf, r, npk, radii, alatt, nhunt, ct, fcrit, nbuff):
+    """(cpu_baseline, oracle_check).  The baseline: the oracle's FP64 restatement of
+    RadialShell.analyse_peak (variant A, full hunt radius, no early exit), OpenMP over peaks
+    on every core this process may use, on a uniform random sample of the step's candidates
+    sized to about --cpu-seconds.  The check: the oracle in the GPU's variant with early exit
+    on the same sample against the GPU's results of the step (halo flags, worst relative
+    error per field)."""
+    import numpy as np
+
+    from huntian_hip import api
+    from oracle import oracle as O
+
+    host = host_info()
+    cores = host["affinity"]
+    hf = {k: v.cpu().numpy() for k, v in fields.items()}
+    pi = pf["i"].cpu().numpy()
+    pj = pf["j"].cpu().numpy()
+    pk_ = pf["k"].cpu().numpy()
+    filt = pf["filt"].cpu().numpy()
+    Rf_h = np.array(radii)[filt]
+    ir2_h = np.array([api.ir2min_of(R, alatt) for R in radii], np.int32)[filt]
+    t_oracle = O.shell_tables(nhunt)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(npk)
+    done, elapsed, nsamp = 0, 0.0, max(cores * 4, 64)
+    while elapsed < args.cpu_seconds and done < npk:
+        sel = order[done:done + nsamp]
+        t1 = time.time()
+        O.analyse_peaks(hf, pi[sel], pj[sel], pk_[sel], Rf_h[sel], ir2_h[sel], t_oracle, ct,
+                        alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff, variant="A",
+                        early_exit=False, nthreads=cores)
+        dt = time.time() - t1
+        elapsed += dt
+        done += len(sel)
+        rate = len(sel) / max(dt, 1e-9)
+        nsamp = int(min(npk - done, max(16, rate * (args.cpu_seconds - elapsed))))
+        if nsamp <= 0:
+            break
+    cpu = dict(value=round(done / elapsed, 2), unit="peaks/s", cores=cores, kind="port",
+               host=host,
+               sample=f"{done} of {npk} candidates drawn uniformly (seed 0); FP64 oracle of "
+                      "RadialShell.analyse_peak (variant A, full hunt radius, no early exit), "
+                      f"OpenMP over peaks on {cores} threads; {elapsed:.1f} s")
+    log(f"cpu baseline: {cpu}")
+    # the check, on the same sample and a larger one (early exit makes it cheap)
+    nchk = min(npk, max(done, 200_000))
+    sel = np.sort(order[:nchk])
+    t1 = time.time()
+    ref = O.analyse_peaks(hf, pi[sel], pj[sel], pk_[sel], Rf_h[sel], ir2_h[sel], t_oracle, ct,
+                          alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff,
+                          variant=args.variant, early_exit=True, nthreads=cores)
+    t_chk = time.time() - t1
+    import torch
+
+    sel_t = torch.as_tensor(sel, device=r["RTHL"].device)
+    g = {k: v[sel_t].cpu().numpy() for k, v in r.items() if k in ref}
+    g["strain"] = np.swapaxes(g["strain"], 1, 2)
+    halo_g, halo_r = g["RTHL"] > 0, ref["RTHL"] > 0
+    worst = {}
+    both = halo_g & halo_r
+    for k in ("RTHL", "Fbarx", "e_v", "p_v", "strain", "eigs", "Srb", "Sbar", "Sbar2", "gradpk",
+              "gradpkf", "gradpkrf", "d2F", "zvir_half"):
+        a = g[k][both].reshape(int(both.sum()), -1)
+        b = ref[k][both].reshape(int(both.sum()), -1)
+        fin = np.isfinite(a) & np.isfinite(b)
+        scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-30)
+        err = np.where(fin, np.abs(a - b) / np.maximum(np.abs(b), 1e-9 * scale), 0.0)
+        worst[k] = float(err.max()) if err.size else 0.0
+    check = dict(candidates=int(nchk), halos_gpu=int(halo_g.sum()), halos_oracle=int(halo_r.sum()),
+                 halo_flags_equal=bool(np.array_equal(halo_g, halo_r)),
+                 nshells_used_equal=bool(np.array_equal(g["nshells_used"], ref["nshells_used"])),
+                 worst_rel_error={k: float(f"{v:.3g}") for k, v in worst.items()},
+                 oracle=f"FP64 oracle, variant {args.variant}, early exit, {nchk} sampled "
+                        f"candidates (seed 0), {t_chk:.1f} s on {cores} threads")
+    log(f"oracle check: {check}")
+    return cpu, check
 
 
 def time_field_synthesis(n, box, nbuff, dev):
@@ -497,19 +650,19 @@ def time_field_synthesis(n, box, nbuff, dev):
                      f"from {M}^3, 2LPT (2 R2C + 9 C2R at {n}^3), Laplacian (1 + 1)")
 
 
-def run_batched(args, cfg):
+def batched_result(args, cfg_name):
     """Uneven tiles: shell analysis of all candidates of 64 tiles, (a) one library call per
     tile as the reference drivers do, (b) one persistent launch over all tiles, (c) the same
-    recorded as a hipGraph plan and replayed.  Peak finding runs per tile, untimed."""
-    import numpy as np
+    recorded as a hipGraph plan and replayed.  Peak finding runs per tile, untimed.  Returns
+    the timings and whether the three give the same results."""
     import torch
 
     from huntian_hip import api, synth
     from huntian_hip.homeltab import WEBSKY_TABLE, read_homeltab
 
+    cfg = CONFIGS[cfg_name]
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(0)
-    specs = synth.batched_specs(args.config)
+    specs = synth.batched_specs(cfg_name)
     nt = len(specs)
     sides = [s for s, _, _ in specs]
     nbuffs = [b for _, b, _ in specs]
@@ -521,7 +674,6 @@ def run_batched(args, cfg):
     Rf_tab = torch.tensor(radii, dtype=torch.float64, device=dev)
     ir2_tab = torch.tensor([api.ir2min_of(R, alatt) for R in radii], dtype=torch.int32,
                            device=dev)
-    ctx = api.context(0)
     tiles, counts = [], []
     t0 = time.time()
     for n, nb, seed in specs:
@@ -543,50 +695,135 @@ def run_batched(args, cfg):
         f"(per tile {min(counts)}..{max(counts)}), setup {time.time() - t0:.1f} s")
     kw = dict(alatt=alatt, ZZon=1.0, fcrit=fcrit, variant=args.variant)
     nbuff_all = int(min(nbuffs))  # one mask buffer width for the batched call
+    per = [None] * nt
+    batch = [None]
 
     def per_tile():
-        for t in tiles:
-            api.shell_analyse(t["fields"], t["pi"], t["pj"], t["pk"], t["Rf"], t["ir2min"],
-                              stab, ct, nbuff=t["nbuff"], mask=t["mask"], **kw)
+        for q, t in enumerate(tiles):
+            per[q] = api.shell_analyse(t["fields"], t["pi"], t["pj"], t["pk"], t["Rf"],
+                                       t["ir2min"], stab, ct, nbuff=nbuff_all, mask=t["mask"],
+                                       **kw)
 
     def batched():
-        api.shell_analyse_tiles(tiles, stab, ct, nbuff=nbuff_all, **kw)
+        batch[0] = api.shell_analyse_tiles(tiles, stab, ct, nbuff=nbuff_all, **kw)
 
     plan = api.ShellPlan(tiles, stab, ct, nbuff=nbuff_all, **kw)
     modes = {"per_tile_calls": per_tile, "one_launch": batched, "hipgraph_plan": plan.launch}
     ms = {}
+    steps = max(1, args.steps)
     for name, fn in modes.items():
-        for _ in range(args.warmup):
+        for _ in range(max(1, args.warmup)):
             fn()
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         e1.record()
         torch.cuda.synchronize()
-        ms[name] = e0.elapsed_time(e1) / args.steps
+        ms[name] = e0.elapsed_time(e1) / steps
         log(f"{name}: {ms[name]:.2f} ms per pass")
+    same = all(same_results(per[q], batch[0][q], counts[q]) and
+               same_results(per[q], plan.results[q], counts[q]) for q in range(nt))
     plan.close()
     best = min(ms, key=ms.get)
+    res = dict(
+        workload=f"{nt} tiles, sides from {list(cfg['sides'])}, nbuff "
+                 f"{cfg['nbuff'][0]}-{cfg['nbuff'][1]}, seeds {cfg['seeds'][0]}-"
+                 f"{cfg['seeds'][1]}, {cfg['nfilt']} top-hat filters {cfg['rmin']}-"
+                 f"{cfg['rmax']} Mpc/h, hunt radius {nhunt}",
+        candidates=total, candidates_per_tile=[min(counts), max(counts)],
+        ms_per_pass={k: round(v, 3) for k, v in ms.items()},
+        best_mode=best, peaks_per_s=round(total / (ms[best] * 1e-3), 1),
+        speedup_vs_per_tile_calls=round(ms["per_tile_calls"] / ms[best], 3),
+        same_results=bool(same))
+    del tiles, plan, per, batch
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_batched(args, cfg):
+    """--config batched64: the batched workload as the JSON line."""
+    res = batched_result(args, args.config)
     out = {
         "metric": "peaks resolved/s, 64 uneven tiles, shell analysis (batched)",
-        "value": round(total / (ms[best] * 1e-3), 1), "unit": "peaks/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms[best], 3),
+        "value": res["peaks_per_s"], "unit": "peaks/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["ms_per_pass"][res["best_mode"]],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (sums of f32 inputs and f32 products, as the reference CPU path)",
         "data": "synthetic",
-        "config": {"workload": f"{nt} tiles, sides from {list(cfg['sides'])}, nbuff "
-                               f"{cfg['nbuff'][0]}-{cfg['nbuff'][1]}, seeds "
-                               f"{cfg['seeds'][0]}-{cfg['seeds'][1]}, 16 top-hat filters "
-                               f"{cfg['rmin']}-{cfg['rmax']} Mpc/h, hunt radius {nhunt}",
-                   "candidates": total, "candidates_per_tile": [min(counts), max(counts)],
-                   "ms_per_pass": {k: round(v, 3) for k, v in ms.items()},
-                   "speedup_vs_per_tile_calls": round(ms["per_tile_calls"] / ms[best], 3),
-                   "best_mode": best, "parallelism": "1 GPU"},
+        "config": dict(res, parallelism="1 GPU"),
     }
     print(json.dumps(out), flush=True)
+
+
+def tile_result(args, name, seed=None, net=None, ablate=False):
+    """One tile workload end to end on the GPU: peak finding over the pre-smoothed fields,
+    then the shell analysis of its candidates (crossing schedule), each timed on a warm call;
+    with ablate, the shell analysis under every schedule (schedule_ablation)."""
+    import torch
+
+    from huntian_hip import api, synth
+    from huntian_hip.homeltab import WEBSKY_TABLE, read_homeltab
+
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    w = synth.workload_tile(name, dev, seed=seed)
+    n, nbuff, nhunt, alatt = w["n"], w["nbuff"], w["nhunt"], w["alatt"]
+    radii = w["radii"]
+    ct = read_homeltab(WEBSKY_TABLE)
+    fcrit = api.fsc_of_z(ct, 0.0)
+    stab = api.build_shell_tables(nhunt)
+    Rf_tab = torch.tensor(radii, dtype=torch.float64, device=dev)
+    ir2_tab = torch.tensor([api.ir2min_of(R, alatt) for R in radii], dtype=torch.int32,
+                           device=dev)
+    mask = api.fortran_empty((n, n, n), torch.int8, dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        out = fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return out, ev[0].elapsed_time(ev[1])
+
+    def pfind():
+        mask.zero_()
+        return api.peak_find(w["smoothed"], [1.5] * len(radii), nbuff, mask,
+                             capacity=(n - 2 * nbuff) ** 3)
+    pf, t_pf = timed(pfind)
+    npk = len(pf["i"])
+    Rf = Rf_tab[pf["filt"].long()]
+    ir2 = ir2_tab[pf["filt"].long()]
+    res = api.alloc_results(npk, dev)
+
+    def shells():
+        mask.zero_()
+        return api.shell_analyse(w["fields"], pf["i"], pf["j"], pf["k"], Rf, ir2, stab, ct,
+                                 alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff,
+                                 variant=args.variant, mask=mask, results=res)
+    r, t_sa = timed(shells)
+    out = dict(workload=f"{name}: {n}^3, box {w['box']:g} Mpc/h, seed {w['seed']}, "
+                        f"{len(radii)} top-hat filters {w['rmin']}-{w['rmax']} Mpc/h, nbuff "
+                        f"{nbuff}, hunt radius {nhunt} ({stab.nshells} shells)",
+               candidates=npk, halos=int((r["RTHL"][:npk] > 0).sum()),
+               peak_find_ms=round(t_pf, 3), shell_analysis_ms=round(t_sa, 3),
+               peaks_per_s=round(npk / ((t_pf + t_sa) * 1e-3), 1),
+               shell_only_peaks_per_s=round(npk / (t_sa * 1e-3), 1))
+    if ablate:
+        ctx = api.context(0)
+        out["schedule_ablation_shell_ms"] = schedule_ablation(
+            args, ctx, w["fields"], pf, r, npk, Rf_tab, ir2_tab, stab, ct, alatt, fcrit,
+            nbuff, mask, net, dev)
+    log(f"{name}: {out} ({time.time() - t0:.1f} s)")
+    del w, pf, r, res, mask
+    api.context(0).release_scratch()
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":
