@@ -67,6 +67,8 @@ struct SAArgs {
     int64_t slot_doubles;
     const int32_t* stop_hint;  // exact stop shell per global peak id (crossing pass) or null
     const float* pred;         // bracket network outputs per global peak id (learned order)
+    double rmax2rs;            // > 0: each peak's shells limited to r^2 <= (rmax2rs Rf/alatt)^2
+    const int8_t* skip;        // per global peak id: 1 = no collapse without analysis, or null
     unsigned long long* qctr;
     int64_t qbeg[8], qend[8];
 };
@@ -168,6 +170,11 @@ __global__ void group_tile_kernel(const int64_t* __restrict__ tile_pad, int ntil
 __global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) p[q] = v;
+}
+
+__global__ void fill_i8(int8_t* p, int64_t n, int8_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Flags peaks outside the grid or with ZZon <= -1 (bit 0 / bit 1 of *bad).
@@ -305,10 +312,27 @@ __device__ __forceinline__ int closest_shell(const int32_t* r2, int lo, int hi, 
 // ShellAnalysisGPU.jl:233-266 for B/C): the first shell whose r^2 is nearest to
 // (Rf/alatt)^2, if within 10, recorded two shells below (A; one below for r^2 = 1) or one
 // below (B, C).
-__device__ void rf_window(const SAArgs& a, double Rf, int& mrf, int& mlow_rf, int& mupp_rf) {
+// The last shell a peak's analysis uses: NS - 1 (the last shell is never finalised), or with
+// rmax2rs > 0 the number of leading shells of 1 .. NS - 1 with r^2 <= (rmax2rs Rf / alatt)^2
+// (ShellAnalysisGPU.jl:89-97, ext/CUDAExt.jl:2176-2185).
+__device__ __forceinline__ int peak_last_shell(const SAArgs& a, double Rf) {
+    if (!(a.rmax2rs > 0.0)) return a.NS - 1;
+    double lim = a.rmax2rs * Rf / a.alatt;
+    lim = lim * lim;
+    int lo = 1, hi = a.NS;  // first shell s in [1, NS - 1] with r^2 > lim, else NS
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (!((double)a.shell_r2[mid - 1] <= lim)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo - 1 < a.NS - 1 ? lo - 1 : a.NS - 1;
+}
+
+__device__ void rf_window(const SAArgs& a, double Rf, int& mrf, int& mlow_rf, int& mupp_rf,
+                          int m_last = -1) {
     const double* __restrict__ rad = a.rad;
     const int NS = a.NS;
-    const int m = NS - 1;
+    const int m = m_last > 0 ? m_last : NS - 1;
     mrf = 1;
     const double rf = Rf / a.alatt;
     const double rf2 = rf * rf;
@@ -494,11 +518,12 @@ __device__ void write_no_collapse(const TileDev& T, int p) {
 // walk.  *lo_true (if not NULL) receives the shell holding RTHL, 0 without a collapse.
 __device__ void post_process(const SAArgs& a, const TileDev& T, int p, const double* __restrict__ P, int ci, int cj,
                              int ck, double ZZon, double fcrit, int ir2min, int mrf,
-                             int mlow_rf, int mupp_rf, int walk_lo = 1, int* lo_true = nullptr) {
+                             int mlow_rf, int mupp_rf, int walk_lo = 1, int* lo_true = nullptr,
+                             int m_last = -1) {
     if (lo_true) *lo_true = 0;
     const double* __restrict__ rad = a.rad;
     const int NS = a.NS;
-    const int m = NS - 1;
+    const int m = m_last > 0 ? m_last : NS - 1;
     const int variant = a.variant;
     auto Fb = [&](int s) -> double { return P[(int64_t)s * PW + K_F]; };
 
@@ -790,9 +815,18 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
     const double* __restrict__ rad = a.rad;
     const int NS = a.NS;
     const int m = NS - 1;
+    // this lane's last shell (fewer with rmax2rs > 0), and peaks that get no analysis
+    const int m_lane = valid ? peak_last_shell(a, Rf) : m;
+    bool none = valid && (m_lane < 2 || (a.skip && a.skip[T.base + p]));
+    if (none) {
+        write_no_collapse(T, p);
+        if (T.res.nshells_used) T.res.nshells_used[p] = 0;
+        if (T.res.crossing_shell) T.res.crossing_shell[p] = 0;
+        if (T.res.bracket) T.res.bracket[p] = 0;
+    }
 
     int mrf = 1, mlow_rf = 1, mupp_rf = 1;
-    if (valid) rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
+    if (valid && !none) rf_window(a, Rf, mrf, mlow_rf, mupp_rf, m_lane);
 
     // ---- gather ----
     const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
@@ -831,11 +865,12 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
         X[K_LAPN] = (double)lapn;
     }
 
-    bool active = valid;
+    bool active = valid && !none;
     bool past = false, crossed = false, stop_set = false, oobA = false;
     double rstop = 0.0;
-    int stop = m, scross = 0;
+    int stop = m_lane, scross = 0;
     for (int s = 2; s <= m; ++s) {
+        if (s > m_lane) active = false;
         if (__ballot(active) == 0ull) break;
         const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
         const double radS = rad[s], radP = rad[s - 1];
@@ -891,22 +926,23 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
             }
         }
     }
-    if (!valid) return;
+    if (!valid || none) return;
     if (T.res.nshells_used) T.res.nshells_used[p] = stop;
     if (T.res.crossing_shell) T.res.crossing_shell[p] = scross;
     if (!a.pred) {
-        post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf);
+        post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf, 1,
+                     nullptr, m_lane);
         if (T.res.bracket) T.res.bracket[p] = 0;
         return;
     }
     // learned bracket [lo, hi]: the gather verified when it ended within hi + 2, the walk
     // when it collapsed at or above lo - 2
     const float* y = a.pred + (T.base + p) * HNT_MLP_OUTPUTS;
-    const int lo = bracket_shell(y[0], m), hi = bracket_shell(y[1], m);
+    const int lo = bracket_shell(y[0], m_lane), hi = bracket_shell(y[1], m_lane);
     const int walk_lo = lo - 2 > 1 ? lo - 2 : 1;
     int lo_true = 0;
     post_process(a, T, p, P, ci, cj, ck, ZZon, fcrit, ir2min, mrf, mlow_rf, mupp_rf, walk_lo,
-                 &lo_true);
+                 &lo_true, m_lane);
     if (T.res.bracket)
         T.res.bracket[p] = (stop <= hi + 2 ? 1 : 0) | (lo_true >= walk_lo && lo_true > 0 ? 2 : 0) |
                            ((y[2] > 0.0f) == (lo_true > 0) ? 4 : 0);
@@ -1745,11 +1781,34 @@ void sort_pairs(SAWork& W, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipEr
                                             (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st);
 }
 
-// What a replay of an analysis launches: the record repacks and the main kernel.
+// Variant B with rmax2rs > 0 skips a peak whose cell the mask holds beyond
+// floor(Rf / alatt * rmax2rs) (ShellAnalysisGPU.jl:112-117).  The flags are taken from the
+// mask as the call finds it, before any peak of the call marks its halo.
+__global__ void premask_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+                               const int32_t* __restrict__ pk, const double* __restrict__ Rf,
+                               int64_t npeaks, int n, const int8_t* __restrict__ mask,
+                               double rmax2rs, double alatt, int8_t* __restrict__ skip) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npeaks) return;
+    const int64_t c = (int64_t)(pi[q] - 1) + (int64_t)(pj[q] - 1) * n +
+                      (int64_t)(pk[q] - 1) * n * (int64_t)n;
+    skip[q] = (double)mask[c] > floor(Rf[q] / alatt * rmax2rs) ? 1 : 0;
+}
+
+// What a replay of an analysis launches: the record repacks, the pre-mask flags and the
+// main kernel.
 struct RepackJob {
     const float *d, *ex, *ey, *ez, *e2x, *e2y, *e2z;
     int n;
     float4* recs;
+};
+struct PremaskJob {
+    const int32_t *pi, *pj, *pk;
+    const double* Rf;
+    int64_t npeaks;
+    int n;
+    const int8_t* mask;
+    int8_t* skip;
 };
 struct LaunchDesc {
     SAArgs A;
@@ -1757,7 +1816,15 @@ struct LaunchDesc {
     int gang = 1;  // lanes per peak: 1, 16 or 64 (one wavefront)
     bool kahan = false;
     std::vector<RepackJob> repacks;
+    std::vector<PremaskJob> premasks;
 };
+
+void launch_premasks(hipStream_t st, const LaunchDesc& d) {
+    for (const PremaskJob& j : d.premasks)
+        hipLaunchKernelGGL(premask_kernel, dim3((unsigned)((j.npeaks + 255) / 256)), dim3(256),
+                           0, st, j.pi, j.pj, j.pk, j.Rf, j.npeaks, j.n, j.mask, d.A.rmax2rs,
+                           d.A.alatt, j.skip);
+}
 
 void launch_repacks(hipStream_t st, const std::vector<RepackJob>& jobs) {
     for (const RepackJob& j : jobs) {
@@ -1805,8 +1872,12 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
                 "hnt_shell_analyse: HNT_ORDER_LEARNED needs the bracket network");
     HNT_REQUIRE(params->order != HNT_ORDER_LEARNED || tables->nshells > 12,
                 "hnt_shell_analyse: HNT_ORDER_LEARNED needs more than 12 shells");
-    if (params->rmax2rs != 0.0) {
-        set_error("hnt_shell_analyse: rmax2rs > 0 is not supported by this library");
+    HNT_REQUIRE(params->rmax2rs >= 0.0 && params->rmax2rs <= 1e300,
+                "hnt_shell_analyse: rmax2rs must be finite and >= 0");
+    const bool limited = params->rmax2rs > 0.0;
+    if (limited && params->variant == HNT_VARIANT_A) {
+        // RadialShell.analyse_peak limits by a cell count (RadialShell.jl), not by r^2
+        set_error("hnt_shell_analyse: rmax2rs > 0 with variant A is not supported");
         return HNT_ERR_UNSUPPORTED;
     }
     HNT_REQUIRE(params->alatt > 0.0, "hnt_shell_analyse: alatt must be > 0");
@@ -1938,7 +2009,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     for (int t = 0; t < ntiles; ++t) prim[t] = batches[t].ir2min;
     // first order: 16^3 bricks when it only feeds the crossing pass (its delta-only reads
     // favour small bricks), else the schedule's brick size
-    const bool two_orders = params->early_exit && params->order != HNT_ORDER_PLAIN;
+    // with rmax2rs > 0 every peak gathers all shells up to its limit (no early exit), as
+    // the reference GPU path (ext/CUDAExt.jl:2235-2236): the schedule pass has nothing to order
+    const bool early = params->early_exit && !limited;
+    const bool two_orders = early && params->order != HNT_ORDER_PLAIN;
     HNT_TRY(order_by(prim.data(), two_orders ? 4 : ctx->brick_log2));
     hipLaunchKernelGGL(group_tile_kernel, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0,
                        st, tile_pad, ntiles, ngroups, (int32_t*)W.gtiles.p);
@@ -1968,10 +2042,32 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     A.wRnor = 3.0 * (1.0 / (4.0 * kPi)) / params->alatt;
     A.nbuff = params->nbuff;
     A.variant = params->variant;
-    A.early = params->early_exit ? 1 : 0;
+    A.early = early ? 1 : 0;
+    A.rmax2rs = params->rmax2rs;
 
     const bool kahan = params->precision == HNT_PREC_FP32_KAHAN;
-    const int gang = ctx->gang;
+    // the ganged kernels analyse every peak to the last shell: a shell limit takes the
+    // lane-per-peak kernel
+    const int gang = limited ? 1 : ctx->gang;
+    std::vector<PremaskJob> premasks;
+    if (limited && params->variant == HNT_VARIANT_B && masks) {
+        for (int t = 0; t < ntiles; ++t)
+            if (masks[t] && batches[t].npeaks > 0) premasks.push_back(PremaskJob{});
+        if (!premasks.empty()) {
+            HNT_TRY(reserve_in(ctx, W.skip, (size_t)ntot, W.arena));
+            int8_t* skip = (int8_t*)W.skip.p;
+            hipLaunchKernelGGL(fill_i8, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st,
+                               skip, ntot, (int8_t)0);
+            premasks.clear();
+            for (int t = 0; t < ntiles; ++t) {
+                const hnt_batch& b = batches[t];
+                if (!masks[t] || b.npeaks == 0) continue;
+                premasks.push_back(PremaskJob{b.pi, b.pj, b.pk, b.Rf, b.npeaks, fields[t].n1,
+                                              masks[t], skip + meta[t]});
+            }
+            A.skip = skip;
+        }
+    }
     const void* kern = gang == 64 ? (kahan ? (const void*)shell_wave_kernel<SumKahan>
                                            : (const void*)shell_wave_kernel<SumF64>)
                      : gang == 16 ? (kahan ? (const void*)shell_gang_kernel<SumKahan>
@@ -2050,6 +2146,8 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     D.gang = gang;
     D.kahan = kahan;
     D.repacks = std::move(jobs);
+    D.premasks = std::move(premasks);
+    launch_premasks(st, D);
     launch_main(st, D);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);
     return HNT_OK;
@@ -2111,6 +2209,7 @@ static hnt_status plan_build(hnt_plan* P, int32_t ntiles, const hnt_fields* fiel
     HNT_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, P->cap, P->D.A.qctr, 8, 0ull);
     launch_repacks(P->cap, P->D.repacks);
+    launch_premasks(P->cap, P->D);
     launch_main(P->cap, P->D);
     HNT_HIP(hipStreamEndCapture(P->cap, &P->graph));
     // the graph must hold kernels only (no memset or copy nodes)

@@ -132,7 +132,10 @@ def _dev1(a, dtype, device="cuda"):
     torch = _torch()
     if isinstance(a, torch.Tensor):
         return a.to(device=device, dtype=dtype).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(a), device=device).to(dtype).contiguous()
+    arr = np.ascontiguousarray(a)
+    if not arr.flags.writeable:  # torch warns on read-only numpy arrays
+        arr = arr.copy()
+    return torch.as_tensor(arr, device=device).to(dtype).contiguous()
 
 
 # ---------------------------------------------------------------- tables
@@ -395,7 +398,8 @@ def _batch_struct(pi, pj, pk, Rf, ir2min, ZZon, fcrit, ZZon_pp, fcrit_pp, dev):
     return b, [pi_, pj_, pk_, Rf_, ir_, zz, fc]
 
 
-def _params(alatt, nbuff, variant, early_exit, order, bracket, precision, extent=None):
+def _params(alatt, nbuff, variant, early_exit, order, bracket, precision, extent=None,
+            rmax2rs=0.0):
     net = None
     if order == "learned":
         if bracket is None:
@@ -403,7 +407,8 @@ def _params(alatt, nbuff, variant, early_exit, order, bracket, precision, extent
         net = bracket.struct()
     if order == "given" and extent is None:
         raise ValueError("order='given' needs the extents (extent=...)")
-    return abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)),
+    return abi.Params(float(alatt), int(nbuff), float(rmax2rs), abi.VARIANTS[variant],
+                      int(bool(early_exit)),
                       abi.ORDERS[order], abi.PRECISIONS[precision],
                       C.pointer(net) if net is not None else None,
                       abi.as_ptr(_ptr(extent), C.c_int32))
@@ -413,14 +418,16 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
                   alatt: float, ZZon: float, fcrit: float, nbuff: int = 0, variant="C",
                   early_exit: bool = True, mask=None, ZZon_pp=None, fcrit_pp=None,
                   results=None, order: str = "crossing", bracket=None,
-                  precision: str = "fp64", extent=None):
+                  precision: str = "fp64", extent=None, rmax2rs: float = 0.0):
     """hnt_shell_analyse over one batch of peaks (device arrays or numpy).  Returns a dict of
     device tensors: scalars [np], vectors [np, 3], strain [np, 3, 3] ([p, L, K]).
 
     order: wavefront schedule ("crossing", "plain", "learned" with `bracket`, a
     huntian_hip.bracket.BracketNet, or "given" with `extent`, a device int32 [np] of gather
     extents); it changes the speed, never a result.
-    precision: per-shell sums in "fp64" (the reference CPU arithmetic) or "fp32_kahan"."""
+    precision: per-shell sums in "fp64" (the reference CPU arithmetic) or "fp32_kahan".
+    rmax2rs > 0 (variants "B" and "C"): each peak uses only its shells with
+    r <= rmax2rs Rf / alatt, all of them gathered."""
     torch = _require_gpu()
     ctx = context()
     fs, keep_f = _fields_struct(fields)
@@ -428,7 +435,7 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     dev = torch.device("cuda", ctx.device)
     b, keep_b = _batch_struct(pi, pj, pk, Rf, ir2min, ZZon, fcrit, ZZon_pp, fcrit_pp, dev)
     ext = _dev1(extent, torch.int32, dev) if extent is not None else None
-    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision, ext)
+    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision, ext, rmax2rs)
     r = results if results is not None else alloc_results(npk, dev)
     rs = _results_struct(r)
     cs, keep_t = _ctab_struct(ct)
@@ -464,7 +471,8 @@ def _tiles_structs(tiles, ZZon, fcrit, dev):
 
 def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
                         fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
-                        order: str = "crossing", bracket=None, precision: str = "fp64"):
+                        order: str = "crossing", bracket=None, precision: str = "fp64",
+                        rmax2rs: float = 0.0):
     """hnt_shell_analyse_tiles: every tile in one persistent launch.
 
     tiles: list of dicts with keys fields, pi, pj, pk, Rf, ir2min and optionally mask,
@@ -473,7 +481,8 @@ def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, Z
     ctx = context()
     dev = torch.device("cuda", ctx.device)
     FA, BA, RA, MA, keep, out = _tiles_structs(tiles, ZZon, fcrit, dev)
-    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
+    prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision,
+                  rmax2rs=rmax2rs)
     cs, keep_t = _ctab_struct(ct)
     ts = tables.struct()
     st = ctx.lib.hnt_shell_analyse_tiles(ctx.handle, len(tiles), FA, BA, C.byref(ts),
@@ -492,12 +501,14 @@ class ShellPlan:
 
     def __init__(self, tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
                  fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
-                 order: str = "crossing", bracket=None, precision: str = "fp64"):
+                 order: str = "crossing", bracket=None, precision: str = "fp64",
+                 rmax2rs: float = 0.0):
         torch = _require_gpu()
         self.ctx = context()
         dev = torch.device("cuda", self.ctx.device)
         FA, BA, RA, MA, keep, out = _tiles_structs(tiles, ZZon, fcrit, dev)
-        prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision)
+        prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision,
+                      rmax2rs=rmax2rs)
         cs, keep_t = _ctab_struct(ct)
         ts = tables.struct()
         h = C.c_void_p()
@@ -679,8 +690,6 @@ def analyse_peaks_gpu_cuda_multirf(delta, etax, etay, etaz, eta2x, eta2y, eta2z,
     Float32 arrays with the fields of the reference's NamedTuple; mask is the updated host
     Int8 array or None.  `threads` is accepted for signature compatibility and ignored."""
     torch = _require_gpu()
-    if rmax2rs != 0.0:
-        raise NotImplementedError("rmax2rs > 0 is not supported")
     ctd = make_ctab(ct_table, X1, X2, Y1, Y2, Z1, Z2, ct_out_val)
     if fcrit_override is not None:
         fcrit = float(fcrit_override)
@@ -713,7 +722,7 @@ def analyse_peaks_gpu_cuda_multirf(delta, etax, etay, etaz, eta2x, eta2y, eta2z,
                       alatt=alatt, ZZon=ZZon, fcrit=fcrit, nbuff=nbuff, variant=variant,
                       early_exit=True, mask=mask_d,
                       ZZon_pp=cc["zz"] if has_pp else None,
-                      fcrit_pp=cc["fc"] if has_pp else None)
+                      fcrit_pp=cc["fc"] if has_pp else None, rmax2rs=rmax2rs)
     host = _host_results(r, tot)
     results = []
     off = 0

@@ -232,7 +232,14 @@ typedef struct {
 typedef struct {
     double alatt;              /* lattice spacing (Mpc/h) */
     int32_t nbuff;             /* mask is only written on (nbuff+1 : n-nbuff)^3 */
-    double rmax2rs;            /* only 0 is supported (the setting of the benchmark workloads) */
+    double rmax2rs;            /* 0: every shell of the tables.  > 0 (variants B and C; A is
+                                  HNT_ERR_UNSUPPORTED): a peak uses only the leading shells with
+                                  r^2 <= (rmax2rs Rf / alatt)^2, none if fewer than 2
+                                  (ShellAnalysisGPU.jl:90-99, ext/CUDAExt.jl:2176-2185); its
+                                  gather takes all of them (early_exit and order are ignored,
+                                  ext/CUDAExt.jl:2235-2236); variant B with a mask skips a peak
+                                  whose cell the mask holds beyond floor(Rf / alatt rmax2rs)
+                                  as the call finds it (ShellAnalysisGPU.jl:112-117) */
     int32_t variant;           /* HNT_VARIANT_A / _B / _C */
     int32_t early_exit;        /* 1: stop gathering once no later shell can change a result */
     int32_t order;             /* wavefront schedule of the peaks, HNT_ORDER_* (0: crossing pass);

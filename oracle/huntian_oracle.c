@@ -566,7 +566,25 @@ static void analyse_one(const hnt_fields* f, const hnt_shell_tables* t, const hn
     const int32_t n1 = f->n1, n2 = f->n2, n3 = f->n3;
     const int64_t s1 = n1, s2 = (int64_t)n1 * n2;
     const int32_t NS = t->nshells;
-    const int m = NS - 1; /* last finalised shell (the last shell never is) */
+    int m = NS - 1; /* last finalised shell (the last shell never is) */
+    if (prm->rmax2rs > 0.0) {
+        /* B and C: only the leading shells with r^2 <= (rmax2rs Rf / alatt)^2
+         * (ShellAnalysisGPU.jl:90-97, ext/CUDAExt.jl:2177-2185) */
+        double lim = prm->rmax2rs * Rf / alatt;
+        lim = lim * lim;
+        int cnt = 0;
+        for (int s = 1; s <= NS - 1; ++s) {
+            if (!((double)t->shell_r2[s - 1] <= lim)) break;
+            cnt = s;
+        }
+        m = cnt;
+        if (m < 2) { /* ShellAnalysisGPU.jl:99 */
+            no_collapse(out);
+            *nused = 0;
+            *ncross = 0;
+            return;
+        }
+    }
     /* hlatt = 1 (RadialShell.jl:296-300) */
     const double aRnor = 3.0 * 1.0 / alatt;
     const double wRnor = 3.0 * (1.0 / (4.0 * PI_D)) / alatt;
@@ -711,7 +729,9 @@ static void analyse_one(const hnt_fields* f, const hnt_shell_tables* t, const hn
         if (!stop_set && crossed && rad[s + 1] >= rstop && s >= mupp_rf) {
             stop = s;
             stop_set = 1;
-            if (prm->early_exit) { last = s; break; }
+            /* with rmax2rs > 0 every shell up to the limit is gathered, as the reference
+             * GPU path passes no shell_r2 to its gather then (ext/CUDAExt.jl:2235-2236) */
+            if (prm->early_exit && !(prm->rmax2rs > 0.0)) { last = s; break; }
         }
     }
     if (nused) *nused = stop;
@@ -1037,8 +1057,24 @@ int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_t
     if (!f || !b || !t || !ct || !p || !r) return 1;
     if (f->n1 != f->n2 || f->n2 != f->n3) return 2;
     if (t->nshells < 3) return 3;
-    if (p->rmax2rs != 0.0) return 4;
     if (p->variant < 0 || p->variant > 2) return 5;
+    /* rmax2rs > 0: the r^2 limit of variants B and C; variant A limits by a cell count
+     * (RadialShell.jl), not restated here */
+    if (!(p->rmax2rs >= 0.0) || (p->rmax2rs > 0.0 && p->variant == HNT_VARIANT_A)) return 4;
+    /* variant B skips a peak whose cell the mask already holds beyond floor(Rf/alatt
+     * rmax2rs) (ShellAnalysisGPU.jl:112-117).  The mask is read as it is at the call: the
+     * peaks of one call are analysed together, so one peak's halo does not skip another of
+     * the same call (a caller wanting that analyses one peak per call) */
+    int8_t* skip = NULL;
+    if (p->rmax2rs > 0.0 && p->variant == HNT_VARIANT_B && mask) {
+        skip = (int8_t*)calloc((size_t)(b->npeaks > 0 ? b->npeaks : 1), 1);
+        if (!skip) return 8;
+        const int64_t s1 = f->n1, s2 = (int64_t)f->n1 * f->n2;
+        for (int64_t q = 0; q < b->npeaks; ++q) {
+            const int64_t c = (b->pi[q] - 1) + (b->pj[q] - 1) * s1 + (b->pk[q] - 1) * s2;
+            skip[q] = (double)mask[c] > floor(b->Rf[q] / p->alatt * p->rmax2rs);
+        }
+    }
     double akk[1101];
     hro_atab4(akk);
     int bad = 0;
@@ -1059,6 +1095,13 @@ int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_t
                 double ZZ = b->ZZon ? b->ZZon[q] : b->ZZon_scalar;
                 double fc = b->fcrit ? b->fcrit[q] : b->fcrit_scalar;
                 if (!(ZZ > -1.0)) { bad |= 16; no_collapse(&v); store(r, q, &v); continue; }
+                if (skip && skip[q]) {
+                    no_collapse(&v);
+                    store(r, q, &v);
+                    if (r->nshells_used) r->nshells_used[q] = 0;
+                    if (r->crossing_shell) r->crossing_shell[q] = 0;
+                    continue;
+                }
                 int32_t nu = 0, nx = 0;
                 analyse_one(f, t, ct, akk, p, b->pi[q], b->pj[q], b->pk[q], b->Rf[q],
                             b->ir2min[q], ZZ, fc, &w, &v, mask, &nu, &nx);
@@ -1069,6 +1112,7 @@ int hro_analyse_peaks(const hnt_fields* f, const hnt_batch* b, const hnt_shell_t
             prof_free(&w);
         }
     }
+    free(skip);
     return bad;
 }
 

@@ -282,7 +282,7 @@ def results_struct(r: dict) -> abi.Results:
 
 def analyse_peaks(flds: dict, pi, pj, pk, Rf, ir2min, t: dict, ct: dict, *, alatt, ZZon,
                   fcrit, nbuff=0, variant="A", early_exit=False, mask=None, nthreads=1,
-                  ZZon_pp=None, fcrit_pp=None):
+                  ZZon_pp=None, fcrit_pp=None, rmax2rs=0.0):
     """Variant "A" = RadialShell.analyse_peak, "B" = analyse_peak_gpu, "C" = B + zvir_half.
     Returns a dict of per-peak arrays (strain as [p, L, K])."""
     npk = len(pi)
@@ -298,7 +298,8 @@ def analyse_peaks(flds: dict, pi, pj, pk, Rf, ir2min, t: dict, ct: dict, *, alat
     b = abi.Batch(npk, _p(pi, C.c_int32), _p(pj, C.c_int32), _p(pk, C.c_int32),
                   _p(Rf, C.c_double), _p(ir2, C.c_int32), _p(zz, C.c_double),
                   _p(fc, C.c_double), float(ZZon), float(fcrit))
-    prm = abi.Params(float(alatt), int(nbuff), 0.0, abi.VARIANTS[variant], int(bool(early_exit)))
+    prm = abi.Params(float(alatt), int(nbuff), float(rmax2rs), abi.VARIANTS[variant],
+                     int(bool(early_exit)))
     r = alloc_results(npk)
     rs = results_struct(r)
     if mask is not None:
