@@ -352,8 +352,8 @@ hnt_status hnt_shell_analyse_tiles(hnt_ctx* ctx, int32_t ntiles, const hnt_field
 /*
  * Plans: hnt_shell_analyse_tiles recorded once and replayed as one hipGraph (no counterpart
  * in the reference).  hnt_plan_create runs the analysis eagerly (results are written) and
- * records its device work -- the record repacks and the persistent analysis kernel -- in a
- * graph of kernel nodes only.  hnt_plan_launch replays it on the context's stream: it
+ * records its device work -- the record repacks, variant B's pre-mask flags (rmax2rs > 0)
+ * and the persistent analysis kernel -- in a graph of kernel nodes only.  hnt_plan_launch replays it on the context's stream: it
  * re-reads the fields, peak arrays, ZZon and fcrit at the same addresses (contents may
  * change between launches, sizes and pointers may not) and rewrites results and masks.  The
  * peak schedule of the creating call is kept, so a replay over changed fields is exact but
