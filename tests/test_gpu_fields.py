@@ -136,8 +136,8 @@ def hipfft_mode():
 
 @pytest.mark.parametrize("L", [512, 1024])
 def test_library_fft_matches_hipfft(hipfft_mode, L):
-    """The library's pruned transforms against hipFFT's full ones on the device (1024: 16
-    columns per workgroup, 512-point rows): the isolated short-range potential of an
+    """The library's pruned transforms against hipFFT's full ones on the device (1024: 8
+    columns per 1024-thread workgroup, 512-point rows): the isolated short-range potential of an
     (L/2)^3 field and the periodic Laplacian of an L^3 field.  Both are Float32
     transforms: relative L2 <= 2e-6."""
     import torch

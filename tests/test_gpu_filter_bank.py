@@ -32,7 +32,7 @@ def smooth_ref(delta, alatt, R, kind):
     fk = np.fft.rfftn(np.ascontiguousarray(delta.T).astype(np.float64))
     KZ, KY, KX, _ = synth._kgrid(n, n * alatt)
     k = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
-    return np.asfortranarray(np.fft.irfftn(fk * window(kind, k, R), s=(n, n, n)).T)
+    return np.asfortranarray(np.fft.irfftn(fk * window(kind, k, R), s=(n, n, n), axes=(0, 1, 2)).T)
 
 
 @pytest.fixture(scope="module")
@@ -208,3 +208,99 @@ def test_tile_rejects_bad_arguments(tile):
         api.peak_find_tile(_dev(delta), [2.0], [1.0], 0, md, alatt=alatt)
     with pytest.raises(HuntianError):
         api.peak_find_tile(_dev(delta), [2.0], [1.0], 2, md, alatt=-1.0)
+
+
+# ---------------------------------------------------------------- the 1024^3 tile
+
+@pytest.fixture(scope="module")
+def tile1024():
+    """A 1024^3 tile of the headline workload's size (2 Mpc/h cells, buffer 64): Gaussian
+    white noise, whose smoothing at a few cells gives peaks everywhere; three filters with
+    thresholds at three times each smoothed field's rms keep the peak lists near 10^5."""
+    import torch
+
+    n, alatt, nbuff = 1024, 2.0, 64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1024)
+    d = api.fortran_empty((n, n, n), torch.float32, "cuda")
+    d.copy_(torch.randn((n, n, n), generator=g, device="cuda").permute(2, 1, 0))
+    radii = [12.0, 6.0, 3.0]
+    print("tile1024: field ready", flush=True)
+    return d, alatt, nbuff, radii
+
+
+def _rel_l2(a, b):
+    import torch
+
+    num = den = 0.0
+    for z in range(0, a.shape[2], 128):  # slabs bound the float64 temporaries
+        x = a[:, :, z:z + 128].double()
+        y = b[:, :, z:z + 128].double()
+        num += float(torch.sum((x - y) ** 2))
+        den += float(torch.sum(y ** 2))
+    return (num / den) ** 0.5
+
+
+def test_tile1024_library_fft_cropped_and_oracle(oracle, tile1024):
+    """At 1024^3 (the 1024-thread column transforms, the window table read in the first
+    inverse pass, the inverse transform cropped to the inner cube): the smoothed fields
+    within relative L2 2e-6 of hipFFT's (tuning fft = 1); the cropped call (no smoothed_out)
+    gives the same peaks, values, d2Rf and mask as the whole-field call; and the peak list and
+    mask equal the oracle's filter loop over the smoothed fields, streamed to the host one
+    filter at a time with the mask carried."""
+    import torch
+
+    d, alatt, nbuff, radii = tile1024
+    n = d.shape[0]
+    ctx = api.context(0)
+    zero = api.fortran_empty((n, n, n), torch.int8, "cuda")
+    zero.zero_()
+    # thresholds from a first pass (library FFT, whole fields)
+    outs = _outs(n, len(radii))
+    md = zero.clone()
+    api.peak_find_tile(d, radii, [1e30] * len(radii), nbuff, md, alatt=alatt,
+                       smoothed_out=outs)
+    fcrits = [3.0 * float(torch.sqrt(torch.mean(o.double() ** 2))) for o in outs]
+    cap = 1 << 24
+    md_whole = zero.clone()
+    whole = api.peak_find_tile(d, radii, fcrits, nbuff, md_whole, alatt=alatt,
+                               with_lapd=True, capacity=cap, smoothed_out=outs)
+    print(f"tile1024: whole-field call, {whole['total']} peaks", flush=True)
+    assert 10_000 < whole["total"] < cap
+    md_crop = zero.clone()
+    crop = api.peak_find_tile(d, radii, fcrits, nbuff, md_crop, alatt=alatt, with_lapd=True,
+                              capacity=cap)
+    assert crop["total"] == whole["total"]
+    for k in ("i", "j", "k", "filt", "delta", "d2Rf"):
+        assert torch.equal(crop[k], whole[k]), k
+    assert torch.equal(md_crop, md_whole)
+    del md_crop, crop
+    # hipFFT: same transforms in full, its own rounding
+    hip_outs = _outs(n, len(radii))
+    try:
+        ctx.set_tuning(fft=1)
+        api.peak_find_tile(d, radii, fcrits, nbuff, zero.clone(), alatt=alatt, capacity=cap,
+                           smoothed_out=hip_outs)
+    finally:
+        ctx.set_tuning(fft=0)
+    for R, a, b in zip(radii, outs, hip_outs):
+        err = _rel_l2(a, b)
+        print(f"tile1024: R={R} library vs hipFFT rel L2 {err:.2e}", flush=True)
+        assert err <= 2e-6, (R, err)
+    del hip_outs
+    torch.cuda.empty_cache()
+    # the oracle over the library's smoothed fields, one filter at a time, mask carried
+    mref = np.zeros((n, n, n), np.int8, order="F")
+    parts = []
+    for f, (o, fc) in enumerate(zip(outs, fcrits)):
+        field = np.asfortranarray(o.cpu().numpy())
+        ref, tot = oracle.peak_find_multi([field], [fc], nbuff, mref, cap=cap, nthreads=16)
+        ref["filt"] = ref["filt"] + f
+        parts.append(ref)
+        del field
+        print(f"tile1024: oracle filter {f}: {tot} peaks", flush=True)
+    ref = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    assert len(ref["i"]) == whole["total"]
+    for k in ("i", "j", "k", "filt", "delta"):
+        assert np.array_equal(whole[k].cpu().numpy(), ref[k]), k
+    assert np.array_equal(md_whole.cpu().numpy(), mref)
