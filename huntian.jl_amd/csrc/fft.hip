@@ -111,7 +111,12 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 // One Stockham stage of radix R on `ncol` columns of length L held in LDS (column c at
 // buf + c * LS): butterfly j of a column reads x[j + r L/R], twiddles by W^(r k L/(Ns R)),
 // k = j mod Ns, and writes x[(j/Ns) Ns R + k + r Ns].  All reads precede all writes.
-template <int R, bool INV, int NT, int DATA>
+// Row-major LDS layout of the direct-load column passes (8 columns): element i of column c
+// at (i + i / 16) * 8 + c, i.e. rows of 8 complex (64 B) in blocks of 16 rows (1 KB, what one
+// wavefront's 16-byte direct loads fill) with one spare row after each block.
+__device__ __forceinline__ int rm_slot(int c, int i) { return ((i + (i >> 4)) << 3) + c; }
+
+template <int R, bool INV, int NT, int DATA, bool RM = false>
 __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, int lgNs,
                                       const float2* __restrict__ W) {
     constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
@@ -121,15 +126,17 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
     const int total = nb * ncol;
     const int Ns = 1 << lgNs;
     const int tw_shift = lgL - lgNs - LGR;
+    // column-major: a thread's butterflies run down one column; row-major (8 columns): the
+    // column index is the fastest, so eight lanes read or write one 64-B row
+    auto at = [&](int c, int i) -> float2& { return RM ? buf[rm_slot(c, i)] : buf[c * LS + pad(i)]; };
     float2 v[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int g = (int)threadIdx.x + q * NT;
         if (g < total) {
-            const int c = g >> lgnb, j = g & (nb - 1);
-            const float2* col = buf + c * LS;
+            const int c = RM ? (g & 7) : g >> lgnb, j = RM ? (g >> 3) : g & (nb - 1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = col[pad(j + r * nb)];
+            for (int r = 0; r < R; ++r) v[q][r] = at(c, j + r * nb);
             const int k = j & (Ns - 1);
             if (k != 0) {
                 // W^k from the table, the higher powers by products (at most four roundings
@@ -161,32 +168,31 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
     for (int q = 0; q < MAXB; ++q) {
         const int g = (int)threadIdx.x + q * NT;
         if (g < total) {
-            const int c = g >> lgnb, j = g & (nb - 1);
+            const int c = RM ? (g & 7) : g >> lgnb, j = RM ? (g >> 3) : g & (nb - 1);
             const int k = j & (Ns - 1);
-            float2* col = buf + c * LS;
             const int d0 = ((j >> lgNs) << (lgNs + LGR)) + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) col[pad(d0 + r * Ns)] = v[q][r];
+            for (int r = 0; r < R; ++r) at(c, d0 + r * Ns) = v[q][r];
         }
     }
     __syncthreads();
 }
 
 // FFT of length 2^lgL of every column, in place, natural order in and out.
-template <bool INV, int NT, int DATA>
+template <bool INV, int NT, int DATA, bool RM = false>
 __device__ __forceinline__ void stockham(float2* buf, int LS, int ncol, int lgL,
                                          const float2* __restrict__ W) {
     int lgNs = 0;
     while (lgNs < lgL) {
         const int rem = lgL - lgNs;
         if (rem == 1) {
-            stage<2, INV, NT, DATA>(buf, LS, ncol, lgL, lgNs, W);
+            stage<2, INV, NT, DATA, RM>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 1;
         } else if (rem == 2 || rem == 4) {
-            stage<4, INV, NT, DATA>(buf, LS, ncol, lgL, lgNs, W);
+            stage<4, INV, NT, DATA, RM>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 2;
         } else {
-            stage<8, INV, NT, DATA>(buf, LS, ncol, lgL, lgNs, W);
+            stage<8, INV, NT, DATA, RM>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 3;
         }
     }
@@ -321,6 +327,67 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols(
     for (int e = threadIdx.x; e < C * L; e += NT) {
         const int c = e & (C - 1), a = e >> lgC, x = x0 + c;
         if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[c * LS + pad(a)];
+    }
+}
+
+// fft_cols for 1024-point columns, 8 per workgroup, every element read (r0 = 0, r1 = L):
+// the spectrum goes from HBM straight into LDS (16-byte direct loads, no registers held), so
+// a workgroup has all of its 64 KB in flight at once instead of two elements per thread.
+// Each wavefront load fills one 1 KB block of the row-major layout (rm_slot): lane l brings
+// columns 2 (l % 4) and 2 (l % 4) + 1 of row 16 b + l / 4.  The operator's table entries
+// are loaded into registers meanwhile and the operator is applied in LDS, element by element
+// as the register path does, so the values are the same bit for bit.
+constexpr int RM_LDS = 17 * 64 * 8;  // 64 blocks of 16 rows plus a spare row, 8 columns
+template <bool INV, int NT, int WAVES>
+__global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
+    const float2* in, float2* out, int64_t strideA, int64_t strideB, int nx, int gx,
+    int ntiles, int b0, int w0, int w1, FftOp op, const float2* __restrict__ W) {
+    constexpr int L = 1024, lgL = 10, C = 8, NB = L / 16;
+    constexpr int EPT = C * L / NT;  // elements per thread
+    __shared__ float2 buf[RM_LDS];
+    const int per = (ntiles + 7) >> 3;
+    const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int x0 = (tile % gx) * C;
+    const int b = b0 + tile / gx;
+    const int64_t boff = (int64_t)b * strideB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // rows of the block, two columns per lane; columns past nx (the last tile) read the
+    // row's pitch padding, whose values are never stored
+    const int lrow = lane >> 2, lcol = 2 * (lane & 3);
+    for (int blk = wave; blk < NB; blk += NT / 64) {
+        const int a = 16 * blk + lrow;
+        const float2* g = in + (x0 + lcol) + (int64_t)a * strideA + boff;
+        __builtin_amdgcn_global_load_lds(
+            g, (__attribute__((address_space(3))) void*)(buf + 17 * 8 * blk), 16, 0, 0);
+    }
+    float t[EPT];
+    if (op.kind == FFT_OP_WINDOW || op.kind == FFT_OP_TRANSFER) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int c = e & (C - 1), a = e >> 3, x = x0 + c;
+            t[q] = x < nx ? op.amp[table_index(L, x, b, a)] : 0.0f;
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (op.kind != FFT_OP_NONE) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int c = e & (C - 1), a = e >> 3, x = x0 + c;
+            if (x < nx) {
+                float2& v = buf[rm_slot(c, a)];
+                v = apply_op(op, v, x, b, a, t[q]);
+            }
+        }
+        __syncthreads();
+    }
+    stockham<INV, NT, C * L, true>(buf, 0, C, lgL, W);
+    for (int e = threadIdx.x; e < C * L; e += NT) {
+        const int c = e & (C - 1), a = e >> 3, x = x0 + c;
+        if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[rm_slot(c, a)];
     }
 }
 
@@ -485,6 +552,8 @@ unsigned cols_grid(unsigned gx, int nb) {
 
 bool cols_big(int L) { return L >= 1024; }
 
+
+
 int cols_lgC(int L) {
     // as many padded columns as the shape's LDS holds, at most 8 (64-B segments)
     const int cap = cols_big(L) ? BIG_LDS : SMALL_LDS;
@@ -502,9 +571,15 @@ int rows_lgG(int M) {
 template <bool INV>
 void launch_cols(hipStream_t st, int L, unsigned gx, int nb, const float2* in, float2* out,
                  int64_t strideA, int64_t strideB, int nx, int b0, int r0, int r1, int w0,
-                 int w1, const FftOp& op, const float2* W) {
+                 int w1, const FftOp& op, const float2* W, bool direct = false) {
     const int lgC = cols_lgC(L);
     const int LS = cols_ls(L, 1 << lgC);
+    if (direct && L == 1024 && lgC == 3 && r0 == 0 && r1 == L) {
+        hipLaunchKernelGGL((fft_cols_direct<INV, BIG_T, 8>), dim3(cols_grid(gx, nb)),
+                           dim3(BIG_T), 0, st, in, out, strideA, strideB, nx, (int)gx,
+                           (int)gx * nb, b0, w0, w1, op, W);
+        return;
+    }
     if (cols_big(L))
         hipLaunchKernelGGL((fft_cols<INV, BIG_T, BIG_LDS, BIG_DATA, 8>), dim3(cols_grid(gx, nb)), dim3(BIG_T),
                            0, st, in, out, ilog2(L), strideA, strideB, nx, lgC, (int)gx,
@@ -584,10 +659,10 @@ hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, in
     // z: every (ix, iy) column, the operator applied as read; planes iz in [s, s + n) kept
     const int64_t P = fft_pitch(L);
     launch_cols<true>(st, L, gx, L, spec, work, P * L, P, nk, 0, 0, L, s,
-                      s + n, op, WL);
+                      s + n, op, WL, ctx->fft_direct != 0);
     // y: planes iz in [s, s + n), rows iy in [s, s + n) kept
     launch_cols<true>(st, L, gx, n, work, work, P, P * L, nk, s, 0, L, s,
-                      s + n, none, WL);
+                      s + n, none, WL, ctx->fft_direct != 0);
     const int lgG = rows_lgG(M);
     const int64_t rows = (int64_t)n * n;
     hipLaunchKernelGGL(fft_rows_c2r, dim3((unsigned)((rows + (1 << lgG) - 1) >> lgG)),

@@ -167,6 +167,51 @@ def test_library_fft_matches_hipfft(hipfft_mode, L):
     api.context(0).release_scratch()
 
 
+def test_direct_column_loads_are_bit_identical():
+    """The 1024-point inverse column passes with the spectrum loaded straight into LDS
+    (tuning fft_direct = 1, the default) and through registers (0): the isolated short-range
+    potential of a 512^3 field (L = 1024, every column read) and the Laplacian of a 1024^3
+    field come out the same bit for bit, as do the tile path's smoothed fields (window
+    table applied in LDS) and its peaks."""
+    import torch
+
+    ctx = api.context(0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4096)
+    n = 512
+    f = api.fortran_empty((n, n, n), torch.float32, "cuda")
+    f.copy_(torch.randn((n, n, n), generator=g, device="cuda"))
+    L = 1024
+    d = api.fortran_empty((L, L, L), torch.float32, "cuda")
+    d.copy_(torch.randn((L, L, L), generator=g, device="cuda"))
+    radii = [10.0, 4.0]
+    out = {}
+    try:
+        for direct in (1, 0):
+            ctx.set_tuning(fft_direct=direct)
+            sp = api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)
+            lap = api.compute_laplacian_gpu(d, L, float(L), return_device=True)
+            sm = [api.fortran_empty((L, L, L), torch.float32, "cuda") for _ in radii]
+            mask = api.fortran_empty((L, L, L), torch.int8, "cuda")
+            mask.zero_()
+            pk = api.peak_find_tile(d, radii, [0.13, 0.5], 64, mask, alatt=2.0,
+                                    capacity=1 << 22, smoothed_out=sm)
+            out[direct] = (sp, lap, sm, pk)
+            print(f"fft_direct={direct}: {pk['total']} peaks", flush=True)
+    finally:
+        ctx.set_tuning(fft_direct=1)
+    (sp1, lap1, sm1, pk1), (sp0, lap0, sm0, pk0) = out[1], out[0]
+    assert torch.equal(sp1, sp0)
+    assert torch.equal(lap1, lap0)
+    for a, b in zip(sm1, sm0):
+        assert torch.equal(a, b)
+    assert pk1["total"] == pk0["total"] > 1000
+    for k in ("i", "j", "k", "filt", "delta"):
+        assert torch.equal(pk1[k], pk0[k]), k
+    del out
+    ctx.release_scratch()
+
+
 def test_library_fft_at_2048(hipfft_mode):
     """L = 2048 (8 columns per workgroup, 1024-point rows), where hipFFT's 32-bit plan is
     refused: the isolated short-range potential of a unit impulse is the kernel's Green
