@@ -513,6 +513,78 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r(
     }
 }
 
+// fft_rows_c2r for L = 1024 (M = 512, 8 rows per workgroup) with the rows loaded straight
+// into LDS: each row's first 512 coefficients as four 1 KB direct loads into an unpadded
+// staging layout (row g at g P, as in the spectrum), the Nyquist coefficient X[M] by an
+// ordinary load; the untwist reads the staging layout and writes the padded FFT layout over
+// it after a barrier.  Same arithmetic as fft_rows_c2r, so the same values bit for bit.
+__global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r_direct(
+    const float2* __restrict__ spec, int s, int n, float scale, int mode, float coef,
+    float* __restrict__ out, const float2* __restrict__ WM, const float2* __restrict__ WL) {
+    constexpr int lgM = 9, M = 512, G = 8, L = 1024, P = fft_pitch(L);
+    static_assert(G * P <= SMALL_LDS, "staging rows fit the FFT buffer");
+    __shared__ float2 buf[SMALL_LDS];
+    const int LS = padded_len(M);
+    const int row0 = (int)blockIdx.x * G;
+    const int nrows = n * n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;  // 8 wavefronts
+    // wavefront w: chunks 4 w' + q of rows ... : 32 chunks (8 rows x 4) over 8 wavefronts
+    for (int u = wave; u < 4 * G; u += SMALL_T / 64) {
+        const int g = u >> 2, q = u & 3;
+        int row = row0 + g;
+        if (row >= nrows) row = nrows - 1;  // a row past the end: loads a kept row, unused
+        const int iy = s + row % n, iz = s + row / n;
+        const float2* src = spec + (int64_t)P * (iy + (int64_t)L * iz) + 128 * q + 2 * lane;
+        __builtin_amdgcn_global_load_lds(
+            src, (__attribute__((address_space(3))) void*)(buf + g * P + 128 * q), 16, 0, 0);
+    }
+    float2 nyq = make_float2(0.0f, 0.0f);
+    if (threadIdx.x < G && row0 + (int)threadIdx.x < nrows) {
+        const int row = row0 + threadIdx.x;
+        const int iy = s + row % n, iz = s + row / n;
+        nyq = spec[M + (int64_t)P * (iy + (int64_t)L * iz)];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x < G) buf[threadIdx.x * P + M] = nyq;
+    __syncthreads();
+    constexpr int MAXB = SMALL_DATA / SMALL_T;
+    float2 z[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int e = (int)threadIdx.x + q * SMALL_T;
+        const int g = e >> lgM, k = e & (M - 1);
+        float2 a = buf[g * P + k];
+        float2 b = buf[g * P + (M - k)];
+        if (k == 0) {
+            a.y = 0.0f;
+            b.y = 0.0f;
+        }
+        const float2 sum = make_float2(a.x + b.x, a.y - b.y);   // a + conj b
+        const float2 dif = make_float2(a.x - b.x, a.y + b.y);   // a - conj b
+        const float2 t = ctw<true>(dif, WL[k]);                 // dif W_L^-k
+        z[q] = make_float2(sum.x - t.y, sum.y + t.x);           // sum + i t
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int e = (int)threadIdx.x + q * SMALL_T;
+        buf[(e >> lgM) * LS + pad(e & (M - 1))] = z[q];
+    }
+    __syncthreads();
+    stockham<true, SMALL_T, SMALL_DATA>(buf, LS, G, lgM, WM);
+    for (int e = threadIdx.x; e < G * n; e += SMALL_T) {
+        const int g = e / n, xo = e - g * n, row = row0 + g;
+        if (row >= nrows) continue;
+        const int xx = s + xo;
+        const float2 zz = buf[g * LS + pad(xx >> 1)];
+        const float v = ((xx & 1) ? zz.y : zz.x) * scale;
+        const int64_t o = (int64_t)row * n + xo;
+        if (mode == 0) out[o] = v;
+        else out[o] = out[o] - (v * v) * coef;
+    }
+}
+
 // sqrt P per (ix, |iy|, |iz|) of the transfer: the (L/2 + 1)^3 table the inverse z pass reads
 // (rows of equal k^2 share it; k^2 as _transfer_kernel! computes it).
 __global__ void fft_amp(float* __restrict__ amp, int L, double dk, const float* __restrict__ tab,
@@ -665,6 +737,12 @@ hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, in
                       s + n, none, WL, ctx->fft_direct != 0);
     const int lgG = rows_lgG(M);
     const int64_t rows = (int64_t)n * n;
+    if (ctx->fft_direct && L == 1024 && lgG == 3) {
+        hipLaunchKernelGGL(fft_rows_c2r_direct, dim3((unsigned)((rows + 7) >> 3)),
+                           dim3(SMALL_T), 0, st, (const float2*)work, s, n, scale, mode, coef,
+                           out, WM, WL);
+        return HNT_OK;
+    }
     hipLaunchKernelGGL(fft_rows_c2r, dim3((unsigned)((rows + (1 << lgG) - 1) >> lgG)),
                        dim3(SMALL_T), 0, st, (const float2*)work, ilog2(M), lgG, s, n, scale,
                        mode, coef, out, WM, WL);
