@@ -345,7 +345,7 @@ static void release_scratch(hnt_ctx* ctx) {
     release_work(ctx->sa);
     for (DevBuf* b : {&ctx->claim, &ctx->counts, &ctx->offsets, &ctx->cub_tmp, &ctx->fb_spec,
                       &ctx->fb_work, &ctx->fb_field, &ctx->fb_lap, &ctx->fb_val, &ctx->fb_d2,
-                      &ctx->fb_wtab,
+                      &ctx->fb_wtab, &ctx->fb_taken,
                       &ctx->rec_tabs, &ctx->rec_keep, &ctx->rec_slot, &ctx->fs_real,
                       &ctx->fs_base, &ctx->fs_work, &ctx->fs_aux, &ctx->fs_tab})
         release(*b);

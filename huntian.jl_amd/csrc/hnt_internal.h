@@ -117,7 +117,7 @@ struct hnt_ctx {
     hnt::MlpScratch mlp;
     hnt::DevBuf tiles, tilemeta, slots, gtiles;  // multi-tile shell analysis
     // tile peak finding: spectrum, windowed spectrum, smoothed fields, per-cell peak values
-    hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2, fb_wtab;
+    hnt::DevBuf fb_spec, fb_work, fb_field, fb_lap, fb_val, fb_d2, fb_wtab, fb_taken;
     hnt::DevBuf rec_tabs, rec_keep, rec_slot;  // halo records: tables, flags, slots
     hnt::MergeBufs mg;                         // catalog merge
     // tile field synthesis: hipFFT plan pairs by size {n, r2c, c2r}, real and two complex

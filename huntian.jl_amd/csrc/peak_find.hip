@@ -62,10 +62,12 @@ __device__ __forceinline__ float from_lane_above(float v) {
 // three rows) and the k direction keeps the last three planes' maxima in registers.
 // One launch runs filters fbase .. fbase + nfilt - 1 (fields[f] is filter fbase + f).  The
 // cells not to claim come from the mask (the caller cleared the claim array), or, in the
-// TILE instance with from_claim = 1, from the claim array itself, read only where a peak is
-// found (masked cells hold BLOCKED, cells of earlier launches their filter); there a claimed
-// cell also records its field value in cellval (and, with celld2, the value of
-// lapfields[f]), indexed like the claim array.
+// TILE instance with from_claim = 1, from the bitset `takenw` that carries the taken bits
+// from one launch to the next (one 32-bit word per lane column, 8 rows and 4 planes, as the
+// registers hold them: masked cells, and peaks of earlier launches); a hit there reads
+// nothing, so no wavefront waits on a load in the claim path.  A claimed cell also records
+// its field value in cellval (and, with celld2, the value of lapfields[f]), indexed like
+// the claim array.
 template <bool TILE>
 __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __restrict__ fields,
                                                const float* __restrict__ fcrit, int nfilt,
@@ -75,7 +77,8 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
                                                uint8_t* __restrict__ claim,
                                                float* __restrict__ cellval,
                                                const float* const* __restrict__ lapfields,
-                                               float* __restrict__ celld2) {
+                                               float* __restrict__ celld2,
+                                               uint32_t* __restrict__ takenw) {
     const int m = n - 2 * nbuff;
     // the wave index is uniform across the wavefront: say so, so row offsets stay scalar
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -99,8 +102,17 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
     // Cells already taken (masked, or a peak of an earlier filter): bit 8*(kk%4)+jr of
     // word kk/4.  The words rotate by one per group of 4 planes, so word 0 is always current.
     uint32_t taken[TK / 4];
+    // word of bitset `takenw` for 4-plane group q of this lane's column (TILE, from_claim)
+    const int NJ = (int)gridDim.y * WJ, NKG = (int)gridDim.z * (TK / 4);
+    auto word = [&](int q) -> int64_t {
+        return (int64_t)io + (int64_t)m * ((j0 / TJ) + (int64_t)NJ * (k0 / 4 + q));
+    };
 #pragma unroll
     for (int q = 0; q < TK / 4; ++q) {
+        if (TILE && from_claim) {
+            taken[q] = out_i ? takenw[word(q)] : 0xFFFFFFFFu;
+            continue;
+        }
         uint32_t bits = 0;
 #pragma unroll 4
         for (int e = 0; e < 32; ++e) {
@@ -175,8 +187,10 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
                 __syncthreads();
                 reduce(cur, mn);
                 uint32_t hit = 0, nout = 0;
+                float cv[TJ];  // plane kk's cell values, the claimed cells' FcRf
 #pragma unroll
                 for (int r = 0; r < TJ; ++r) {
+                    cv[r] = cc[r];
                     hit |= (uint32_t)!(cc[r] < mn[r]) << (8 * u + r);
                     const float c = cur[r + 1];
                     nout |= (uint32_t)((c < fc) || (c < fmaxf(mx[r], mn[r]))) << r;
@@ -190,15 +204,15 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
                     for (int r = 0; r < TJ; ++r)
                         if (hit & (1u << (8 * u + r))) {
                             const int64_t ci = io + (j0 + r) * mm + (k0 + kk) * mm2;
-                            // masked (BLOCKED) or taken by an earlier launch's filter
-                            if (TILE && from_claim && claim[ci] != NONE) continue;
                             claim[ci] = (uint8_t)(fbase + f);
                             if (TILE && cellval) {
-                                const int64_t a = (int64_t)(io + fb) +
-                                                  (int64_t)(j0 + r + fb) * fside +
-                                                  (k0 + kk + fb) * fs2;
-                                cellval[ci] = F[a];
-                                if (celld2) celld2[ci] = lapfields[f][a];
+                                cellval[ci] = cv[r];
+                                if (celld2) {
+                                    const int64_t a = (int64_t)(io + fb) +
+                                                      (int64_t)(j0 + r + fb) * fside +
+                                                      (k0 + kk + fb) * fs2;
+                                    celld2[ci] = lapfields[f][a];
+                                }
                             }
                         }
                 }
@@ -209,6 +223,37 @@ __global__ __launch_bounds__(PF_THREADS) void pf_mark(const float* const* __rest
             for (int q = 0; q + 1 < TK / 4; ++q) taken[q] = taken[q + 1];
             taken[TK / 4 - 1] = t0;
         }
+    }
+    // the words have rotated through a whole cycle: taken[q] is group q again
+    if (TILE && from_claim && out_i) {
+#pragma unroll
+        for (int q = 0; q < TK / 4; ++q) takenw[word(q)] = taken[q];
+    }
+    (void)NKG;
+}
+
+// Bitset of pf_mark's TILE launches before the first filter: for every lane column io of
+// the inner cube, 8-row block jb and 4-plane group kg (the grid's, some past the cube), bit
+// 8 kk + jr = cell (io, 8 jb + jr, 4 kg + kk) is outside the inner cube or masked.
+__global__ __launch_bounds__(256) void pf_taken_init(const int8_t* __restrict__ mask, int n,
+                                                     int nbuff, int NJ, int NKG,
+                                                     uint32_t* __restrict__ takenw) {
+    const int m = n - 2 * nbuff;
+    const int64_t nw = (int64_t)m * NJ * NKG, s2 = (int64_t)n * n;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        const int io = (int)(w % m);
+        const int64_t r = w / m;
+        const int jb = (int)(r % NJ), kg = (int)(r / NJ);
+        uint32_t bits = 0;
+        for (int e = 0; e < 32; ++e) {
+            const int jj = 8 * jb + (e & 7), kq = 4 * kg + (e >> 3);
+            const bool t = jj >= m || kq >= m ||
+                           mask[(int64_t)(io + nbuff) + (int64_t)(jj + nbuff) * n +
+                                (kq + nbuff) * s2] == 1;
+            bits |= (uint32_t)t << e;
+        }
+        takenw[w] = bits;
     }
 }
 
@@ -526,7 +571,7 @@ extern "C" hnt_status hnt_peak_find(hnt_ctx* ctx, const float* const* smoothed,
     HNT_HIP(hipMemsetAsync(ctx->claim.p, NONE, (size_t)ncell, st));
     hipLaunchKernelGGL(pf_mark<false>, mark_grid(m), dim3(PF_THREADS), 0, st, dfields,
                        (const float*)ctx->fcrit.p, nfilt, 0, n, nbuff, n, 0, (const int8_t*)mask, 0,
-                       (uint8_t*)ctx->claim.p, nullptr, nullptr, nullptr);
+                       (uint8_t*)ctx->claim.p, nullptr, nullptr, nullptr, nullptr);
     stage_end(ctx, HNT_STAGE_PF_MARK);
     return compact_claims(ctx, "hnt_peak_find", n, nbuff, nfilt, tmp_bytes, dfields, nullptr,
                           nullptr, mask, out, nullptr);
@@ -580,6 +625,11 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
                 "hnt_peak_find_tile: a %d^3 tile needs the library FFT (power-of-two n, tuning "
                 "fft = 0)", n);
     if (own) HNT_TRY(reserve(ctx, ctx->fb_wtab, sizeof(float) * (size_t)nwtab));
+    // taken bits carried from filter to filter (one word per lane column, 8 rows, 4 planes)
+    const dim3 mg = mark_grid(m);
+    const int NJ = (int)mg.y * WJ, NKG = (int)mg.z * (TK / 4);
+    const int64_t ntw = (int64_t)m * NJ * NKG;
+    HNT_TRY(reserve(ctx, ctx->fb_taken, sizeof(uint32_t) * (size_t)ntw));
     hipfftHandle r2c = nullptr, c2r = nullptr;
     if (!own) {
         if (ctx->fft_n != n) {
@@ -617,6 +667,8 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
     stage_end(ctx, HNT_STAGE_PF_SMOOTH);
     hipLaunchKernelGGL(pf_claim_init, dim3(2048), dim3(256), 0, st, (const int8_t*)mask, n,
                        nbuff, (uint8_t*)ctx->claim.p);
+    hipLaunchKernelGGL(pf_taken_init, dim3(2048), dim3(256), 0, st, (const int8_t*)mask, n,
+                       nbuff, NJ, NKG, (uint32_t*)ctx->fb_taken.p);
     const double dk = 2.0 * M_PI / ((double)n * bank->alatt);
     const float inv_n3 = (float)(1.0 / (double)nreal);
     const int64_t nwin = (int64_t)(n / 2 + 1) * (n / 2 + 1) * (n / 2 + 1);
@@ -665,11 +717,11 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         }
         stage_end(ctx, HNT_STAGE_PF_SMOOTH);
         stage_begin(ctx, HNT_STAGE_PF_MARK);
-        hipLaunchKernelGGL(pf_mark<true>, mark_grid(m), dim3(PF_THREADS), 0, st, dptrs,
+        hipLaunchKernelGGL(pf_mark<true>, mg, dim3(PF_THREADS), 0, st, dptrs,
                            (const float*)ctx->fcrit.p + f, 1, f, n, nbuff, fside, fshift,
                            (const int8_t*)mask, 1, (uint8_t*)ctx->claim.p,
                            (float*)ctx->fb_val.p, lap ? dptrs + 1 : nullptr,
-                           lap ? (float*)ctx->fb_d2.p : nullptr);
+                           lap ? (float*)ctx->fb_d2.p : nullptr, (uint32_t*)ctx->fb_taken.p);
         stage_end(ctx, HNT_STAGE_PF_MARK);
     }
     return compact_claims(ctx, "hnt_peak_find_tile", n, nbuff, nfilt, tmp_bytes, dptrs,
