@@ -387,18 +387,18 @@ __device__ __forceinline__ float window32(double k2, double Rf, int wsmooth) {
 // fb_window leaves the spectrum as it is: v * 1 = v).  The library's inverse transform reads
 // it as it first reads the spectrum (FFT_OP_WINDOW), so the windowed spectrum is never
 // written: 0.54 GB of table instead of 8.6 GB of spectrum traffic per filter at 1024^3.
+// One workgroup per (|ky|, |kz|) row pair of the table: blockIdx.x + nk blockIdx.y is
+// ay + nk az for rows ay < nk, az < nk (no 64-bit division per element).
 __global__ __launch_bounds__(256) void fb_wtable(int n, double dk, double Rf, int wsmooth,
                                                  float* __restrict__ w) {
     const int nk = n / 2 + 1;
-    const int64_t total = (int64_t)nk * nk * nk;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int ix = (int)(e % nk);
-        const int64_t rem = e / nk;
-        const int ay = (int)(rem % nk), az = (int)(rem / nk);
-        const double kx = (double)ix * dk, ky = (double)ay * dk, kz = (double)az * dk;
+    const int ay = (int)blockIdx.x, az = (int)blockIdx.y;
+    const double ky = (double)ay * dk, kz = (double)az * dk;
+    float* row = w + (int64_t)nk * (ay + (int64_t)nk * az);
+    for (int ix = threadIdx.x; ix < nk; ix += 256) {
+        const double kx = (double)ix * dk;
         const double k2 = kx * kx + ky * ky + kz * kz;
-        w[e] = k2 != 0.0 ? window32(k2, Rf, wsmooth) : 1.0f;
+        row[ix] = k2 != 0.0 ? window32(k2, Rf, wsmooth) : 1.0f;
     }
 }
 
@@ -673,7 +673,6 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
     const float inv_n3 = (float)(1.0 / (double)nreal);
     const int64_t nwin = (int64_t)(n / 2 + 1) * (n / 2 + 1) * (n / 2 + 1);
     const unsigned wblocks = (unsigned)std::min<int64_t>((nwin + 255) / 256, 16384);
-    const unsigned tblocks = (unsigned)std::min<int64_t>((nwtab + 255) / 256, 16384);
     for (int f = 0; f < nfilt; ++f) {
         // The peak test reads the inner cube and its one-cell halo, (m + 2)^3 cells from tile
         // index nbuff - 1: the library's inverse transform writes only those (the same values
@@ -684,8 +683,8 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         // spectrum times the table (and 1/n^3) into `dst`
         auto window_inverse = [&](int wsmooth, float* dst) -> hnt_status {
             float* wtab = (float*)ctx->fb_wtab.p;
-            hipLaunchKernelGGL(fb_wtable, dim3(tblocks), dim3(256), 0, st, n, dk, bank->Rf[f],
-                               wsmooth, wtab);
+            hipLaunchKernelGGL(fb_wtable, dim3((unsigned)(n / 2 + 1), (unsigned)(n / 2 + 1)),
+                               dim3(256), 0, st, n, dk, bank->Rf[f], wsmooth, wtab);
             FftOp wop;
             wop.kind = FFT_OP_WINDOW;
             wop.L = n;
