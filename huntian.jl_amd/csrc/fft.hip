@@ -178,11 +178,12 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
     __syncthreads();
 }
 
-// FFT of length 2^lgL of every column, in place, natural order in and out.
+// FFT of length 2^lgL of every column, in place, natural order in and out (from stage
+// lgNs0 on: the caller has run the stages before it).
 template <bool INV, int NT, int DATA, bool RM = false>
 __device__ __forceinline__ void stockham(float2* buf, int LS, int ncol, int lgL,
-                                         const float2* __restrict__ W) {
-    int lgNs = 0;
+                                         const float2* __restrict__ W, int lgNs0 = 0) {
+    int lgNs = lgNs0;
     while (lgNs < lgL) {
         const int rem = lgL - lgNs;
         if (rem == 1) {
@@ -373,18 +374,27 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (op.kind != FFT_OP_NONE) {
+        // the first radix-8 stage (span 1, no twiddles) with the operator applied to its
+        // inputs as they are read: thread (c, j) takes elements j + 128 r of column c, the
+        // very elements whose table entries it loaded (t[r])
+        static_assert(EPT == 8 && NT == 1024, "one radix-8 butterfly per thread");
+        const int c = (int)threadIdx.x & 7, j = (int)threadIdx.x >> 3, x = x0 + c;
+        float2 v[8];
 #pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = (int)threadIdx.x + q * NT;
-            const int c = e & (C - 1), a = e >> 3, x = x0 + c;
-            if (x < nx) {
-                float2& v = buf[rm_slot(c, a)];
-                v = apply_op(op, v, x, b, a, t[q]);
-            }
+        for (int r = 0; r < 8; ++r) {
+            const int a = j + 128 * r;
+            v[r] = buf[rm_slot(c, a)];
+            if (x < nx) v[r] = apply_op(op, v[r], x, b, a, t[r]);
         }
+        dft<8, INV>(v);
         __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[rm_slot(c, 8 * j + r)] = v[r];
+        __syncthreads();
+        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, W, 3);
+    } else {
+        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, W);
     }
-    stockham<INV, NT, C * L, true>(buf, 0, C, lgL, W);
     for (int e = threadIdx.x; e < C * L; e += NT) {
         const int c = e & (C - 1), a = e >> 3, x = x0 + c;
         if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[rm_slot(c, a)];
@@ -711,10 +721,10 @@ hnt_status fft_forward(hnt_ctx* ctx, const float* f, int w, int L, float2* spec)
     // y: planes iz < w, rows iy < w are filled
     const int64_t P = fft_pitch(L);
     launch_cols<false>(st, L, gx, w, spec, spec, P, P * L, nk, 0, 0, w, 0,
-                       L, none, WL);
+                       L, none, WL, ctx->fft_direct != 0);
     // z: every (ix, iy) column, planes iz < w are filled
     launch_cols<false>(st, L, gx, L, spec, spec, P * L, P, nk, 0, 0, w, 0,
-                       L, none, WL);
+                       L, none, WL, ctx->fft_direct != 0);
     return HNT_OK;
 }
 
