@@ -235,10 +235,21 @@ __global__ void adam_step(float* __restrict__ p, const float* __restrict__ g,
 
 // ------------------------------------------------------------------ fused inference
 // All four layers in one kernel: the BF16 weights (80 KB) are staged in LDS once per
-// workgroup; each wavefront then pushes tiles of 16 samples through the network, keeping
-// the 16 x 128 activations in its own LDS tile.  Inputs are normalised and rounded to BF16
+// workgroup; each wavefront then pushes groups of MT = 4 tiles of 16 samples through the
+// network, keeping their 64 x 128 activations in its own LDS tile.  Every B operand read
+// from LDS serves the MT tiles' matrix products.  Inputs are normalised and rounded to BF16
 // on the fly (the A operand of layer 1 comes straight from registers).
 constexpr int FW = 4;  // wavefronts per workgroup
+constexpr int MT = 4;  // 16-sample tiles per wavefront pass
+
+// Inference activations: SiLU with the hardware exp2 and reciprocal (a few FP32 ulps from
+// the correctly rounded form, far below the BF16 rounding that follows) and the hardware
+// round-to-nearest-even BF16 conversion.  The pass is bound by these per-element VALU
+// operations, not by the matrix products.
+__device__ __forceinline__ float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ __bf16 act_bf16(float f) { return (__bf16)f; }
 
 // the wave's LDS writes are complete and visible to its other lanes before later reads
 __device__ __forceinline__ void wave_lds_fence() {
@@ -253,7 +264,7 @@ __global__ __launch_bounds__(64 * FW) void mlp_forward_fused(
     const float* __restrict__ P, float* __restrict__ out) {
     __shared__ __attribute__((aligned(16))) __bf16 sw1[H * KI], sw2[H * H], sw3[H * H],
         sw4[KO * H];
-    __shared__ __attribute__((aligned(16))) __bf16 act[FW][16 * H];
+    __shared__ __attribute__((aligned(16))) __bf16 act[FW][MT * 16 * H];
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     // stage the weights (16-byte chunks)
     for (int q = tid; q < H * KI / 8; q += 64 * FW)
@@ -269,91 +280,112 @@ __global__ __launch_bounds__(64 * FW) void mlp_forward_fused(
     const float* b2 = P + OFF_B2;
     const float* b3 = P + OFF_B3;
     const float* b4 = P + OFF_B4;
-    __bf16* A = act[wave];
+    __bf16* A = act[wave];  // tile mt's 16 x H activations at A + mt * 16 * H
     const int rl = l & 15, kq = 8 * (l >> 4);
-    const int64_t ntiles = (B + 15) / 16;
-    for (int64_t t = (int64_t)blockIdx.x * FW + wave; t < ntiles; t += (int64_t)gridDim.x * FW) {
-        const int64_t row = t * 16 + rl;
-        // layer 1: A operand from x (k = kq..kq+7 of this lane's row)
-        bf16x8 a0;
+    const int64_t ngroups = (B + 16 * MT - 1) / (16 * MT);
+    for (int64_t gi = (int64_t)blockIdx.x * FW + wave; gi < ngroups;
+         gi += (int64_t)gridDim.x * FW) {
+        const int64_t row0 = gi * 16 * MT;
+        // layer 1: A operands from x (k = kq..kq+7 of this lane's row of each tile)
+        bf16x8 a0[MT];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = kq + e;
-            float v = 0.0f;
-            if (row < B && k < NI) v = (x[row * NI + k] - norm[k]) / norm[NI + k];
-            a0[e] = to_bf16(v);
+        for (int mt = 0; mt < MT; ++mt) {
+            const int64_t row = row0 + mt * 16 + rl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kq + e;
+                float v = 0.0f;
+                if (row < B && k < NI) v = (x[row * NI + k] - norm[k]) / norm[NI + k];
+                a0[mt][e] = (__bf16)v;
+            }
         }
 #pragma unroll
         for (int nt = 0; nt < H / 16; ++nt) {
             const bf16x8 bv = *(const bf16x8*)&sw1[(nt * 16 + rl) * KI + kq];
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, c, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = 4 * (l >> 4) + r, n = nt * 16 + rl;
-                A[m * H + n] = to_bf16(silu(c[r] + b1[n]));
+            for (int mt = 0; mt < MT; ++mt) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[mt], bv, c, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = 4 * (l >> 4) + r, n = nt * 16 + rl;
+                    A[mt * 16 * H + m * H + n] = act_bf16(silu_fast(c[r] + b1[n]));
+                }
             }
         }
         wave_lds_fence();
-        // layers 2 and 3: 128 -> 128
+        // layers 2 and 3: 128 -> 128; all A operands in registers before the tile is
+        // overwritten in place
 #pragma unroll
         for (int layer = 0; layer < 2; ++layer) {
             const __bf16* W = layer == 0 ? sw2 : sw3;
             const float* bb = layer == 0 ? b2 : b3;
-            bf16x8 av[H / 32];
+            bf16x8 av[MT][H / 32];
 #pragma unroll
-            for (int ks = 0; ks < H / 32; ++ks) av[ks] = *(const bf16x8*)&A[rl * H + ks * 32 + kq];
-            f32x4 c[H / 16];
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int ks = 0; ks < H / 32; ++ks)
+                    av[mt][ks] = *(const bf16x8*)&A[mt * 16 * H + rl * H + ks * 32 + kq];
+            wave_lds_fence();
 #pragma unroll
             for (int nt = 0; nt < H / 16; ++nt) {
-                c[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                f32x4 c[MT];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) c[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int ks = 0; ks < H / 32; ++ks) {
                     const bf16x8 bv = *(const bf16x8*)&W[(nt * 16 + rl) * H + ks * 32 + kq];
-                    c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[ks], bv, c[nt], 0, 0, 0);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        c[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[mt][ks], bv, c[mt],
+                                                                        0, 0, 0);
                 }
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = 4 * (l >> 4) + r, n = nt * 16 + rl;
+                        A[mt * 16 * H + m * H + n] = act_bf16(silu_fast(c[mt][r] + bb[n]));
+                    }
             }
-#pragma unroll
-            for (int nt = 0; nt < H / 16; ++nt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = 4 * (l >> 4) + r, n = nt * 16 + rl;
-                    A[m * H + n] = to_bf16(silu(c[nt][r] + bb[n]));
-                }
             wave_lds_fence();
         }
         // layer 4: 128 -> 16 (3 used)
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        f32x4 c[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) c[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int ks = 0; ks < H / 32; ++ks) {
-            const bf16x8 av = *(const bf16x8*)&A[rl * H + ks * 32 + kq];
             const bf16x8 bv = *(const bf16x8*)&sw4[rl * H + ks * 32 + kq];
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const bf16x8 av = *(const bf16x8*)&A[mt * 16 * H + rl * H + ks * 32 + kq];
+                c[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c[mt], 0, 0, 0);
+            }
         }
         if (rl < NO) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t m = t * 16 + 4 * (l >> 4) + r;
-                if (m < B) out[m * NO + rl] = c[r] + b4[rl];
-            }
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t m = row0 + mt * 16 + 4 * (l >> 4) + r;
+                    if (m < B) out[m * NO + rl] = c[mt][r] + b4[rl];
+                }
         }
         wave_lds_fence();
     }
 }
 
 // ------------------------------------------------------------------ features
-// Per peak, from the cell records {delta, delta^T, psi1 xyz, psi2 xyz} of shells 1..12:
+// Per peak, from delta over shells 1..12:
 //  f[0..11]  mean delta of shell s (in-bounds cells; 0 if none)
-//  f[12..15] e, p of the symmetrised psi1 gradient sum_cells psi1_L d_K over shells 2..6,
-//            then over shells 2..12 (e = (l3 - l1) / (2 F), p = (l3 + l1 - 2 l2) / (2 F),
+//  f[12..15] e, p of the delta quadrupole sum_cells delta d_L d_K over shells 2..6, then
+//            over shells 2..12 (e = (l3 - l1) / (2 F), p = (l3 + l1 - 2 l2) / (2 F),
 //            F = l1 + l2 + l3; both 0 if F <= 0)
 //  f[16]     mean delta over shells 1..12
 //  f[17]     Rf / alatt
 struct FeatArgs {
     const float* delta;
-    const float* ex;
-    const float* ey;
-    const float* ez;
     int n;
     int64_t s1, s2;
     const int32_t* pi;
@@ -375,12 +407,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
     if (q >= a.npeaks) return;
     const int ci = a.pi[q], cj = a.pj[q], ck = a.pk[q];
     const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * a.s1 + (ck - 1) * a.s2;
-    auto load = [&](int64_t off, float& d, float& ex, float& ey, float& ez) {
-        d = a.delta[off];
-        ex = a.ex[off];
-        ey = a.ey[off];
-        ez = a.ez[off];
-    };
+    auto load = [&](int64_t off) -> float { return a.delta[off]; };
     peak_features(a.cells, a.shell_start, a.shell_count, ci, cj, ck, c0, a.n, a.s1, a.s2,
                   a.Rf[q] / a.alatt, load, a.out + q * NI);
 }
@@ -505,9 +532,10 @@ hnt_status mlp_predict(hnt_ctx* ctx, const hnt_mlp* net, const float* x, int64_t
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(pack_weights, dim3((H * H + 255) / 256), dim3(256), 0, st, net->params,
                        w1t, w2t, w3t, w4t, w1, w2, w3, w4);
-    const int64_t ntiles = (B + 15) / 16;
-    int64_t nblocks = (ntiles + FW - 1) / FW;
-    const int64_t cap = (int64_t)ctx->num_cus * 2;
+    // one workgroup per CU (80 KB of weights and 64 KB of activations in LDS)
+    const int64_t ngroups = (B + 16 * MT - 1) / (16 * MT);
+    int64_t nblocks = (ngroups + FW - 1) / FW;
+    const int64_t cap = (int64_t)ctx->num_cus;
     if (nblocks > cap) nblocks = cap;
     hipLaunchKernelGGL(mlp_forward_fused, dim3((unsigned)nblocks), dim3(64 * FW), 0, st, x, B,
                        net->norm, w1t, w2t, w3t, w4t, net->params, out);
@@ -521,9 +549,6 @@ hnt_status bracket_features(hnt_ctx* ctx, const hnt_fields* fields, const hnt_ba
     if (batch->npeaks == 0) return HNT_OK;
     FeatArgs a{};
     a.delta = fields->delta;
-    a.ex = fields->etax;
-    a.ey = fields->etay;
-    a.ez = fields->etaz;
     a.n = fields->n1;
     a.s1 = fields->n1;
     a.s2 = (int64_t)fields->n1 * fields->n1;

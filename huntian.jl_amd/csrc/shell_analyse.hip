@@ -1696,14 +1696,8 @@ __global__ __launch_bounds__(64) void features_sched_kernel(SAArgs a, int64_t G8
     if (p < 0) return;
     const int ci = T.pi[p], cj = T.pj[p], ck = T.pk[p];
     const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
-    const float4* __restrict__ recs = T.recs;
-    auto load = [&](int64_t off, float& d, float& ex, float& ey, float& ez) {
-        const float4 v0 = recs[2 * off];
-        d = v0.x;
-        ex = v0.z;
-        ey = v0.w;
-        ez = recs[2 * off + 1].x;
-    };
+    const float* __restrict__ delta = T.delta;
+    auto load = [&](int64_t off) -> float { return delta[off]; };
     peak_features(a.cells, a.shell_start, a.shell_count, ci, cj, ck, c0, T.n, T.s1, T.s2,
                   T.Rf[p] / a.alatt, load, feats + (T.base + p) * HNT_MLP_INPUTS);
 }

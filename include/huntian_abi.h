@@ -575,8 +575,9 @@ hnt_status hnt_kernel_strain(hnt_ctx* ctx, const double* rad, const double* G,
 /* ---------------------------------------------------------------- bracket network */
 /* No counterpart in the reference.  An MLP 18 -> 128 -> 128 -> 128 -> 3 with SiLU, run in
  * BF16 on the matrix cores with FP32 accumulation.  Inputs (hnt_bracket_features): mean
- * delta of shells 1..12, (e, p) of the symmetrised psi1 gradient over shells 2..6 and
- * 2..12, mean delta over shells 1..12, Rf / alatt.  Outputs, the RTHL bracket and the
+ * delta of shells 1..12, (e, p) of the delta quadrupole sum delta d_L d_K (d the cell
+ * offset) over shells 2..6 and 2..12, mean delta over shells 1..12, Rf / alatt: delta only
+ * (4 B per cell).  Outputs, the RTHL bracket and the
  * collapse logit: log(1 + lo), lo the shell holding RTHL (RTHL in (rad[lo - 1], rad[lo]],
  * 1 for no collapse); log(1 + hi), hi the gather extent in shells; the collapse logit.
  * Parameters: one FP32 vector, row-major W1 [18x128], b1, W2 [128x128], b2, W3 [128x128],

@@ -107,10 +107,10 @@ def features_ref(f, pi, pj, pk, Rf, alatt, t):
                 i1, i2, i3 = pi[q] + d[0], pj[q] + d[1], pk[q] + d[2]
                 if not (1 <= i1 <= n and 1 <= i2 <= n and 1 <= i3 <= n):
                     continue
-                F += float(f["delta"][i1 - 1, i2 - 1, i3 - 1])
+                dv = float(f["delta"][i1 - 1, i2 - 1, i3 - 1])
+                F += dv
                 nin += 1
-                e = [float(f[k][i1 - 1, i2 - 1, i3 - 1]) for k in ("etax", "etay", "etaz")]
-                T += np.outer(e, d)
+                T += dv * np.outer(d, d)
             out[q, s - 1] = F / nin if nin else 0.0
             tot += F
             ntot += nin
