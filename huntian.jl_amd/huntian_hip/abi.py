@@ -258,6 +258,9 @@ EXPORTED = (
     "hnt_isolated_poisson_psi",
     "hnt_isolated_short_pot",
     "hnt_interpolate_to_tile",
+    "hnt_tile_noise",
+    "hnt_downsample_noise",
+    "hnt_extended_residual",
     "hnt_shell_gather_full",
     "hnt_eig3_symmetric_batch",
     "hnt_interp3_trilinear",
@@ -328,6 +331,11 @@ def load(path: str | None = None):
     lib.hnt_isolated_poisson_psi.argtypes = [vp, _f32p, C.c_int32, C.c_double, fpp]
     lib.hnt_isolated_short_pot.argtypes = [vp, _f32p, C.c_int32, C.c_double, C.c_double, _f32p]
     lib.hnt_interpolate_to_tile.argtypes = [vp, _f32p] + [C.c_int32] * 7 + [_f32p]
+    lib.hnt_tile_noise.argtypes = [vp] + [C.c_int32] * 5 + [C.c_int64, C.c_uint64, _f32p]
+    lib.hnt_downsample_noise.argtypes = [vp, C.c_int64, C.c_int32, C.c_uint64, _f32p]
+    lib.hnt_extended_residual.argtypes = ([vp] + [C.c_int32] * 5 +
+                                          [C.c_int64, C.c_uint64, _f32p, C.c_int32, C.c_int32,
+                                           _f32p])
     lib.hnt_shell_analyse.argtypes = [vp, C.POINTER(Fields), C.POINTER(Batch),
                                       C.POINTER(ShellTables), C.POINTER(CTab),
                                       C.POINTER(Params), C.POINTER(Results), _i8p]

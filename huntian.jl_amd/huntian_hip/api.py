@@ -999,3 +999,50 @@ def interpolate_to_tile_gpu(coarse, it, jt, kt, nsub, nmesh, N, M, *, threads=25
         ctx.handle, abi.as_ptr(x.data_ptr(), C.c_float), int(M), int(N), int(it), int(jt),
         int(kt), int(nsub), int(nmesh), abi.as_ptr(out.data_ptr(), C.c_float)))
     return out if return_device else out.cpu().numpy()
+
+
+# ---------------------------------------------------------------- tile white noise
+
+def generate_tile_noise(it, jt, kt, nsub, nmesh, N, seed, *, return_device=False):
+    """Mirror of _generate_tile_noise (src/MultiResolution.jl:35-57) on the device
+    (hnt_tile_noise): the Threefry Gaussian stream of the global N^3 grid over tile
+    (it, jt, kt)'s nmesh^3 cells (1-based tile indices, wrapped mod N), Float32."""
+    torch = _require_gpu()
+    ctx = context()
+    out = _field_out(int(nmesh), torch.device("cuda", ctx.device))[0]
+    abi.check(ctx.lib, ctx.lib.hnt_tile_noise(
+        ctx.handle, int(it), int(jt), int(kt), int(nsub), int(nmesh), int(N), int(seed),
+        abi.as_ptr(out.data_ptr(), C.c_float)))
+    return out if return_device else out.cpu().numpy()
+
+
+def downsample_noise(N, M, seed, *, return_device=False):
+    """Mirror of _downsample_noise (src/MultiResolution.jl:64-87) on the device
+    (hnt_downsample_noise): each coarse cell the FP64 sum of its (N/M)^3 fine Gaussians over
+    sqrt((N/M)^3), Float32 [M^3]."""
+    torch = _require_gpu()
+    ctx = context()
+    out = _field_out(int(M), torch.device("cuda", ctx.device))[0]
+    abi.check(ctx.lib, ctx.lib.hnt_downsample_noise(
+        ctx.handle, int(N), int(M), int(seed), abi.as_ptr(out.data_ptr(), C.c_float)))
+    return out if return_device else out.cpu().numpy()
+
+
+def generate_extended_residual(it, jt, kt, nsub, nmesh, N, seed, coarse_noise, M, nshell, *,
+                               return_device=False):
+    """Mirror of _generate_extended_residual (src/MultiResolution.jl:190-227) on the device
+    (hnt_extended_residual): the Hoffman-Ribak residual fine - block mean of coarse_noise
+    (the M^3 output of downsample_noise) over the (nmesh + 2 nshell)^3 region around tile
+    (it, jt, kt), Float32."""
+    _require_gpu()
+    c = to_device_field(coarse_noise)
+    if tuple(c.shape) != (int(M),) * 3:
+        raise ValueError(f"coarse_noise must be {M}^3, got {tuple(c.shape)}")
+    ctx = context(c.device.index or 0)
+    side = int(nmesh) + 2 * int(nshell)
+    out = _field_out(side, c.device)[0]
+    abi.check(ctx.lib, ctx.lib.hnt_extended_residual(
+        ctx.handle, int(it), int(jt), int(kt), int(nsub), int(nmesh), int(N), int(seed),
+        abi.as_ptr(c.data_ptr(), C.c_float), int(M), int(nshell),
+        abi.as_ptr(out.data_ptr(), C.c_float)))
+    return out if return_device else out.cpu().numpy()

@@ -539,6 +539,29 @@ hnt_status hnt_interpolate_to_tile(hnt_ctx* ctx, const float* coarse, int32_t M,
                                    int32_t it, int32_t jt, int32_t kt, int32_t nsub,
                                    int32_t nmesh, float* tile);
 
+/* The tile driver's white noise (the Threefry-2x64-20 Gaussian stream of the global N^3
+ * grid, RandomField._threefry_gaussian, src/InitialConditions/RandomField.jl:45-53: cell idx
+ * (0-based, column-major) takes Gaussian idx & 1 of the Box-Muller pair of counter
+ * (idx >> 1, 0) under key (seed, 0)).  Outputs are device Float32, column-major.
+ *
+ * hnt_tile_noise -- replaces _generate_tile_noise (src/MultiResolution.jl:35-57): the
+ *   nmesh^3 cells of tile (it, jt, kt) (1-based), starting at global cell
+ *   (it-1) nsub + 1 - (nmesh - nsub) / 2 on each axis, wrapped mod N.
+ * hnt_downsample_noise -- replaces _downsample_noise (src/MultiResolution.jl:64-87): coarse
+ *   [M^3], each the FP64 sum of its block^3 fine Gaussians (block = N / M, N a multiple of
+ *   M) in the reference's loop order, divided by sqrt(block^3).
+ * hnt_extended_residual -- replaces _generate_extended_residual (src/MultiResolution.jl:
+ *   190-227): the (nmesh + 2 nshell)^3 cells around tile (it, jt, kt), each its Float32
+ *   Gaussian less coarse[block cell] / sqrt(block^3) (Float32), coarse a device [M^3] array
+ *   (the output of hnt_downsample_noise). */
+hnt_status hnt_tile_noise(hnt_ctx* ctx, int32_t it, int32_t jt, int32_t kt, int32_t nsub,
+                          int32_t nmesh, int64_t N, uint64_t seed, float* out);
+hnt_status hnt_downsample_noise(hnt_ctx* ctx, int64_t N, int32_t M, uint64_t seed,
+                                float* coarse);
+hnt_status hnt_extended_residual(hnt_ctx* ctx, int32_t it, int32_t jt, int32_t kt,
+                                 int32_t nsub, int32_t nmesh, int64_t N, uint64_t seed,
+                                 const float* coarse, int32_t M, int32_t nshell, float* out);
+
 /* ---------------------------------------------------------------- primitive entries */
 /* Single-primitive entry points; the reference exposes them as test harnesses
  * (src/Huntian.jl:98-217). */

@@ -1534,3 +1534,114 @@ void hro_finalize_eulerian(const float* rec, int64_t N, int32_t nf, const hnt_co
         for (int q = 10; q < nf; ++q) o[q] = keep[q];
     }
 }
+
+/* ------------------------------------------------------------------ tile white noise */
+
+/* Threefry-2x64 with 20 rounds (Salmon, Moraes, Dror, Shaw, SC'11, the Random123 generator
+ * RandomField.jl calls as threefry((seed, 0), (pair, 0), Val(20)), key first): key words
+ * k0, k1 and the parity word 0x1BD11BDAA9FC1A22 ^ k0 ^ k1; each round adds word 1 into
+ * word 0, rotates word 1 left by 16 42 12 31 16 32 24 21 (round mod 8) and xors word 0 into
+ * it; after rounds 4, 8, .., 20 the key word s mod 3 is added to word 0 and key word
+ * (s + 1) mod 3 plus s to word 1 (s = 1 .. 5); the words start as counter + key. */
+void hro_threefry2x64_20(uint64_t c0, uint64_t c1, uint64_t k0, uint64_t k1, uint64_t* out) {
+    static const int rot[8] = {16, 42, 12, 31, 16, 32, 24, 21};
+    const uint64_t ks[3] = {k0, k1, 0x1BD11BDAA9FC1A22ull ^ k0 ^ k1};
+    uint64_t x0 = c0 + ks[0], x1 = c1 + ks[1];
+    for (int r = 0; r < 20; ++r) {
+        x0 += x1;
+        x1 = (x1 << rot[r % 8]) | (x1 >> (64 - rot[r % 8]));
+        x1 ^= x0;
+        if (r % 4 == 3) {
+            int s = (r + 1) / 4;
+            x0 += ks[s % 3];
+            x1 += ks[(s + 1) % 3] + (uint64_t)s;
+        }
+    }
+    out[0] = x0;
+    out[1] = x1;
+}
+
+/* _threefry_gaussian (RandomField.jl:45-53) for the cell of 0-based global index idx0:
+ * g1 for an even index, g2 for an odd one, of pair idx0 >> 1. */
+double hro_threefry_gauss(uint64_t seed, uint64_t idx0) {
+    uint64_t w[2];
+    hro_threefry2x64_20(idx0 >> 1, 0, seed, 0, w);
+    double u1 = ((double)(w[0] >> 11) + 1.0) * 1.1102230246251565e-16; /* (x + 1) 2^-53 */
+    double u2 = ((double)(w[1] >> 11) + 1.0) * 1.1102230246251565e-16;
+    double r = sqrt(-2.0 * log(u1));
+    double th = (2.0 * 3.141592653589793) * u2; /* 2 pi in Float64 */
+    return (idx0 % 2 == 0) ? r * cos(th) : r * sin(th);
+}
+
+static int64_t mod1_(int64_t x, int64_t n) { /* Julia's mod1 */
+    int64_t m = (x - 1) % n;
+    if (m < 0) m += n;
+    return m + 1;
+}
+
+/* _generate_tile_noise (MultiResolution.jl:35-57) and _generate_extended_residual
+ * (:190-227): the side^3 region from global cell (i0, j0, k0), 1-based, wrapped; with
+ * coarse, Float32 fine value less Float32 coarse block value times Float32(1/sqrt(block^3)). */
+static void region_noise(int64_t i0, int64_t j0, int64_t k0, int64_t side, int64_t N,
+                         uint64_t seed, const float* coarse, int64_t M, float* out) {
+    int64_t block = coarse ? N / M : 1;
+    double b = (double)block;
+    float inv = (float)(1.0 / sqrt(b * b * b));
+#pragma omp parallel for schedule(static)
+    for (int64_t lk = 1; lk <= side; ++lk)
+        for (int64_t lj = 1; lj <= side; ++lj)
+            for (int64_t li = 1; li <= side; ++li) {
+                int64_t gi = mod1_(i0 + li - 1, N), gj = mod1_(j0 + lj - 1, N);
+                int64_t gk = mod1_(k0 + lk - 1, N);
+                int64_t idx = gi + (gj - 1) * N + (gk - 1) * N * N;
+                float fine = (float)hro_threefry_gauss(seed, (uint64_t)(idx - 1));
+                int64_t o = (li - 1) + (lj - 1) * side + (lk - 1) * side * side;
+                if (!coarse) {
+                    out[o] = fine;
+                } else {
+                    int64_t ci = (gi - 1) / block + 1, cj = (gj - 1) / block + 1;
+                    int64_t ck = (gk - 1) / block + 1;
+                    float mean = coarse[(ci - 1) + (cj - 1) * M + (ck - 1) * M * M] * inv;
+                    out[o] = fine - mean;
+                }
+            }
+}
+
+void hro_tile_noise(int32_t it, int32_t jt, int32_t kt, int32_t nsub, int32_t nmesh, int64_t N,
+                    uint64_t seed, float* out) {
+    int64_t nbuff = (nmesh - nsub) / 2;
+    region_noise((int64_t)(it - 1) * nsub + 1 - nbuff, (int64_t)(jt - 1) * nsub + 1 - nbuff,
+                 (int64_t)(kt - 1) * nsub + 1 - nbuff, nmesh, N, seed, NULL, 1, out);
+}
+
+void hro_extended_residual(int32_t it, int32_t jt, int32_t kt, int32_t nsub, int32_t nmesh,
+                           int64_t N, uint64_t seed, const float* coarse, int32_t M,
+                           int32_t nshell, float* out) {
+    int64_t nbuff = (nmesh - nsub) / 2;
+    region_noise((int64_t)(it - 1) * nsub + 1 - nbuff - nshell,
+                 (int64_t)(jt - 1) * nsub + 1 - nbuff - nshell,
+                 (int64_t)(kt - 1) * nsub + 1 - nbuff - nshell, nmesh + 2 * nshell, N, seed,
+                 coarse, M, out);
+}
+
+/* _downsample_noise (MultiResolution.jl:64-87): FP64 block sums in (dk, dj, di) order. */
+void hro_downsample_noise(int64_t N, int32_t M, uint64_t seed, float* coarse) {
+    int64_t block = N / M;
+    double b = (double)block;
+    double norm = sqrt(b * b * b);
+#pragma omp parallel for schedule(static)
+    for (int64_t K = 1; K <= M; ++K)
+        for (int64_t J = 1; J <= M; ++J)
+            for (int64_t I = 1; I <= M; ++I) {
+                double val = 0.0;
+                for (int64_t dk = 0; dk < block; ++dk)
+                    for (int64_t dj = 0; dj < block; ++dj)
+                        for (int64_t di = 0; di < block; ++di) {
+                            int64_t gi = (I - 1) * block + di + 1, gj = (J - 1) * block + dj + 1;
+                            int64_t gk = (K - 1) * block + dk + 1;
+                            int64_t idx = gi + (gj - 1) * N + (gk - 1) * N * N;
+                            val += hro_threefry_gauss(seed, (uint64_t)(idx - 1));
+                        }
+                coarse[(I - 1) + (J - 1) * M + (K - 1) * M * M] = (float)(val / norm);
+            }
+}

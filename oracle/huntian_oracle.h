@@ -94,6 +94,18 @@ int32_t hro_merge_passes(const float* rec, int64_t nhalo, int32_t nf, int32_t vo
 void hro_sphere_overlap(double d, double r1, double r2, double* v1, double* v2);
 double hro_chi_to_z(const hnt_cosmo_tables* t, double chi);  /* chi_to_z */
 
+/* tile white noise: Threefry-2x64-20 (key k0 k1, counter c0 c1), the Gaussian of a global
+ * cell, _generate_tile_noise, _downsample_noise, _generate_extended_residual
+ * (src/InitialConditions/RandomField.jl:45-53, src/MultiResolution.jl:35-87,190-227) */
+void hro_threefry2x64_20(uint64_t c0, uint64_t c1, uint64_t k0, uint64_t k1, uint64_t* out);
+double hro_threefry_gauss(uint64_t seed, uint64_t idx0);
+void hro_tile_noise(int32_t it, int32_t jt, int32_t kt, int32_t nsub, int32_t nmesh, int64_t N,
+                    uint64_t seed, float* out);
+void hro_downsample_noise(int64_t N, int32_t M, uint64_t seed, float* coarse);
+void hro_extended_residual(int32_t it, int32_t jt, int32_t kt, int32_t nsub, int32_t nmesh,
+                           int64_t N, uint64_t seed, const float* coarse, int32_t M,
+                           int32_t nshell, float* out);
+
 #ifdef __cplusplus
 }
 #endif

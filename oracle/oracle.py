@@ -98,6 +98,14 @@ def lib():
         L.hro_chi_to_z.restype = C.c_double
         L.hro_pow_m3.argtypes = [C.c_double]
         L.hro_pow_m3.restype = C.c_double
+        L.hro_threefry2x64_20.argtypes = [C.c_uint64] * 4 + [C.POINTER(C.c_uint64)]
+        L.hro_threefry_gauss.argtypes = [C.c_uint64, C.c_uint64]
+        L.hro_threefry_gauss.restype = C.c_double
+        L.hro_tile_noise.argtypes = [C.c_int32] * 5 + [C.c_int64, C.c_uint64, _f32p]
+        L.hro_downsample_noise.argtypes = [C.c_int64, C.c_int32, C.c_uint64, _f32p]
+        L.hro_extended_residual.argtypes = ([C.c_int32] * 5 +
+                                            [C.c_int64, C.c_uint64, _f32p, C.c_int32,
+                                             C.c_int32, _f32p])
         _lib = L
     return _lib
 
@@ -428,3 +436,36 @@ def sphere_overlap(d, r1, r2):
     v1, v2 = C.c_double(), C.c_double()
     lib().hro_sphere_overlap(float(d), float(r1), float(r2), C.byref(v1), C.byref(v2))
     return v1.value, v2.value
+
+
+# ---------------------------------------------------------------- tile white noise
+
+def threefry2x64_20(c0, c1, k0, k1):
+    out = (C.c_uint64 * 2)()
+    lib().hro_threefry2x64_20(c0, c1, k0, k1, out)
+    return int(out[0]), int(out[1])
+
+
+def threefry_gauss(seed, idx0):
+    return lib().hro_threefry_gauss(int(seed), int(idx0))
+
+
+def tile_noise(it, jt, kt, nsub, nmesh, N, seed):
+    out = np.empty((nmesh, nmesh, nmesh), np.float32, order="F")
+    lib().hro_tile_noise(it, jt, kt, nsub, nmesh, int(N), int(seed), _p(out, C.c_float))
+    return out
+
+
+def downsample_noise(N, M, seed):
+    out = np.empty((M, M, M), np.float32, order="F")
+    lib().hro_downsample_noise(int(N), int(M), int(seed), _p(out, C.c_float))
+    return out
+
+
+def extended_residual(it, jt, kt, nsub, nmesh, N, seed, coarse, M, nshell):
+    side = nmesh + 2 * nshell
+    out = np.empty((side, side, side), np.float32, order="F")
+    c = np.asfortranarray(coarse, np.float32)
+    lib().hro_extended_residual(it, jt, kt, nsub, nmesh, int(N), int(seed), _p(c, C.c_float),
+                                int(M), int(nshell), _p(out, C.c_float))
+    return out
