@@ -600,25 +600,26 @@ def cpu_baseline_and_check(args, fields, pf, r, npk, radii, alatt, nhunt, ct, fc
 
 def time_field_synthesis(n, box, nbuff, dev):
     """One tile's fields as the tile driver builds them (src/MultiResolution.jl:881-1010):
-    delta_self and psi1_self from one zero-padded (2n)^3 isolated convolution of the residual
-    noise, the coarse delta and psi1 interpolated onto the tile (M = 256 coarse cells over a
-    N = 4096 global grid, tile (2, 2, 2)), then the periodic 2LPT and Laplacian of the combined
-    delta.  Each piece timed once after a warm call, on synthetic noise and coarse fields."""
+    the tile's Hoffman-Ribak residual noise (Threefry stream of a N = 4096 global grid less
+    the block means of its M = 256 coarse grid, tile (2, 2, 2)), delta_self and psi1_self from
+    one zero-padded (2n)^3 isolated convolution of it, the coarse delta and psi1 interpolated
+    onto the tile, then the periodic 2LPT and Laplacian of the combined delta.  Each piece
+    timed once after a warm call; the coarse grid's downsample (once per run in the driver)
+    timed once; the coarse fields interpolated are synthetic."""
     import torch
     from huntian_hip import api, synth
 
     pk = synth.PowerSpectrum().file
     g = torch.Generator(device=dev)
     g.manual_seed(7)
-    residual = api.fortran_empty((n, n, n), torch.float32, dev)
-    residual.copy_(torch.randn((n, n, n), generator=g, device=dev))
-    M, N, nsub = 256, 4096, n - 2 * nbuff
+    M, N, nsub, seed = 256, 4096, n - 2 * nbuff, 7
     coarse = [torch.randn((M, M, M), generator=g, device=dev) for _ in range(4)]
     kernels = [(0, 0, 0), (1, 1, 0), (1, 2, 0), (1, 3, 0)]
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
 
-    def timed(fn):
-        fn()  # warm (plans, scratch)
+    def timed(fn, warm=True):
+        if warm:
+            fn()  # warm (plans, scratch)
         torch.cuda.synchronize()
         ev[0].record()
         out = fn()
@@ -626,6 +627,11 @@ def time_field_synthesis(n, box, nbuff, dev):
         torch.cuda.synchronize()
         return out, round(ev[0].elapsed_time(ev[1]), 3)
 
+    coarse_noise, t_down = timed(lambda: api.downsample_noise(N, M, seed, return_device=True),
+                                 warm=False)
+    residual, t_res = timed(lambda: api.generate_extended_residual(
+        2, 2, 2, nsub, n, N, seed, coarse_noise, M, 0, return_device=True))
+    del coarse_noise
     selfs, t_conv = timed(lambda: api.isolated_convolve_gpu_multi(
         residual, pk, box, n, kernels=kernels, return_device=True))
     del residual
@@ -643,11 +649,15 @@ def time_field_synthesis(n, box, nbuff, dev):
     del lap, delta_tile
     api.context(0).release_scratch()
     torch.cuda.empty_cache()
-    return dict(isolated_convolve_ms=t_conv, interpolate_ms=t_interp, compute_2lpt_ms=t_2lpt,
-                laplacian_ms=t_lap, total_ms=round(t_conv + t_interp + t_2lpt + t_lap, 3),
-                note=f"{n}^3 tile: isolated convolution of delta + psi1 x3 over the "
-                     f"{2 * n}^3 padded box (1 R2C + 4 C2R), 4 tri-cubic interpolations "
-                     f"from {M}^3, 2LPT (2 R2C + 9 C2R at {n}^3), Laplacian (1 + 1)")
+    return dict(residual_ms=t_res, isolated_convolve_ms=t_conv, interpolate_ms=t_interp,
+                compute_2lpt_ms=t_2lpt, laplacian_ms=t_lap,
+                total_ms=round(t_res + t_conv + t_interp + t_2lpt + t_lap, 3),
+                downsample_once_per_run_ms=t_down,
+                note=f"{n}^3 tile of a {N}^3 grid: Threefry residual noise less the block "
+                     f"means of the {M}^3 coarse grid, isolated convolution of delta + psi1 "
+                     f"x3 over the {2 * n}^3 padded box (1 R2C + 4 C2R), 4 tri-cubic "
+                     f"interpolations from {M}^3, 2LPT (2 R2C + 9 C2R at {n}^3), Laplacian "
+                     f"(1 + 1); the coarse downsample (all {N}^3 cells) runs once per run")
 
 
 def batched_result(args, cfg_name):

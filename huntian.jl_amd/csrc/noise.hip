@@ -8,7 +8,8 @@
 // Threefry-2x64 with 20 rounds (Salmon et al., SC'11, the algorithm of Random123) of counter
 // (pair, 0) under key (seed, 0), the two words turned into doubles in (0, 1] as
 // ((w >> 11) + 1) 2^-53, then Box-Muller in FP64: sqrt(-2 log u1) (cos, sin)(2 pi u2).  Every
-// cell depends only on (seed, idx), so each thread computes its own cell.
+// cell depends only on (seed, idx), so threads compute their cells independently (the two
+// cells of a pair together where they are neighbours).
 #include "hnt_internal.h"
 
 namespace {
@@ -43,15 +44,23 @@ __device__ inline void threefry2x64_20(uint64_t c0, uint64_t c1, uint64_t k0, ui
     o1 = x1;
 }
 
-// The Gaussian of global cell idx0 (0-based linear index), in FP64.
-__device__ inline double gauss_at(uint64_t seed, uint64_t idx0) {
+// Both Gaussians of pair `pair` (cells 2 pair and 2 pair + 1), in FP64.
+__device__ inline void gauss_pair(uint64_t seed, uint64_t pair, double& g1, double& g2) {
     uint64_t r0, r1;
-    threefry2x64_20(idx0 >> 1, 0, seed, 0, r0, r1);
+    threefry2x64_20(pair, 0, seed, 0, r0, r1);
     const double u1 = ((double)(r0 >> 11) + 1.0) * 1.1102230246251565e-16;
     const double u2 = ((double)(r1 >> 11) + 1.0) * 1.1102230246251565e-16;
     const double r = sqrt(-2.0 * log(u1));
     const double th = 6.283185307179586 * u2;  // 2 pi (Float64) times u2
-    return (idx0 & 1) ? r * sin(th) : r * cos(th);
+    g1 = r * cos(th);
+    g2 = r * sin(th);
+}
+
+// The Gaussian of global cell idx0 (0-based linear index), in FP64.
+__device__ inline double gauss_at(uint64_t seed, uint64_t idx0) {
+    double g1, g2;
+    gauss_pair(seed, idx0 >> 1, g1, g2);
+    return (idx0 & 1) ? g2 : g1;
 }
 
 // mod1(x, N) - 1: the 0-based periodic index of 1-based x
@@ -62,27 +71,55 @@ __device__ __forceinline__ int64_t wrap0(int64_t x, int64_t N) {
 
 // Cells (li, lj, lk) of a side^3 region whose first cell is global (i0, j0, k0) (1-based,
 // wrapped): out = Float32(gauss), or with coarse, Float32(gauss) - coarse[block cell] * inv.
+// For an even N the two cells of a pair are neighbours along x in one row (the parity of a
+// cell's global index is that of its wrapped x index), so one thread takes a pair: the
+// region's row is cut into pairs from the parity of its first cell, a lone cell at either
+// end.  An odd N takes one cell per thread.
+__device__ __forceinline__ void region_store(int64_t e, int64_t gi, int64_t gj, int64_t gk,
+                                             double g, const float* __restrict__ coarse, int M,
+                                             int64_t block, float inv, float* __restrict__ out) {
+    const float fine = (float)g;
+    if (!coarse) {
+        out[e] = fine;
+        return;
+    }
+    const int64_t ci = gi / block, cj = gj / block, ck = gk / block;
+    const float mean = coarse[ci + (int64_t)M * (cj + (int64_t)M * ck)] * inv;
+    out[e] = fine - mean;
+}
+
 __global__ __launch_bounds__(256) void region_noise_kernel(uint64_t seed, int64_t N,
                                                            int64_t i0, int64_t j0, int64_t k0,
                                                            int side, const float* __restrict__ coarse,
                                                            int M, int64_t block, float inv,
                                                            float* __restrict__ out) {
-    const int64_t total = (int64_t)side * side * side;
+    const bool paired = (N & 1) == 0;
+    const int64_t gi0 = wrap0(i0, N);                  // global x of the row's first cell
+    const int shift = paired ? (int)(gi0 & 1) : 0;     // 1: the first cell ends a pair
+    const int per_row = paired ? (side + shift + 1) / 2 : side;
+    const int64_t rows = (int64_t)side * side, total = rows * per_row;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const int li = (int)(e % side);
-        const int64_t rem = e / side;
-        const int lj = (int)(rem % side), lk = (int)(rem / side);
-        const int64_t gi = wrap0(i0 + li, N), gj = wrap0(j0 + lj, N), gk = wrap0(k0 + lk, N);
-        const uint64_t idx0 = (uint64_t)(gi + gj * N + gk * N * N);
-        const float fine = (float)gauss_at(seed, idx0);
-        if (!coarse) {
-            out[e] = fine;
+        const int t = (int)(e % per_row);
+        const int64_t row = e / per_row;
+        const int lj = (int)(row % side), lk = (int)(row / side);
+        const int64_t gj = wrap0(j0 + lj, N), gk = wrap0(k0 + lk, N);
+        const int64_t obase = (int64_t)side * row;
+        if (!paired) {
+            const int64_t gi = wrap0(i0 + t, N);
+            region_store(obase + t, gi, gj, gk,
+                         gauss_at(seed, (uint64_t)(gi + gj * N + gk * N * N)), coarse, M,
+                         block, inv, out);
             continue;
         }
-        const int64_t ci = gi / block, cj = gj / block, ck = gk / block;
-        const float mean = coarse[ci + (int64_t)M * (cj + (int64_t)M * ck)] * inv;
-        out[e] = fine - mean;
+        const int la = 2 * t - shift;  // the pair's even-index cell, maybe outside the row
+        const int64_t gi = wrap0(i0 + (la >= 0 ? la : la + 1), N) - (la >= 0 ? 0 : 1);
+        const uint64_t idx0 = (uint64_t)(gi + gj * N + gk * N * N);  // even
+        double g1, g2;
+        gauss_pair(seed, idx0 >> 1, g1, g2);
+        if (la >= 0) region_store(obase + la, gi, gj, gk, g1, coarse, M, block, inv, out);
+        if (la + 1 < side)
+            region_store(obase + la + 1, gi + 1, gj, gk, g2, coarse, M, block, inv, out);
     }
 }
 
@@ -100,7 +137,19 @@ __global__ __launch_bounds__(256) void downsample_kernel(uint64_t seed, int64_t 
         for (int64_t dk = 0; dk < block; ++dk)
             for (int64_t dj = 0; dj < block; ++dj) {
                 const int64_t row = (I * block) + (J * block + dj) * N + (K * block + dk) * N * N;
-                for (int64_t di = 0; di < block; ++di) val += gauss_at(seed, (uint64_t)(row + di));
+                if ((block & 1) == 0 && (N & 1) == 0) {
+                    // an even block in an even grid starts every row on a pair: one pair
+                    // per two cells, added in cell order
+                    for (int64_t di = 0; di < block; di += 2) {
+                        double g1, g2;
+                        gauss_pair(seed, (uint64_t)(row + di) >> 1, g1, g2);
+                        val += g1;
+                        val += g2;
+                    }
+                } else {
+                    for (int64_t di = 0; di < block; ++di)
+                        val += gauss_at(seed, (uint64_t)(row + di));
+                }
             }
         coarse[e] = (float)(val / norm);
     }

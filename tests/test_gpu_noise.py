@@ -33,18 +33,21 @@ def _check(got, want, label):
 
 
 @pytest.mark.parametrize("tile", [(1, 1, 1), (3, 2, 1), (2, 3, 3)])
-def test_tile_noise_matches_oracle(oracle, tile):
-    N, nsub, nmesh, seed = 240, 80, 104, 2024  # nbuff 12: the corner tiles wrap
+@pytest.mark.parametrize("nmesh", [104, 107])
+def test_tile_noise_matches_oracle(oracle, tile, nmesh):
+    """nbuff 12 (rows start on a pair) and 13 (rows start on a pair's second cell, odd
+    region side); the corner tiles wrap."""
+    N, nsub, seed = 240, 80, 2024
     got = api.generate_tile_noise(*tile, nsub, nmesh, N, seed)
     want = oracle.tile_noise(*tile, nsub, nmesh, N, seed)
-    _check(got, want, f"tile {tile}")
+    _check(got, want, f"tile {tile} nmesh {nmesh}")
 
 
 def test_downsample_and_residual_match_oracle(oracle):
     N, M, seed = 256, 32, 77
     coarse = api.downsample_noise(N, M, seed)
     _check(coarse, oracle.downsample_noise(N, M, seed), "coarse 32^3 of 256^3")
-    for tile, nshell in (((1, 1, 1), 0), ((2, 3, 1), 6)):
+    for tile, nshell in (((1, 1, 1), 0), ((2, 3, 1), 6), ((3, 1, 2), 5)):
         got = api.generate_extended_residual(*tile, 96, 128, N, seed, coarse, M, nshell)
         want = oracle.extended_residual(*tile, 96, 128, N, seed, coarse, M, nshell)
         _check(got, want, f"residual {tile} nshell {nshell}")
