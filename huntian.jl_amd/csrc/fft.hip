@@ -346,6 +346,7 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     constexpr int L = 1024, lgL = 10, C = 8, NB = L / 16;
     constexpr int EPT = C * L / NT;  // elements per thread
     __shared__ float2 buf[RM_LDS];
+    __shared__ float2 tw[L];  // the twiddle table, read by every butterfly of the stages
     const int per = (ntiles + 7) >> 3;
     const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (tile >= ntiles) return;
@@ -353,6 +354,12 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     const int b = b0 + tile / gx;
     const int64_t boff = (int64_t)b * strideB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the twiddles (8 KB) into LDS with the first eight wavefronts' first direct loads: the
+    // stages then read them from LDS instead of waiting on L2 at every stage
+    if (wave < L / 128)
+        __builtin_amdgcn_global_load_lds(
+            W + 128 * wave + 2 * lane, (__attribute__((address_space(3))) void*)(tw + 128 * wave),
+            16, 0, 0);
     // rows of the block, two columns per lane; columns past nx (the last tile) read the
     // row's pitch padding, whose values are never stored
     const int lrow = lane >> 2, lcol = 2 * (lane & 3);
@@ -391,9 +398,9 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
 #pragma unroll
         for (int r = 0; r < 8; ++r) buf[rm_slot(c, 8 * j + r)] = v[r];
         __syncthreads();
-        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, W, 3);
+        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, tw, 3);
     } else {
-        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, W);
+        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, tw);
     }
     for (int e = threadIdx.x; e < C * L; e += NT) {
         const int c = e & (C - 1), a = e >> 3, x = x0 + c;
