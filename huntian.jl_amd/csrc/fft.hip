@@ -111,12 +111,17 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 // One Stockham stage of radix R on `ncol` columns of length L held in LDS (column c at
 // buf + c * LS): butterfly j of a column reads x[j + r L/R], twiddles by W^(r k L/(Ns R)),
 // k = j mod Ns, and writes x[(j/Ns) Ns R + k + r Ns].  All reads precede all writes.
-// Row-major LDS layout of the direct-load column passes (8 columns): element i of column c
-// at (i + i / 16) * 8 + c, i.e. rows of 8 complex (64 B) in blocks of 16 rows (1 KB, what one
-// wavefront's 16-byte direct loads fill) with one spare row after each block.
-__device__ __forceinline__ int rm_slot(int c, int i) { return ((i + (i >> 4)) << 3) + c; }
+// Row-major LDS layout of the direct-load column passes (C = 8 or 4 columns): element i of
+// column c at (i + i / (128 / C)) * C + c, i.e. rows of C complex (8 C bytes) in blocks of
+// 128 / C rows (1 KB, what one wavefront's 16-byte direct loads fill) with one spare row
+// after each block.
+template <int C>
+__device__ __forceinline__ int rm_slot(int c, int i) {
+    constexpr int LGC = C == 8 ? 3 : 2, LGB = 7 - LGC;
+    return ((i + (i >> LGB)) << LGC) + c;
+}
 
-template <int R, bool INV, int NT, int DATA, bool RM = false>
+template <int R, bool INV, int NT, int DATA, int RMC = 0>
 __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, int lgNs,
                                       const float2* __restrict__ W) {
     constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
@@ -126,15 +131,20 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
     const int total = nb * ncol;
     const int Ns = 1 << lgNs;
     const int tw_shift = lgL - lgNs - LGR;
-    // column-major: a thread's butterflies run down one column; row-major (8 columns): the
-    // column index is the fastest, so eight lanes read or write one 64-B row
-    auto at = [&](int c, int i) -> float2& { return RM ? buf[rm_slot(c, i)] : buf[c * LS + pad(i)]; };
+    // column-major (RMC = 0): a thread's butterflies run down one column; row-major (RMC
+    // columns): the column index is the fastest, so RMC lanes read or write one row
+    constexpr bool RM = RMC != 0;
+    constexpr int LGRM = RMC == 8 ? 3 : 2;
+    auto at = [&](int c, int i) -> float2& {
+        if constexpr (RM) return buf[rm_slot<RMC>(c, i)];
+        else return buf[c * LS + pad(i)];
+    };
     float2 v[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int g = (int)threadIdx.x + q * NT;
         if (g < total) {
-            const int c = RM ? (g & 7) : g >> lgnb, j = RM ? (g >> 3) : g & (nb - 1);
+            const int c = RM ? (g & (RMC - 1)) : g >> lgnb, j = RM ? (g >> LGRM) : g & (nb - 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = at(c, j + r * nb);
             const int k = j & (Ns - 1);
@@ -168,7 +178,7 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
     for (int q = 0; q < MAXB; ++q) {
         const int g = (int)threadIdx.x + q * NT;
         if (g < total) {
-            const int c = RM ? (g & 7) : g >> lgnb, j = RM ? (g >> 3) : g & (nb - 1);
+            const int c = RM ? (g & (RMC - 1)) : g >> lgnb, j = RM ? (g >> LGRM) : g & (nb - 1);
             const int k = j & (Ns - 1);
             const int d0 = ((j >> lgNs) << (lgNs + LGR)) + k;
 #pragma unroll
@@ -180,20 +190,20 @@ __device__ __forceinline__ void stage(float2* buf, int LS, int ncol, int lgL, in
 
 // FFT of length 2^lgL of every column, in place, natural order in and out (from stage
 // lgNs0 on: the caller has run the stages before it).
-template <bool INV, int NT, int DATA, bool RM = false>
+template <bool INV, int NT, int DATA, int RMC = 0>
 __device__ __forceinline__ void stockham(float2* buf, int LS, int ncol, int lgL,
                                          const float2* __restrict__ W, int lgNs0 = 0) {
     int lgNs = lgNs0;
     while (lgNs < lgL) {
         const int rem = lgL - lgNs;
         if (rem == 1) {
-            stage<2, INV, NT, DATA, RM>(buf, LS, ncol, lgL, lgNs, W);
+            stage<2, INV, NT, DATA, RMC>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 1;
         } else if (rem == 2 || rem == 4) {
-            stage<4, INV, NT, DATA, RM>(buf, LS, ncol, lgL, lgNs, W);
+            stage<4, INV, NT, DATA, RMC>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 2;
         } else {
-            stage<8, INV, NT, DATA, RM>(buf, LS, ncol, lgL, lgNs, W);
+            stage<8, INV, NT, DATA, RMC>(buf, LS, ncol, lgL, lgNs, W);
             lgNs += 3;
         }
     }
@@ -207,11 +217,14 @@ __device__ __forceinline__ double pick(int d, double kx, double ky, double kz) {
 
 // Operator on element (ix, iy, iz) of the half spectrum of an L^3 field, as the reference
 // kernels compute it (_transfer_kernel!, _periodic_kernel!, _short_pot_kernel).
-// Entry of element (ix, iy, iz) in a (L/2 + 1)^3 table over (ix, |iy|, |iz|).
+// Entry of element (ix, iy, iz) in a (L/2 + 1)^3 table over (ix, |iy|, |iz|), |iz| fastest:
+// the z pass, which reads it, walks iz down a column, so a wavefront's entries are runs of
+// contiguous floats (with ix fastest every entry was a 4-byte piece of its own line, 4 MB
+// from the next: 12 of the 34 ms of a 2048-point inverse z pass with the transfer).
 __device__ __forceinline__ int64_t table_index(int L, int ix, int iy, int iz) {
     const int half = L / 2;
     const int ay = iy <= half ? iy : L - iy, az = iz <= half ? iz : L - iz;
-    return ix + (int64_t)(half + 1) * (ay + (int64_t)(half + 1) * az);
+    return az + (int64_t)(half + 1) * (ix + (int64_t)(half + 1) * ay);
 }
 
 // `tab` is op.amp[table_index(...)] for WINDOW and TRANSFER (loaded by the caller together
@@ -331,22 +344,34 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols(
     }
 }
 
-// fft_cols for 1024-point columns, 8 per workgroup, every element read (r0 = 0, r1 = L):
-// the spectrum goes from HBM straight into LDS (16-byte direct loads, no registers held), so
-// a workgroup has all of its 64 KB in flight at once instead of two elements per thread.
-// Each wavefront load fills one 1 KB block of the row-major layout (rm_slot): lane l brings
-// columns 2 (l % 4) and 2 (l % 4) + 1 of row 16 b + l / 4.  The operator's table entries
-// are loaded into registers meanwhile and the operator is applied in LDS, element by element
-// as the register path does, so the values are the same bit for bit.
-constexpr int RM_LDS = 17 * 64 * 8;  // 64 blocks of 16 rows plus a spare row, 8 columns
-template <bool INV, int NT, int WAVES>
+// fft_cols for 1024-point columns, 8 per workgroup, and 2048-point columns, 4 per
+// workgroup, every element read (r0 = 0, r1 = L): the spectrum goes from HBM straight into
+// LDS (16-byte direct loads, no registers held), so a workgroup has all of its 64 KB in
+// flight at once instead of two elements per thread.  Each wavefront load fills one 1 KB
+// block of the row-major layout (rm_slot<C>): lane l brings columns 2 (l % (C/2)) and
+// 2 (l % (C/2)) + 1 of row (128/C) b + l / (C/2).  The operator's table entries are loaded
+// into registers meanwhile and the operator is applied in LDS, element by element as the
+// register path does, so the values are the same bit for bit.
+template <int L, int C>
+struct DirectShape {
+    static constexpr int LGL = L == 2048 ? 11 : 10, LGC = C == 8 ? 3 : 2;
+    static constexpr int ROWS = 128 / C;        // rows per 1 KB block
+    static constexpr int NB = L / ROWS;         // blocks
+    static constexpr int LDS = NB * (ROWS + 1) * C;  // float2, a spare row per block
+    static_assert((L == 1024 && C == 8) || (L == 2048 && C == 4), "direct column shapes");
+};
+// Twiddles the Stockham stages read: index (k << (lgL - lgNs - lgR)) with k < Ns is below
+// L / R of the last stage, i.e. below 256 (1024 = 8 8 4 4) and 512 (2048 = 8 8 8 4).
+constexpr int DIRECT_TW = 512;
+template <bool INV, int NT, int WAVES, int L, int C>
 __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     const float2* in, float2* out, int64_t strideA, int64_t strideB, int nx, int gx,
     int ntiles, int b0, int w0, int w1, FftOp op, const float2* __restrict__ W) {
-    constexpr int L = 1024, lgL = 10, C = 8, NB = L / 16;
+    using S = DirectShape<L, C>;
+    constexpr int lgL = S::LGL, LGC = S::LGC, NB = S::NB, ROWS = S::ROWS;
     constexpr int EPT = C * L / NT;  // elements per thread
-    __shared__ float2 buf[RM_LDS];
-    __shared__ float2 tw[L];  // the twiddle table, read by every butterfly of the stages
+    __shared__ float2 buf[S::LDS];
+    __shared__ float2 tw[DIRECT_TW];  // the twiddles, read by every butterfly of the stages
     const int per = (ntiles + 7) >> 3;
     const int tile = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (tile >= ntiles) return;
@@ -354,27 +379,27 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     const int b = b0 + tile / gx;
     const int64_t boff = (int64_t)b * strideB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // the twiddles (8 KB) into LDS with the first eight wavefronts' first direct loads: the
+    // the twiddles (4 KB) into LDS with the first four wavefronts' first direct loads: the
     // stages then read them from LDS instead of waiting on L2 at every stage
-    if (wave < L / 128)
+    if (wave < DIRECT_TW / 128)
         __builtin_amdgcn_global_load_lds(
             W + 128 * wave + 2 * lane, (__attribute__((address_space(3))) void*)(tw + 128 * wave),
             16, 0, 0);
     // rows of the block, two columns per lane; columns past nx (the last tile) read the
     // row's pitch padding, whose values are never stored
-    const int lrow = lane >> 2, lcol = 2 * (lane & 3);
+    const int lrow = lane >> (LGC - 1), lcol = 2 * (lane & (C / 2 - 1));
     for (int blk = wave; blk < NB; blk += NT / 64) {
-        const int a = 16 * blk + lrow;
+        const int a = ROWS * blk + lrow;
         const float2* g = in + (x0 + lcol) + (int64_t)a * strideA + boff;
         __builtin_amdgcn_global_load_lds(
-            g, (__attribute__((address_space(3))) void*)(buf + 17 * 8 * blk), 16, 0, 0);
+            g, (__attribute__((address_space(3))) void*)(buf + (ROWS + 1) * C * blk), 16, 0, 0);
     }
     float t[EPT];
     if (op.kind == FFT_OP_WINDOW || op.kind == FFT_OP_TRANSFER) {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = (int)threadIdx.x + q * NT;
-            const int c = e & (C - 1), a = e >> 3, x = x0 + c;
+            const int c = e & (C - 1), a = e >> LGC, x = x0 + c;
             t[q] = x < nx ? op.amp[table_index(L, x, b, a)] : 0.0f;
         }
     }
@@ -382,29 +407,29 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     __syncthreads();
     if (op.kind != FFT_OP_NONE) {
         // the first radix-8 stage (span 1, no twiddles) with the operator applied to its
-        // inputs as they are read: thread (c, j) takes elements j + 128 r of column c, the
+        // inputs as they are read: thread (c, j) takes elements j + (L/8) r of column c, the
         // very elements whose table entries it loaded (t[r])
         static_assert(EPT == 8 && NT == 1024, "one radix-8 butterfly per thread");
-        const int c = (int)threadIdx.x & 7, j = (int)threadIdx.x >> 3, x = x0 + c;
+        const int c = (int)threadIdx.x & (C - 1), j = (int)threadIdx.x >> LGC, x = x0 + c;
         float2 v[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int a = j + 128 * r;
-            v[r] = buf[rm_slot(c, a)];
+            const int a = j + (L / 8) * r;
+            v[r] = buf[rm_slot<C>(c, a)];
             if (x < nx) v[r] = apply_op(op, v[r], x, b, a, t[r]);
         }
         dft<8, INV>(v);
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 8; ++r) buf[rm_slot(c, 8 * j + r)] = v[r];
+        for (int r = 0; r < 8; ++r) buf[rm_slot<C>(c, 8 * j + r)] = v[r];
         __syncthreads();
-        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, tw, 3);
+        stockham<INV, NT, C * L, C>(buf, 0, C, lgL, tw, 3);
     } else {
-        stockham<INV, NT, C * L, true>(buf, 0, C, lgL, tw);
+        stockham<INV, NT, C * L, C>(buf, 0, C, lgL, tw);
     }
     for (int e = threadIdx.x; e < C * L; e += NT) {
-        const int c = e & (C - 1), a = e >> 3, x = x0 + c;
-        if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[rm_slot(c, a)];
+        const int c = e & (C - 1), a = e >> LGC, x = x0 + c;
+        if (x < nx && a >= w0 && a < w1) out[x + a * strideA + boff] = buf[rm_slot<C>(c, a)];
     }
 }
 
@@ -450,7 +475,7 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_r2c(
         // (zk - conj zm) / 2i
         const float2 od = make_float2((zk.y + zm.y) * 0.5f, -(zk.x - zm.x) * 0.5f);
         const float2 t = ctw<false>(od, WL[k]);
-        spec[k + (int64_t)fft_pitch(L) * (iy + (int64_t)L * iz)] = cadd(ev, t);
+        spec[k + (int64_t)fft_pitch(L) * iy + fft_plane(L) * iz] = cadd(ev, t);
     }
 }
 
@@ -477,7 +502,7 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r(
             v[u] = make_float2(0.0f, 0.0f);
             if (e < G * nk && row < nrows) {
                 const int iy = s + row % n, iz = s + row / n;
-                v[u] = spec[k + (int64_t)fft_pitch(L) * (iy + (int64_t)L * iz)];
+                v[u] = spec[k + (int64_t)fft_pitch(L) * iy + fft_plane(L) * iz];
             }
         }
 #pragma unroll
@@ -530,28 +555,32 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r(
     }
 }
 
-// fft_rows_c2r for L = 1024 (M = 512, 8 rows per workgroup) with the rows loaded straight
-// into LDS: each row's first 512 coefficients as four 1 KB direct loads into an unpadded
-// staging layout (row g at g P, as in the spectrum), the Nyquist coefficient X[M] by an
-// ordinary load; the untwist reads the staging layout and writes the padded FFT layout over
-// it after a barrier.  Same arithmetic as fft_rows_c2r, so the same values bit for bit.
+// fft_rows_c2r for L = 1024 (M = 512, 8 rows per workgroup) and L = 2048 (M = 1024, 4
+// rows) with the rows loaded straight into LDS: each row's first M coefficients as M / 128
+// 1 KB direct loads into an unpadded staging layout (row g at g P, as in the spectrum), the
+// Nyquist coefficient X[M] by an ordinary load; the untwist reads the staging layout and
+// writes the padded FFT layout over it after a barrier.  Same arithmetic as fft_rows_c2r,
+// so the same values bit for bit.
+template <int LGM>
 __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r_direct(
     const float2* __restrict__ spec, int s, int n, float scale, int mode, float coef,
     float* __restrict__ out, const float2* __restrict__ WM, const float2* __restrict__ WL) {
-    constexpr int lgM = 9, M = 512, G = 8, L = 1024, P = fft_pitch(L);
+    constexpr int lgM = LGM, M = 1 << LGM, G = SMALL_DATA / M, L = 2 * M, P = fft_pitch(L);
+    constexpr int Q = M / 128, LGQ = LGM - 7;  // 1 KB chunks per row
+    static_assert((LGM == 9 || LGM == 10) && G * padded_len(M) <= SMALL_LDS, "row shapes");
     static_assert(G * P <= SMALL_LDS, "staging rows fit the FFT buffer");
     __shared__ float2 buf[SMALL_LDS];
     const int LS = padded_len(M);
     const int row0 = (int)blockIdx.x * G;
     const int nrows = n * n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;  // 8 wavefronts
-    // wavefront w: chunks 4 w' + q of rows ... : 32 chunks (8 rows x 4) over 8 wavefronts
-    for (int u = wave; u < 4 * G; u += SMALL_T / 64) {
-        const int g = u >> 2, q = u & 3;
+    // 32 chunks (G rows x Q) over the 8 wavefronts
+    for (int u = wave; u < Q * G; u += SMALL_T / 64) {
+        const int g = u >> LGQ, q = u & (Q - 1);
         int row = row0 + g;
         if (row >= nrows) row = nrows - 1;  // a row past the end: loads a kept row, unused
         const int iy = s + row % n, iz = s + row / n;
-        const float2* src = spec + (int64_t)P * (iy + (int64_t)L * iz) + 128 * q + 2 * lane;
+        const float2* src = spec + (int64_t)P * iy + fft_plane(L) * iz + 128 * q + 2 * lane;
         __builtin_amdgcn_global_load_lds(
             src, (__attribute__((address_space(3))) void*)(buf + g * P + 128 * q), 16, 0, 0);
     }
@@ -559,7 +588,7 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r_direct(
     if (threadIdx.x < G && row0 + (int)threadIdx.x < nrows) {
         const int row = row0 + threadIdx.x;
         const int iy = s + row % n, iz = s + row / n;
-        nyq = spec[M + (int64_t)P * (iy + (int64_t)L * iz)];
+        nyq = spec[M + (int64_t)P * iy + fft_plane(L) * iz];
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -603,18 +632,20 @@ __global__ __launch_bounds__(SMALL_T, 8) void fft_rows_c2r_direct(
 }
 
 // sqrt P per (ix, |iy|, |iz|) of the transfer: the (L/2 + 1)^3 table the inverse z pass reads
-// (rows of equal k^2 share it; k^2 as _transfer_kernel! computes it).
+// (rows of equal k^2 share it; k^2 as _transfer_kernel! computes it), laid out as
+// table_index: a wavefront per (ix, |iy|) row, lanes over |iz|.
 __global__ void fft_amp(float* __restrict__ amp, int L, double dk, const float* __restrict__ tab,
                         int ntab, float log_k_min, float d_log_k) {
     const int half = L / 2, nh = half + 1;
     const int lane = threadIdx.x & 63;
     for (int row = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); row < nh * nh;
          row += (int)gridDim.x * 4) {
-        const int ay = row % nh, az = row / nh;
-        const double ky = (double)ay * dk, kz = (double)az * dk;
-        const double kyz = ky * ky, kzz = kz * kz;
-        for (int ix = lane; ix < nh; ix += 64) {
-            const double kx = (double)ix * dk;
+        const int ix = row % nh, ay = row / nh;
+        const double kx = (double)ix * dk, ky = (double)ay * dk;
+        const double kyz = ky * ky;
+        for (int az = lane; az < nh; az += 64) {
+            const double kz = (double)az * dk;
+            const double kzz = kz * kz;
             const double k2 = kx * kx + kyz + kzz;
             float v = 0.0f;
             if (k2 != 0.0) {
@@ -628,7 +659,7 @@ __global__ void fft_amp(float* __restrict__ amp, int L, double dk, const float* 
                     v = tab[i0 - 1] + frac * (tab[i0] - tab[i0 - 1]);
                 }
             }
-            amp[ix + (int64_t)nh * row] = v;
+            amp[az + (int64_t)nh * row] = v;
         }
     }
 }
@@ -663,8 +694,14 @@ void launch_cols(hipStream_t st, int L, unsigned gx, int nb, const float2* in, f
                  int w1, const FftOp& op, const float2* W, bool direct = false) {
     const int lgC = cols_lgC(L);
     const int LS = cols_ls(L, 1 << lgC);
-    if (direct && L == 1024 && lgC == 3 && r0 == 0 && r1 == L) {
-        hipLaunchKernelGGL((fft_cols_direct<INV, BIG_T, 8>), dim3(cols_grid(gx, nb)),
+    if (direct && r0 == 0 && r1 == L && L == 1024 && lgC == 3) {
+        hipLaunchKernelGGL((fft_cols_direct<INV, BIG_T, 8, 1024, 8>), dim3(cols_grid(gx, nb)),
+                           dim3(BIG_T), 0, st, in, out, strideA, strideB, nx, (int)gx,
+                           (int)gx * nb, b0, w0, w1, op, W);
+        return;
+    }
+    if (direct && r0 == 0 && r1 == L && L == 2048 && lgC == 2) {
+        hipLaunchKernelGGL((fft_cols_direct<INV, BIG_T, 8, 2048, 4>), dim3(cols_grid(gx, nb)),
                            dim3(BIG_T), 0, st, in, out, strideA, strideB, nx, (int)gx,
                            (int)gx * nb, b0, w0, w1, op, W);
         return;
@@ -726,11 +763,11 @@ hnt_status fft_forward(hnt_ctx* ctx, const float* f, int w, int L, float2* spec)
     const int lgC = cols_lgC(L);
     const unsigned gx = (unsigned)((nk + (1 << lgC) - 1) >> lgC);
     // y: planes iz < w, rows iy < w are filled
-    const int64_t P = fft_pitch(L);
-    launch_cols<false>(st, L, gx, w, spec, spec, P, P * L, nk, 0, 0, w, 0,
+    const int64_t P = fft_pitch(L), PL = fft_plane(L);
+    launch_cols<false>(st, L, gx, w, spec, spec, P, PL, nk, 0, 0, w, 0,
                        L, none, WL, ctx->fft_direct != 0);
     // z: every (ix, iy) column, planes iz < w are filled
-    launch_cols<false>(st, L, gx, L, spec, spec, P * L, P, nk, 0, 0, w, 0,
+    launch_cols<false>(st, L, gx, L, spec, spec, PL, P, nk, 0, 0, w, 0,
                        L, none, WL, ctx->fft_direct != 0);
     return HNT_OK;
 }
@@ -746,16 +783,22 @@ hnt_status fft_inverse(hnt_ctx* ctx, const float2* spec, float2* work, int L, in
     const unsigned gx = (unsigned)((nk + (1 << lgC) - 1) >> lgC);
     FftOp none;
     // z: every (ix, iy) column, the operator applied as read; planes iz in [s, s + n) kept
-    const int64_t P = fft_pitch(L);
-    launch_cols<true>(st, L, gx, L, spec, work, P * L, P, nk, 0, 0, L, s,
+    const int64_t P = fft_pitch(L), PL = fft_plane(L);
+    launch_cols<true>(st, L, gx, L, spec, work, PL, P, nk, 0, 0, L, s,
                       s + n, op, WL, ctx->fft_direct != 0);
     // y: planes iz in [s, s + n), rows iy in [s, s + n) kept
-    launch_cols<true>(st, L, gx, n, work, work, P, P * L, nk, s, 0, L, s,
+    launch_cols<true>(st, L, gx, n, work, work, P, PL, nk, s, 0, L, s,
                       s + n, none, WL, ctx->fft_direct != 0);
     const int lgG = rows_lgG(M);
     const int64_t rows = (int64_t)n * n;
     if (ctx->fft_direct && L == 1024 && lgG == 3) {
-        hipLaunchKernelGGL(fft_rows_c2r_direct, dim3((unsigned)((rows + 7) >> 3)),
+        hipLaunchKernelGGL(fft_rows_c2r_direct<9>, dim3((unsigned)((rows + 7) >> 3)),
+                           dim3(SMALL_T), 0, st, (const float2*)work, s, n, scale, mode, coef,
+                           out, WM, WL);
+        return HNT_OK;
+    }
+    if (ctx->fft_direct && L == 2048 && lgG == 2) {
+        hipLaunchKernelGGL(fft_rows_c2r_direct<10>, dim3((unsigned)((rows + 3) >> 2)),
                            dim3(SMALL_T), 0, st, (const float2*)work, s, n, scale, mode, coef,
                            out, WM, WL);
         return HNT_OK;

@@ -402,7 +402,7 @@ hnt_status plans_for(hnt_ctx* ctx, int n, hipfftHandle& r2c, hipfftHandle& c2r) 
 
 int64_t cube(int n) { return (int64_t)n * n * n; }
 // elements of a half spectrum buffer: the library's padded rows (hipFFT's n/2 + 1 fit in them)
-int64_t half_spectrum(int n) { return (int64_t)fft_pitch(n) * n * n; }
+int64_t half_spectrum(int n) { return fft_plane(n) * n; }
 
 // Zero-pad a device n^3 field to (2n)^3 in fs_real and transform it into fs_base.
 hnt_status padded_rfft(hnt_ctx* ctx, const float* f, int nin, int n2, hipfftHandle r2c) {

@@ -387,18 +387,19 @@ __device__ __forceinline__ float window32(double k2, double Rf, int wsmooth) {
 // fb_window leaves the spectrum as it is: v * 1 = v).  The library's inverse transform reads
 // it as it first reads the spectrum (FFT_OP_WINDOW), so the windowed spectrum is never
 // written: 0.54 GB of table instead of 8.6 GB of spectrum traffic per filter at 1024^3.
-// One workgroup per (|ky|, |kz|) row pair of the table: blockIdx.x + nk blockIdx.y is
-// ay + nk az for rows ay < nk, az < nk (no 64-bit division per element).
+// The table is laid out as the z pass reads it, |kz| fastest: entry az + nk (ix + nk ay).
+// One workgroup per (kx, |ky|) row of the table: blockIdx.x + nk blockIdx.y is
+// ix + nk ay for ix < nk, ay < nk (no 64-bit division per element).
 __global__ __launch_bounds__(256) void fb_wtable(int n, double dk, double Rf, int wsmooth,
                                                  float* __restrict__ w) {
     const int nk = n / 2 + 1;
-    const int ay = (int)blockIdx.x, az = (int)blockIdx.y;
-    const double ky = (double)ay * dk, kz = (double)az * dk;
-    float* row = w + (int64_t)nk * (ay + (int64_t)nk * az);
-    for (int ix = threadIdx.x; ix < nk; ix += 256) {
-        const double kx = (double)ix * dk;
+    const int ix = (int)blockIdx.x, ay = (int)blockIdx.y;
+    const double kx = (double)ix * dk, ky = (double)ay * dk;
+    float* row = w + (int64_t)nk * (ix + (int64_t)nk * ay);
+    for (int az = threadIdx.x; az < nk; az += 256) {
+        const double kz = (double)az * dk;
         const double k2 = kx * kx + ky * ky + kz * kz;
-        row[ix] = k2 != 0.0 ? window32(k2, Rf, wsmooth) : 1.0f;
+        row[az] = k2 != 0.0 ? window32(k2, Rf, wsmooth) : 1.0f;
     }
 }
 
@@ -602,7 +603,7 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
     hipStream_t st = ctx->stream;
     const int64_t ncell = (int64_t)m * m * m;
     // spectrum buffers: the library's padded rows (hipFFT's n/2 + 1 fit in them)
-    const int64_t nreal = (int64_t)n * n * n, ncplx = (int64_t)fft_pitch(n) * n * n;
+    const int64_t nreal = (int64_t)n * n * n, ncplx = fft_plane(n) * n;
     const bool lap = d2Rf != nullptr;
 
     size_t tmp_bytes = 0;

@@ -212,8 +212,43 @@ def test_direct_column_loads_are_bit_identical():
     ctx.release_scratch()
 
 
+def test_direct_2048_passes_are_bit_identical():
+    """The 2048-point inverse column passes (4 columns per workgroup) and the 1024-point x
+    rows of L = 2048 with the spectrum loaded straight into LDS (fft_direct = 1) and through
+    registers (0): the isolated convolution of a 1024^3 field over the 2048^3 padded box
+    (transfer operator, sqrt P table read in the z pass) for delta and a 1LPT component, and
+    its short-range potential (operator computed in the z pass), the same bit for bit."""
+    import torch
+
+    ctx = api.context(0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2048)
+    n = 1024
+    f = api.fortran_empty((n, n, n), torch.float32, "cuda")
+    f.copy_(torch.randn((n, n, n), generator=g, device="cuda"))
+    out = {}
+    try:
+        for direct in (1, 0):
+            ctx.set_tuning(fft_direct=direct)
+            conv = api.isolated_convolve_gpu_multi(f, pk_test, 2.0 * n, n,
+                                                   kernels=[(0, 0, 0), (1, 2, 0)],
+                                                   return_device=True)
+            sp = api.isolated_short_pot_gpu(f, 2.0 * n, 3.0)
+            out[direct] = (conv, sp)
+            print(f"fft_direct={direct}: done", flush=True)
+    finally:
+        ctx.set_tuning(fft_direct=1)
+    (c1, sp1), (c0, sp0) = out[1], out[0]
+    for a, b in zip(c1, c0):
+        assert torch.equal(a, b)
+    assert torch.equal(sp1, sp0)
+    assert float(c1[0].abs().max()) > 0.0
+    del out, f
+    ctx.release_scratch()
+
+
 def test_library_fft_at_2048(hipfft_mode):
-    """L = 2048 (8 columns per workgroup, 1024-point rows), where hipFFT's 32-bit plan is
+    """L = 2048 (4 columns per workgroup, 1024-point rows), where hipFFT's 32-bit plan is
     refused: the isolated short-range potential of a unit impulse is the kernel's Green
     function, symmetric under any permutation of the axes and, being short-ranged (rs = 3
     cells), the same near the origin as at L = 1024 (checked against hipFFT above) up to the
