@@ -345,9 +345,9 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols(
 }
 
 // fft_cols for 1024-point columns, 8 per workgroup, and 2048-point columns, 4 per
-// workgroup, every element read (r0 = 0, r1 = L): the spectrum goes from HBM straight into
-// LDS (16-byte direct loads, no registers held), so a workgroup has all of its 64 KB in
-// flight at once instead of two elements per thread.  Each wavefront load fills one 1 KB
+// workgroup, reading rows [0, r1) with r1 a multiple of 128 / C (the rest are zero): the
+// spectrum goes from HBM straight into LDS (16-byte direct loads, no registers held), so a
+// workgroup has all of its 64 KB in flight at once instead of two elements per thread.  Each wavefront load fills one 1 KB
 // block of the row-major layout (rm_slot<C>): lane l brings columns 2 (l % (C/2)) and
 // 2 (l % (C/2)) + 1 of row (128/C) b + l / (C/2).  The operator's table entries are loaded
 // into registers meanwhile and the operator is applied in LDS, element by element as the
@@ -366,7 +366,7 @@ constexpr int DIRECT_TW = 512;
 template <bool INV, int NT, int WAVES, int L, int C>
 __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
     const float2* in, float2* out, int64_t strideA, int64_t strideB, int nx, int gx,
-    int ntiles, int b0, int w0, int w1, FftOp op, const float2* __restrict__ W) {
+    int ntiles, int b0, int r1, int w0, int w1, FftOp op, const float2* __restrict__ W) {
     using S = DirectShape<L, C>;
     constexpr int lgL = S::LGL, LGC = S::LGC, NB = S::NB, ROWS = S::ROWS;
     constexpr int EPT = C * L / NT;  // elements per thread
@@ -387,12 +387,20 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
             16, 0, 0);
     // rows of the block, two columns per lane; columns past nx (the last tile) read the
     // row's pitch padding, whose values are never stored
+    // (the blocks from r1 on are zero: the padding of a forward transform)
     const int lrow = lane >> (LGC - 1), lcol = 2 * (lane & (C / 2 - 1));
+    const int nread = r1 / ROWS;
     for (int blk = wave; blk < NB; blk += NT / 64) {
-        const int a = ROWS * blk + lrow;
-        const float2* g = in + (x0 + lcol) + (int64_t)a * strideA + boff;
-        __builtin_amdgcn_global_load_lds(
-            g, (__attribute__((address_space(3))) void*)(buf + (ROWS + 1) * C * blk), 16, 0, 0);
+        float2* dst = buf + (ROWS + 1) * C * blk;
+        if (blk < nread) {
+            const int a = ROWS * blk + lrow;
+            const float2* g = in + (x0 + lcol) + (int64_t)a * strideA + boff;
+            __builtin_amdgcn_global_load_lds(
+                g, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+        } else {
+            dst[2 * lane] = make_float2(0.0f, 0.0f);
+            dst[2 * lane + 1] = make_float2(0.0f, 0.0f);
+        }
     }
     float t[EPT];
     if (op.kind == FFT_OP_WINDOW || op.kind == FFT_OP_TRANSFER) {
@@ -416,7 +424,7 @@ __global__ __launch_bounds__(NT, WAVES) void fft_cols_direct(
         for (int r = 0; r < 8; ++r) {
             const int a = j + (L / 8) * r;
             v[r] = buf[rm_slot<C>(c, a)];
-            if (x < nx) v[r] = apply_op(op, v[r], x, b, a, t[r]);
+            if (x < nx && a < r1) v[r] = apply_op(op, v[r], x, b, a, t[r]);
         }
         dft<8, INV>(v);
         __syncthreads();
@@ -694,16 +702,16 @@ void launch_cols(hipStream_t st, int L, unsigned gx, int nb, const float2* in, f
                  int w1, const FftOp& op, const float2* W, bool direct = false) {
     const int lgC = cols_lgC(L);
     const int LS = cols_ls(L, 1 << lgC);
-    if (direct && r0 == 0 && r1 == L && L == 1024 && lgC == 3) {
+    if (direct && r0 == 0 && r1 <= L && r1 % 16 == 0 && L == 1024 && lgC == 3) {
         hipLaunchKernelGGL((fft_cols_direct<INV, BIG_T, 8, 1024, 8>), dim3(cols_grid(gx, nb)),
                            dim3(BIG_T), 0, st, in, out, strideA, strideB, nx, (int)gx,
-                           (int)gx * nb, b0, w0, w1, op, W);
+                           (int)gx * nb, b0, r1, w0, w1, op, W);
         return;
     }
-    if (direct && r0 == 0 && r1 == L && L == 2048 && lgC == 2) {
+    if (direct && r0 == 0 && r1 <= L && r1 % 32 == 0 && L == 2048 && lgC == 2) {
         hipLaunchKernelGGL((fft_cols_direct<INV, BIG_T, 8, 2048, 4>), dim3(cols_grid(gx, nb)),
                            dim3(BIG_T), 0, st, in, out, strideA, strideB, nx, (int)gx,
-                           (int)gx * nb, b0, w0, w1, op, W);
+                           (int)gx * nb, b0, r1, w0, w1, op, W);
         return;
     }
     if (cols_big(L))
