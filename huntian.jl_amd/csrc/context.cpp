@@ -401,6 +401,12 @@ hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value) {
         ctx->fft_direct = value;
         return HNT_OK;
     }
+    if (what == HNT_TUNE_TRICUBIC_TAB) {
+        HNT_REQUIRE(value == 0 || value == 1, "hnt_ctx_set_tuning: tricubic_tab %d (0 or 1)",
+                    value);
+        ctx->tricubic_tab = value;
+        return HNT_OK;
+    }
     if (what == HNT_TUNE_BRICK_LOG2) {
         HNT_REQUIRE(value >= 0 && value <= 13, "hnt_ctx_set_tuning: brick_log2 %d", value);
         ctx->brick_log2 = value;

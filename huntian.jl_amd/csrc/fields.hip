@@ -365,6 +365,80 @@ __global__ __launch_bounds__(FT) void fs_tricubic(const float* __restrict__ coar
     }
 }
 
+// fs_tricubic for a power-of-two block (fine cells per coarse cell) of at most TRI_MAXB.
+// Then a cell's x stencil depends only on its position in its coarse cell, which is that of
+// tile cell li mod block: (g - 1/2) / block + 1/2 is exact in double, so cells li and
+// li mod block have the same weights bit for bit.  Each row forms the 64 products
+// ((wx wy) wz) of each of the block positions once, into the wavefront's LDS table, and a
+// cell takes 64 multiplies and adds instead of 208 operations: the same products summed in
+// the same order, so the same values bit for bit.
+constexpr int TRI_MAXB = 32;
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(FT) void fs_tricubic_tab(const float* __restrict__ coarse, int M,
+                                                      int nmesh, int block,
+                                                      const float4* __restrict__ sw,
+                                                      const int4* __restrict__ si,
+                                                      float* __restrict__ out) {
+    __shared__ float4 wtab[WAVES][TRI_MAXB * 16];  // [position][4 z + y], x in the float4
+    const int lane = threadIdx.x & 63;
+    float4* W = wtab[threadIdx.x >> 6];
+    const int run = (nmesh + 63) >> 6;
+    const int l0 = min(lane * run, nmesh), l1 = min(l0 + run, nmesh);
+    FOR_ROWS(nmesh * nmesh) {
+        const int lj = row % nmesh, lk = row / nmesh;
+        const float4 wy4 = sw[nmesh + lj], wz4 = sw[2 * nmesh + lk];
+        const int4 ys4 = si[nmesh + lj], zs4 = si[2 * nmesh + lk];
+        const float wy[4] = {wy4.x, wy4.y, wy4.z, wy4.w}, wz[4] = {wz4.x, wz4.y, wz4.z, wz4.w};
+        const int ys[4] = {ys4.x, ys4.y, ys4.z, ys4.w}, zs[4] = {zs4.x, zs4.y, zs4.z, zs4.w};
+        wave_lds_fence();  // the previous row's table reads are done
+        for (int e = lane; e < block * 16; e += 64) {
+            const int q = e >> 4, c = (e >> 2) & 3, b = e & 3;
+            const float4 wx = sw[q];
+            const float y = wy[b], z = wz[c];
+            W[e] = make_float4(wx.x * y * z, wx.y * y * z, wx.z * y * z, wx.w * y * z);
+        }
+        wave_lds_fence();
+        float* o = out + (int64_t)row * nmesh;
+        float cv[4][4][4];
+        int4 have = make_int4(-1, -1, -1, -1);
+        for (int li = l0; li < l1; ++li) {
+            const int4 xs4 = si[li];
+            if (xs4.x != have.x || xs4.y != have.y || xs4.z != have.z || xs4.w != have.w) {
+                const int xs[4] = {xs4.x, xs4.y, xs4.z, xs4.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float* cz = coarse + (int64_t)M * M * zs[c];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const float* cy = cz + (int64_t)M * ys[b];
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) cv[c][b][a] = cy[xs[a]];
+                    }
+                }
+                have = xs4;
+            }
+            const float4* Wq = W + 16 * (li & (block - 1));
+            float val = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const float4 w = Wq[4 * c + b];
+                    val += w.x * cv[c][b][0];
+                    val += w.y * cv[c][b][1];
+                    val += w.z * cv[c][b][2];
+                    val += w.w * cv[c][b][3];
+                }
+            o[li] = val;
+        }
+    }
+}
+
 // hipFFT R2C / C2R plans of an n^3 real field (cached per size on the context).  The 32-bit
 // plan call gives wrong transforms from 2^31 elements on (2048^3: an impulse response loses
 // its axis symmetry at 1e-3, a smooth field is off by half its norm), so larger sizes are
@@ -706,7 +780,13 @@ extern "C" hnt_status hnt_interpolate_to_tile(hnt_ctx* ctx, const float* coarse,
     int4* si = (int4*)(sw + 3 * (size_t)nmesh);
     hipLaunchKernelGGL(fs_tricubic_axes, dim3((unsigned)((3 * nmesh + 255) / 256)), dim3(256), 0,
                        ctx->stream, M, N, block, (int)i0, (int)j0, (int)k0, nmesh, sw, si);
-    hipLaunchKernelGGL(fs_tricubic, dim3(grid_rows((int64_t)nmesh * nmesh)), dim3(FT), 0,
-                       ctx->stream, coarse, M, nmesh, (const float4*)sw, (const int4*)si, tile);
+    if ((block & (block - 1)) == 0 && block <= TRI_MAXB && nmesh >= block && ctx->tricubic_tab)
+        hipLaunchKernelGGL(fs_tricubic_tab, dim3(grid_rows((int64_t)nmesh * nmesh)), dim3(FT), 0,
+                           ctx->stream, coarse, M, nmesh, block, (const float4*)sw,
+                           (const int4*)si, tile);
+    else
+        hipLaunchKernelGGL(fs_tricubic, dim3(grid_rows((int64_t)nmesh * nmesh)), dim3(FT), 0,
+                           ctx->stream, coarse, M, nmesh, (const float4*)sw, (const int4*)si,
+                           tile);
     return finish_call(ctx, "hnt_interpolate_to_tile");
 }

@@ -130,7 +130,8 @@ struct hnt_ctx {
     struct FftTw { int L; hnt::DevBuf buf; };
     std::vector<FftTw> fft_tw;
     int fft_mode = 0;
-    int fft_direct = 1;  // 1024-point inverse column passes load straight into LDS
+    int fft_direct = 1;    // 1024- and 2048-point passes load straight into LDS
+    int tricubic_tab = 1;  // interpolation weight products tabulated per row
     void* fft_r2c = nullptr;  // hipfftHandle of an n^3 real-to-complex plan
     void* fft_c2r = nullptr;
     int fft_n = 0;

@@ -166,7 +166,8 @@ HNT_ORDER_CROSSING, HNT_ORDER_PLAIN, HNT_ORDER_LEARNED, HNT_ORDER_GIVEN = 0, 1, 
 ORDERS = {"crossing": HNT_ORDER_CROSSING, "plain": HNT_ORDER_PLAIN,
           "learned": HNT_ORDER_LEARNED, "given": HNT_ORDER_GIVEN}
 PRECISIONS = {"fp64": 0, "fp32_kahan": 1}
-TUNING = {"brick_log2": 0, "waves_per_cu": 1, "gang": 2, "fft": 3, "fft_direct": 4}
+TUNING = {"brick_log2": 0, "waves_per_cu": 1, "gang": 2, "fft": 3, "fft_direct": 4,
+          "tricubic_tab": 5}
 
 
 class Mlp(C.Structure):

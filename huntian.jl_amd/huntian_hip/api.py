@@ -61,8 +61,9 @@ class Context:
         """Launch-shape knobs (brick_log2, waves_per_cu, gang = lanes per peak: 1, 16 or
         64): speed only, never results; fft = 0 (the library's FFTs) or 1 (hipFFT) for the
         field-synthesis entries, which differ by the transforms' Float32 rounding;
-        fft_direct = 1 (default) or 0: the 1024-point inverse column passes load into LDS
-        directly or through registers (speed only, same bits)."""
+        fft_direct = 1 (default) or 0: the 1024- and 2048-point passes load into LDS directly
+        or through registers; tricubic_tab = 1 (default) or 0: the interpolation's weight
+        products tabulated per row or formed per cell (both speed only, same bits)."""
         for k, v in knobs.items():
             abi.check(self.lib, self.lib.hnt_ctx_set_tuning(self.handle, abi.TUNING[k], int(v)))
 

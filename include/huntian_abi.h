@@ -98,13 +98,17 @@ hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t
  *  FFT:          transforms of the field-synthesis entries.  0 (default): the library's own
  *                FFTs for power-of-two sizes 8..2048 (hipFFT otherwise); 1: hipFFT always.
  *                Both are Float32 transforms; results differ by their rounding.
- *  FFT_DIRECT:   1 (default): the library's 1024-point passes that read every element
- *                (the inverse column and row passes, the forward column passes of an
- *                unpadded field) load straight into LDS (no registers held, a workgroup's
- *                64 KB in flight at once); 0: through registers, two elements per thread at
- *                a time.  Speed only: the results are the same bit for bit. */
+ *  FFT_DIRECT:   1 (default): the library's 1024- and 2048-point column passes (reading
+ *                whole 1 KB blocks of rows, the rest zero) and its 1024- and 2048-point
+ *                complex-to-real row passes load straight into LDS (no registers held, a
+ *                workgroup's 64 KB in flight at once); 0: through registers, two elements
+ *                per thread at a time.  Speed only: the results are the same bit for bit.
+ *  TRICUBIC_TAB: 1 (default): hnt_interpolate_to_tile with a power-of-two block of at most
+ *                32 fine cells per coarse cell forms each row's weight products once per
+ *                position in the coarse cell (a table in LDS); 0: per cell.  Speed only:
+ *                the same products in the same order, the same results bit for bit. */
 enum { HNT_TUNE_BRICK_LOG2 = 0, HNT_TUNE_WAVES_PER_CU = 1, HNT_TUNE_GANG = 2, HNT_TUNE_FFT = 3,
-       HNT_TUNE_FFT_DIRECT = 4 };
+       HNT_TUNE_FFT_DIRECT = 4, HNT_TUNE_TRICUBIC_TAB = 5 };
 hnt_status hnt_ctx_set_tuning(hnt_ctx* ctx, int32_t what, int32_t value);
 
 /* ---------------------------------------------------------------- shells and tables */

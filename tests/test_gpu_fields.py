@@ -101,8 +101,12 @@ def test_isolated_poisson_and_short_potential():
     assert rel_l2(pot, F.isolated_short_pot(f, box, 3.0)) <= 1e-5
 
 
+# blocks N / M of 16, 32, 2 and 1 (the per-row weight table) and 6, 3 (weights per cell)
 @pytest.mark.parametrize("tile,nsub,nmesh,N,M", [((1, 1, 1), 48, 64, 192, 12),
                                                  ((3, 2, 4), 48, 64, 192, 12),
+                                                 ((2, 3, 1), 48, 64, 256, 8),
+                                                 ((1, 2, 2), 20, 30, 64, 32),
+                                                 ((3, 1, 2), 20, 30, 64, 64),
                                                  ((2, 5, 1), 30, 40, 150, 25),
                                                  ((4, 4, 4), 48, 64, 192, 64)])
 def test_interpolate_to_tile_bit_for_bit(tile, nsub, nmesh, N, M):
@@ -110,6 +114,29 @@ def test_interpolate_to_tile_bit_for_bit(tile, nsub, nmesh, N, M):
     coarse = rng.standard_normal((M, M, M)).astype(np.float32)
     got = api.interpolate_to_tile_gpu(coarse, *tile, nsub, nmesh, N, M)
     np.testing.assert_array_equal(got, F.interpolate_to_tile(coarse, *tile, nsub, nmesh, N, M))
+
+
+def test_interpolate_weight_table_bit_identical_at_full_size():
+    """A 1024^3 tile (nbuff 64) of a 4096^3 grid from its 256^3 coarse grid, block 16: the
+    per-row weight table (tricubic_tab = 1, the default) and the per-cell products (0) give
+    the same values bit for bit; the per-cell kernel is the one pinned to the oracle above
+    at blocks 6 and 3."""
+    import torch
+
+    ctx = api.context(0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(256)
+    coarse = torch.randn((256, 256, 256), generator=g, device="cuda")
+    out = {}
+    try:
+        for tab in (1, 0):
+            ctx.set_tuning(tricubic_tab=tab)
+            out[tab] = api.interpolate_to_tile_gpu(coarse, 2, 3, 1, 896, 1024, 4096, 256,
+                                                   return_device=True)
+    finally:
+        ctx.set_tuning(tricubic_tab=1)
+    assert torch.equal(out[1], out[0])
+    assert float(out[1].abs().max()) > 0.0
 
 
 def test_bad_arguments_are_rejected():
