@@ -382,8 +382,8 @@ __device__ __forceinline__ float window32(double k2, double Rf, int wsmooth) {
     return (float)w;
 }
 
-// The window of one filter as a table over (kx, |ky|, |kz|): w[ix + nk (ay + nk az)],
-// nk = n/2 + 1, with k from those magnitudes as fb_window computes it, and 1 at k = 0 (where
+// The window of one filter as a table over (kx, |ky|, |kz|), nk = n/2 + 1 entries a side,
+// with k from those magnitudes as fb_window computes it, and 1 at k = 0 (where
 // fb_window leaves the spectrum as it is: v * 1 = v).  The library's inverse transform reads
 // it as it first reads the spectrum (FFT_OP_WINDOW), so the windowed spectrum is never
 // written: 0.54 GB of table instead of 8.6 GB of spectrum traffic per filter at 1024^3.
