@@ -81,6 +81,41 @@ def test_residual_at_size_has_zero_block_means():
                                cb.cpu().numpy(), rtol=1e-5, atol=1e-5)
 
 
+def test_full_size_grid_against_oracle_samples(oracle):
+    """The benchmark's grid: N = 4096 (global indices past 2^32), coarse M = 256 (blocks of
+    16^3), seed 7.  Sampled coarse cells (corners included) against the FP64 sum of their
+    4096 oracle Gaussians in the reference's loop order (dk, dj, di); the residual of the
+    1024^3 tile (2, 2, 2), nbuff 64, at its first and last 16^3 cells, and of tile (1, 1, 1),
+    whose first cells wrap to the grid's far end, against the oracle's residual of the same
+    16^3 regions (nsub = nmesh = 16: region it starts at global cell 16 (it - 1) + 1)."""
+    N, M, seed = 4096, 256, 7
+    b = N // M
+    coarse = api.downsample_noise(N, M, seed, return_device=True)
+    rng = np.random.default_rng(5)
+    cells = [(0, 0, 0), (M - 1, M - 1, M - 1), (M - 1, 0, M - 1)]
+    cells += [tuple(int(v) for v in rng.integers(0, M, 3)) for _ in range(5)]
+    got, want = [], []
+    for I, J, K in cells:
+        val = 0.0
+        for dk in range(b):
+            for dj in range(b):
+                base = I * b + (J * b + dj) * N + (K * b + dk) * N * N
+                for di in range(b):
+                    val += oracle.threefry_gauss(seed, base + di)
+        want.append(np.float32(val / math.sqrt(b ** 3)))
+        got.append(float(coarse[I, J, K]))
+    _check(np.array(got, np.float32), np.array(want, np.float32), "coarse cells of 4096^3")
+    ch = coarse.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)  # [I, J, K]
+    nsub, nmesh = 896, 1024
+    for tile, corners in (((2, 2, 2), ((0, 53), (1008, 116))), ((1, 1, 1), ((0, 253),))):
+        res = api.generate_extended_residual(*tile, nsub, nmesh, N, seed, coarse, M, 0,
+                                             return_device=True)
+        for lo, it in corners:
+            part = res[lo:lo + 16, lo:lo + 16, lo:lo + 16].cpu().numpy()
+            ref = oracle.extended_residual(it, it, it, 16, 16, N, seed, ch, M, 0)
+            _check(part, ref, f"residual of tile {tile} at {lo}")
+
+
 def test_noise_rejects_bad_sizes():
     from huntian_hip.abi import HuntianError
 
