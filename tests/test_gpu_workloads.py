@@ -119,7 +119,7 @@ def _assert_reduced_precision(worst, flips, nhalo):
         assert worst[k + "_over_bar"] <= lim, (k, worst[k + "_over_bar"], worst[k])
 
 
-def test_tile128_every_candidate_variants_A_and_C(oracle, websky_ct):
+def test_tile128_every_candidate_variants_A_B_and_C(oracle, websky_ct):
     dev = torch.device("cuda", 0)
     w = synth.workload_tile("tile128", dev)
     n, nbuff, nhunt, alatt = w["n"], w["nbuff"], w["nhunt"], w["alatt"]
@@ -137,7 +137,7 @@ def test_tile128_every_candidate_variants_A_and_C(oracle, websky_ct):
     t = oracle.shell_tables(nhunt)
     stab = api.build_shell_tables(nhunt)
     kw = dict(alatt=alatt, ZZon=1.0, fcrit=fcrit, nbuff=nbuff)
-    for variant, early in (("A", False), ("C", True)):
+    for variant, early in (("A", False), ("B", True), ("C", True)):
         mask.zero_()
         r = api.shell_analyse(w["fields"], pf["i"], pf["j"], pf["k"], Rf, ir2, stab, websky_ct,
                               variant=variant, early_exit=early, mask=mask, **kw)
