@@ -1593,12 +1593,91 @@ __global__ __launch_bounds__(64) void shell_wave_kernel(SAArgs a) {
 // peaks ordered by that stop shell and a wavefront's 64 lanes finish together.
 // Workgroup b takes group (b % 8) * G8 + b / 8: each XCD walks its own contiguous eighth of
 // the (ir2min, Morton) order.
+// Sum of delta over shell cells [st, st + nc) of the peak at (ci, cj, ck), whose cell is c0,
+// read from the caller's field; cells outside the tile are skipped (CHECK) and nin counts
+// the cells summed.  The reference's cell order, FP64 sum of the Float32 values.
+template <bool CHECK>
+__device__ __forceinline__ void cross_shell_hbm(const SAArgs& a, const TileDev& T, int st, int nc,
+                                                int ci, int cj, int ck, int64_t c0, double& F,
+                                                int64_t& nin) {
+    const float* __restrict__ delta = T.delta;
+    F = 0.0;
+    nin = 0;
+    if (!CHECK) {
+        int c = 0;
+        for (; c + 4 <= nc; c += 4) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int4 o = a.cells[st + c + u];
+                v[u] = delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) F += (double)v[u];
+        }
+        for (; c < nc; ++c) {
+            const int4 o = a.cells[st + c];
+            F += (double)delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
+        }
+        nin = nc;
+    } else {
+        for (int c = 0; c < nc; ++c) {
+            const int4 o = a.cells[st + c];
+            const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
+            if (i1 < 1 || i1 > T.n || i2 < 1 || i2 > T.n || i3 < 1 || i3 > T.n) continue;
+            F += (double)delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
+            nin += 1;
+        }
+    }
+}
+
+// One step of the Fbar recurrence (RadialShell.jl:410-416) and the crossing test: shell s with
+// delta sum F over nin cells; sets scross and clears active at the first shell past ir2min
+// whose Fbar is below fcrit.
+__device__ __forceinline__ void cross_step(const SAArgs& a, int s, double F, int64_t nin,
+                                           int ir2min, double fcrit, double& Fprev,
+                                           double& dFbarp, bool& past, bool& active,
+                                           int& scross) {
+    const double dFbar = nin > 0 ? F / (double)nin : 0.0;
+    const double rad3p = cube(a.rad[s - 1]), rad3 = cube(a.rad[s]);
+    const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
+    Fprev = Fb;
+    dFbarp = dFbar;
+    if (!past && a.shell_r2[s - 1] > ir2min) past = true;
+    if (past && Fb < fcrit) {
+        scross = s;
+        active = false;
+    }
+}
+
+// The gather extent of a peak whose crossing shell is scross (0: none within the hunt
+// radius): the main gather stops at the first s >= max(s_c, mupp_rf) with
+// rad[s+1] >= rad[s_c+1] + 2, else at the last shell.
+__device__ int crossing_stop(const SAArgs& a, double Rf, int scross) {
+    const int m = a.NS - 1;
+    int stop = m;
+    if (scross > 0) {
+        int mrf, mlow_rf, mupp_rf;
+        rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
+        const int s0 = scross > mupp_rf ? scross : mupp_rf;
+        if (s0 <= m) {
+            // first s in [s0, m] with rad[s + 1] >= rstop, i.e. first shell index t = s + 1
+            // in [s0 + 1, m + 1] with rad[t] >= rstop
+            const int t = first_ge(a.rad, s0 + 1, m + 1, a.rad[scross + 1] + 2.0);
+            if (t <= m + 1) stop = t - 1;
+        }
+    }
+    return stop;
+}
+
+// The crossing pass, one lane per peak of group g of the first order: delta only, the same
+// arithmetic as the gather, so each peak's exact gather extent.  Workgroup b takes group
+// (b % 8) * G8 + b / 8, G8 an eighth of the groups: each XCD walks its own contiguous eighth.
 __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, int64_t G8,
                                                       int32_t* __restrict__ stop_out) {
     const int64_t g = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
     if (g >= a.ngroups) return;
     const TileDev& T = a.tiles[a.gtile[g]];
-    const float* __restrict__ delta = T.delta;
     const int p = a.slot[g * 64 + threadIdx.x];
     const bool valid = p >= 0;
     int ci = 1, cj = 1, ck = 1, ir2min = 0;
@@ -1611,7 +1690,6 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, int64_t G8,
         Rf = T.Rf[p];
         fcrit = T.fcrit ? T.fcrit[p] : T.fcrit_s;
     }
-    const double* __restrict__ rad = a.rad;
     const int m = a.NS - 1;
     const int64_t c0 = (int64_t)(ci - 1) + (cj - 1) * T.s1 + (ck - 1) * T.s2;
     const int R = a.rmax;
@@ -1619,7 +1697,7 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, int64_t G8,
                         ck - R >= 1 && ck + R <= T.n;
     const bool wave_inside = __ballot(!valid || inside) == ~0ull;
 
-    double Fprev = valid ? (double)delta[c0] : 0.0;
+    double Fprev = valid ? (double)T.delta[c0] : 0.0;
     double dFbarp = Fprev;
     bool active = valid, past = false;
     int scross = 0;
@@ -1627,62 +1705,15 @@ __global__ __launch_bounds__(64) void crossing_kernel(SAArgs a, int64_t G8,
         if (__ballot(active) == 0ull) break;
         const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
         if (active) {
-            double F = 0.0;
-            int64_t nin = 0;
-            if (wave_inside) {
-                int c = 0;
-                for (; c + 4 <= nc; c += 4) {
-                    float v[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int4 o = a.cells[st + c + u];
-                        v[u] = delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) F += (double)v[u];
-                }
-                for (; c < nc; ++c) {
-                    const int4 o = a.cells[st + c];
-                    F += (double)delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
-                }
-                nin = nc;
-            } else {
-                for (int c = 0; c < nc; ++c) {
-                    const int4 o = a.cells[st + c];
-                    const int i1 = ci + o.x, i2 = cj + o.y, i3 = ck + o.z;
-                    if (i1 < 1 || i1 > T.n || i2 < 1 || i2 > T.n || i3 < 1 || i3 > T.n)
-                        continue;
-                    F += (double)delta[c0 + o.x + o.y * T.s1 + o.z * T.s2];
-                    nin += 1;
-                }
-            }
-            const double dFbar = nin > 0 ? F / (double)nin : 0.0;
-            const double rad3p = cube(rad[s - 1]), rad3 = cube(rad[s]);
-            const double Fb = (rad3p * Fprev + 0.5 * (dFbarp + dFbar) * (rad3 - rad3p)) / rad3;
-            Fprev = Fb;
-            dFbarp = dFbar;
-            if (!past && a.shell_r2[s - 1] > ir2min) past = true;
-            if (past && Fb < fcrit) {
-                scross = s;
-                active = false;
-            }
+            double F;
+            int64_t nin;
+            if (wave_inside) cross_shell_hbm<false>(a, T, st, nc, ci, cj, ck, c0, F, nin);
+            else cross_shell_hbm<true>(a, T, st, nc, ci, cj, ck, c0, F, nin);
+            cross_step(a, s, F, nin, ir2min, fcrit, Fprev, dFbarp, past, active, scross);
         }
     }
     if (!valid) return;
-    // the main gather stops at the first s >= max(s_c, mupp_rf) with rad[s+1] >= rad[s_c+1]+2
-    int stop = m;
-    if (scross > 0) {
-        int mrf, mlow_rf, mupp_rf;
-        rf_window(a, Rf, mrf, mlow_rf, mupp_rf);
-        const int s0 = scross > mupp_rf ? scross : mupp_rf;
-        if (s0 <= m) {
-            // first s in [s0, m] with rad[s + 1] >= rstop, i.e. first shell index t = s + 1
-            // in [s0 + 1, m + 1] with rad[t] >= rstop
-            const int t = first_ge(rad, s0 + 1, m + 1, rad[scross + 1] + 2.0);
-            if (t <= m + 1) stop = t - 1;
-        }
-    }
-    stop_out[T.base + p] = stop;
+    stop_out[T.base + p] = crossing_stop(a, Rf, scross);
 }
 
 // Bracket-network features of every peak in schedule order (same grouping and XCD mapping
@@ -2127,7 +2158,14 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
             A.pred = pred;  // the main kernel checks each peak against its bracket
         }
         for (int t = 0; t < ntiles; ++t) prim[t] = stops + meta[t];
-        if (params->order == HNT_ORDER_CROSSING) A.stop_hint = stops;
+        if (params->order == HNT_ORDER_CROSSING) {
+            A.stop_hint = stops;
+            for (int t = 0; t < ntiles; ++t)
+                if (results[t].schedule_extent && batches[t].npeaks > 0)
+                    HNT_HIP(hipMemcpyAsync(results[t].schedule_extent, stops + meta[t],
+                                           (size_t)batches[t].npeaks * 4,
+                                           hipMemcpyDeviceToDevice, st));
+        }
         HNT_TRY(order_by(prim.data(), ctx->brick_log2));
         stage_end(ctx, HNT_STAGE_SA_CROSS);
     }

@@ -197,6 +197,8 @@ class Params(C.Structure):
 RESULT_SCALARS = ("RTHL", "Fbarx", "e_v", "p_v", "Srb", "d2F", "zvir_half")
 RESULT_VEC3 = ("eigs", "Sbar", "Sbar2", "gradpk", "gradpkf", "gradpkrf")
 RESULT_INT = ("nshells_used", "crossing_shell", "bracket")
+# every Int32 field of hnt_results (schedule_extent is allocated only on request)
+RESULT_I32 = RESULT_INT + ("schedule_extent",)
 
 
 class Results(C.Structure):
@@ -218,6 +220,7 @@ class Results(C.Structure):
         ("nshells_used", _i32p),
         ("crossing_shell", _i32p),
         ("bracket", _i32p),
+        ("schedule_extent", _i32p),
     ]
 
 

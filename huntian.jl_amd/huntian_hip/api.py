@@ -360,7 +360,9 @@ def _fields_struct(fields: dict):
     return abi.Fields(n1, n2, n3, *ptrs), keep
 
 
-def alloc_results(npk: int, device="cuda"):
+def alloc_results(npk: int, device="cuda", schedule_extent=False):
+    """Result arrays of npk peaks; schedule_extent adds that optional output (the crossing
+    pass's extents, HNT_ORDER_CROSSING), which stays unset otherwise."""
     torch = _torch()
     r = {k: torch.empty(max(npk, 1), dtype=torch.float64, device=device)
          for k in abi.RESULT_SCALARS}
@@ -369,6 +371,8 @@ def alloc_results(npk: int, device="cuda"):
     r["strain"] = torch.empty((max(npk, 1), 3, 3), dtype=torch.float64, device=device)
     for k in abi.RESULT_INT:
         r[k] = torch.zeros(max(npk, 1), dtype=torch.int32, device=device)
+    if schedule_extent:
+        r["schedule_extent"] = torch.zeros(max(npk, 1), dtype=torch.int32, device=device)
     return r
 
 
@@ -376,7 +380,7 @@ def _results_struct(r):
     vals = []
     for name, _ in abi.Results._fields_:
         t = r.get(name)
-        ct = C.c_int32 if name in abi.RESULT_INT else C.c_double
+        ct = C.c_int32 if name in abi.RESULT_I32 else C.c_double
         vals.append(abi.as_ptr(_ptr(t), ct))
     return abi.Results(*vals)
 
@@ -453,7 +457,7 @@ def shell_analyse(fields: dict, pi, pj, pk, Rf, ir2min, tables: ShellTables, ct:
     return r
 
 
-def _tiles_structs(tiles, ZZon, fcrit, dev):
+def _tiles_structs(tiles, ZZon, fcrit, dev, schedule_extent=False):
     nt = len(tiles)
     FA = (abi.Fields * nt)()
     BA = (abi.Batch * nt)()
@@ -464,7 +468,7 @@ def _tiles_structs(tiles, ZZon, fcrit, dev):
         fs, kf = _fields_struct(t["fields"])
         b, kb = _batch_struct(t["pi"], t["pj"], t["pk"], t["Rf"], t["ir2min"], ZZon, fcrit,
                               t.get("ZZon_pp"), t.get("fcrit_pp"), dev)
-        r = alloc_results(int(len(t["pi"])), dev)
+        r = alloc_results(int(len(t["pi"])), dev, schedule_extent)
         FA[q], BA[q], RA[q] = fs, b, _results_struct(r)
         MA[q] = abi.as_ptr(_ptr(t.get("mask")), C.c_int8)
         keep += [kf, kb]
@@ -475,15 +479,16 @@ def _tiles_structs(tiles, ZZon, fcrit, dev):
 def shell_analyse_tiles(tiles, tables: ShellTables, ct: dict, *, alatt: float, ZZon: float,
                         fcrit: float, nbuff: int = 0, variant="C", early_exit: bool = True,
                         order: str = "crossing", bracket=None, precision: str = "fp64",
-                        rmax2rs: float = 0.0):
+                        rmax2rs: float = 0.0, schedule_extent: bool = False):
     """hnt_shell_analyse_tiles: every tile in one persistent launch.
 
     tiles: list of dicts with keys fields, pi, pj, pk, Rf, ir2min and optionally mask,
-    ZZon_pp, fcrit_pp.  Returns one result dict (as shell_analyse) per tile."""
+    ZZon_pp, fcrit_pp.  Returns one result dict (as shell_analyse) per tile, with the
+    crossing pass's extents as "schedule_extent" if asked."""
     torch = _require_gpu()
     ctx = context()
     dev = torch.device("cuda", ctx.device)
-    FA, BA, RA, MA, keep, out = _tiles_structs(tiles, ZZon, fcrit, dev)
+    FA, BA, RA, MA, keep, out = _tiles_structs(tiles, ZZon, fcrit, dev, schedule_extent)
     prm = _params(alatt, nbuff, variant, early_exit, order, bracket, precision,
                   rmax2rs=rmax2rs)
     cs, keep_t = _ctab_struct(ct)

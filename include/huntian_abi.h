@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define HNT_ABI_VERSION 2
+#define HNT_ABI_VERSION 3
 
 typedef enum {
     HNT_OK = 0,
@@ -309,6 +309,11 @@ typedef struct {
                                   bit 1 the peak collapsed and the shell holding RTHL is
                                   >= lo - 2 (both set: the bracket verified), bit 2 the
                                   collapse logit predicted the collapse flag; other orders: 0 */
+    int32_t* schedule_extent;  /* optional (NULL), HNT_ORDER_CROSSING: each peak's gather extent
+                                  as the crossing pass finds it, the schedule's sort key (equal
+                                  to nshells_used with FP64 sums); written by the call that
+                                  orders the peaks (a plan: when it is built); other orders:
+                                  left as it is */
 } hnt_results;
 
 /*

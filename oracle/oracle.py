@@ -282,9 +282,9 @@ def results_struct(r: dict) -> abi.Results:
     for name, _ in abi.Results._fields_:
         a = r.get(name)
         if a is None:
-            vals.append(C.cast(None, _i32p if name in abi.RESULT_INT else _f64p))
+            vals.append(C.cast(None, _i32p if name in abi.RESULT_I32 else _f64p))
         else:
-            vals.append(_p(a, C.c_int32 if name in abi.RESULT_INT else C.c_double))
+            vals.append(_p(a, C.c_int32 if name in abi.RESULT_I32 else C.c_double))
     return abi.Results(*vals)
 
 

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(abi.EXPORTED)
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.hnt_abi_version() == 2
+    assert lib.hnt_abi_version() == 3
 
 
 def test_struct_layouts_match_the_header(tmp_path):
