@@ -65,7 +65,7 @@ void release_work(SAWork& w) {
     }
     DevBuf* bufs[] = {&w.recs, &w.keys_a, &w.keys_b, &w.vals_a, &w.vals_b, &w.cub_tmp, &w.slots,
                       &w.gtiles, &w.tiles, &w.tilemeta, &w.stops, &w.feats, &w.prof, &w.queue,
-                      &w.flag, &w.skip, &w.tables.cells, &w.tables.start, &w.tables.count,
+                      &w.flag, &w.skip, &w.paint, &w.tables.cells, &w.tables.start, &w.tables.count,
                       &w.tables.r2, &w.tables.rad, &w.tables.kw, &w.tables.kw_idx,
                       &w.ctab.table, &w.akk};
     for (DevBuf* b : bufs) release(*b);

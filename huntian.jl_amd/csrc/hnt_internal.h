@@ -92,7 +92,7 @@ struct Arena {
 // otherwise they are the context's.  With an arena every buffer is carved from it.
 struct SAWork {
     DevBuf recs, keys_a, keys_b, vals_a, vals_b, cub_tmp, slots, gtiles, tiles, tilemeta, stops,
-        feats, prof, queue, flag, skip;
+        feats, prof, queue, flag, skip, paint;
     std::vector<TileDev> tile_host;
     std::vector<int64_t> meta_host;
     bool own_tables = false;

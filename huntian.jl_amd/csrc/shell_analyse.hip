@@ -69,6 +69,8 @@ struct SAArgs {
     const float* pred;         // bracket network outputs per global peak id (learned order)
     double rmax2rs;            // > 0: each peak's shells limited to r^2 <= (rmax2rs Rf/alatt)^2
     const int8_t* skip;        // per global peak id: 1 = no collapse without analysis, or null
+    int32_t* paint;            // per global peak id: r^2 of a halo's mask ball (-1: none), set
+                               // for paint_mask_kernel; null: post_process marks the mask
     unsigned long long* qctr;
     int64_t qbeg[8], qend[8];
 };
@@ -715,7 +717,10 @@ __device__ void post_process(const SAArgs& a, const TileDev& T, int p, const dou
         const double* Y = P + (int64_t)sR * PW;
         if (Y[K_LAPN] > 0.0) d2F = Y[K_LAP] / Y[K_LAPN];
     }
-    if (T.mask) {
+    if (T.mask && a.paint) {
+        // shells 1 .. sR are the cells with d^2 <= r^2 of shell sR: paint_mask_kernel marks them
+        a.paint[T.base + p] = a.shell_r2[sR - 1];
+    } else if (T.mask) {
         const int nb = a.nbuff;
         for (int s = 1; s <= sR; ++s) {
             const int st = a.shell_start[s - 1], nc = a.shell_count[s - 1];
@@ -1809,6 +1814,53 @@ void sort_pairs(SAWork& W, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipEr
 // Variant B with rmax2rs > 0 skips a peak whose cell the mask holds beyond
 // floor(Rf / alatt * rmax2rs) (ShellAnalysisGPU.jl:112-117).  The flags are taken from the
 // mask as the call finds it, before any peak of the call marks its halo.
+// The mask of the halos of group g (a wavefront per group of 64 schedule slots, workgroup b
+// taking group (b % 8) * G8 + b / 8): for each halo in turn, with r2 the r^2 of its last shell
+// within RTHL, every cell at d^2 <= r2 from the peak is set to 1 if it lies in the tile's
+// interior (nbuff < index <= n - nbuff on every axis), as the reference marks the cells of
+// shells 1 .. sR (RadialShell.jl:786-797, ShellAnalysisGPU.jl:514-520).  A lane takes one
+// row (dj, dk) of the ball and writes its run of |di| <= sqrt(r2 - dj^2 - dk^2) bytes, so a
+// store touches a row's one or two lines instead of one line per cell of 64 halos.
+__device__ __forceinline__ int isqrt_floor(int v) {
+    int h = (int)sqrtf((float)v);
+    while (h * h > v) --h;
+    while ((h + 1) * (h + 1) <= v) ++h;
+    return h;
+}
+
+__global__ __launch_bounds__(64) void paint_mask_kernel(SAArgs a, int64_t G8) {
+    const int64_t g = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
+    if (g >= a.ngroups) return;
+    const TileDev& T = a.tiles[a.gtile[g]];
+    if (!T.mask) return;
+    const int lane = threadIdx.x;
+    const int myp = a.slot[g * 64 + lane];
+    const int myr2 = myp >= 0 ? a.paint[T.base + myp] : -1;
+    // every lane's peak cell at once, read out per halo below
+    const int mi = myr2 >= 0 ? T.pi[myp] : 0, mj = myr2 >= 0 ? T.pj[myp] : 0;
+    const int mk = myr2 >= 0 ? T.pk[myp] : 0;
+    const int n = T.n, nb = a.nbuff;
+    for (int k = 0; k < 64; ++k) {
+        const int r2 = __builtin_amdgcn_readlane(myr2, k);
+        if (r2 < 0) continue;
+        const int ci = __builtin_amdgcn_readlane(mi, k), cj = __builtin_amdgcn_readlane(mj, k);
+        const int ck = __builtin_amdgcn_readlane(mk, k);
+        const int r = isqrt_floor(r2), side = 2 * r + 1, nrows = side * side;
+        for (int q0 = 0; q0 < nrows; q0 += 64) {
+            const int q = q0 + lane;
+            const int dj = q % side - r, dk = q / side - r;
+            const int rem = r2 - dj * dj - dk * dk;
+            const int j2 = cj + dj, k3 = ck + dk;
+            if (q >= nrows || rem < 0 || j2 <= nb || j2 > n - nb || k3 <= nb || k3 > n - nb)
+                continue;
+            const int h = isqrt_floor(rem);
+            const int lo = max(ci - h, nb + 1), hi = min(ci + h, n - nb);
+            int8_t* row = T.mask + (int64_t)(j2 - 1) * T.s1 + (int64_t)(k3 - 1) * T.s2 - 1;
+            for (int i = lo; i <= hi; ++i) row[i] = 1;
+        }
+    }
+}
+
 __global__ void premask_kernel(const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
                                const int32_t* __restrict__ pk, const double* __restrict__ Rf,
                                int64_t npeaks, int n, const int8_t* __restrict__ mask,
@@ -1842,9 +1894,13 @@ struct LaunchDesc {
     bool kahan = false;
     std::vector<RepackJob> repacks;
     std::vector<PremaskJob> premasks;
+    int64_t ntot = 0;  // peaks of the call (the paint radii)
 };
 
 void launch_premasks(hipStream_t st, const LaunchDesc& d) {
+    if (d.A.paint)
+        hipLaunchKernelGGL(fill_i32, dim3((unsigned)((d.ntot + 255) / 256)), dim3(256), 0, st,
+                           d.A.paint, d.ntot, -1);
     for (const PremaskJob& j : d.premasks)
         hipLaunchKernelGGL(premask_kernel, dim3((unsigned)((j.npeaks + 255) / 256)), dim3(256),
                            0, st, j.pi, j.pj, j.pk, j.Rf, j.npeaks, j.n, j.mask, d.A.rmax2rs,
@@ -1857,6 +1913,12 @@ void launch_repacks(hipStream_t st, const std::vector<RepackJob>& jobs) {
         hipLaunchKernelGGL(repack_kernel, g, dim3(256), 0, st, j.d, j.ex, j.ey, j.ez, j.e2x,
                            j.e2y, j.e2z, j.n, j.recs);
     }
+}
+
+void launch_paint(hipStream_t st, const LaunchDesc& d) {
+    if (!d.A.paint) return;
+    const int64_t G8 = (d.A.ngroups + 7) / 8;
+    hipLaunchKernelGGL(paint_mask_kernel, dim3((unsigned)(8 * G8)), dim3(64), 0, st, d.A, G8);
 }
 
 void launch_main(hipStream_t st, const LaunchDesc& d) {
@@ -2093,6 +2155,13 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
             A.skip = skip;
         }
     }
+    // halos' masks are marked after the analysis, row by row (paint_mask_kernel)
+    bool any_mask = false;
+    for (int t = 0; t < ntiles; ++t) any_mask = any_mask || (masks && masks[t] && batches[t].npeaks);
+    if (any_mask) {
+        HNT_TRY(reserve_in(ctx, W.paint, (size_t)ntot * 4, W.arena));
+        A.paint = (int32_t*)W.paint.p;
+    }
     const void* kern = gang == 64 ? (kahan ? (const void*)shell_wave_kernel<SumKahan>
                                            : (const void*)shell_wave_kernel<SumF64>)
                      : gang == 16 ? (kahan ? (const void*)shell_gang_kernel<SumKahan>
@@ -2179,8 +2248,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     D.kahan = kahan;
     D.repacks = std::move(jobs);
     D.premasks = std::move(premasks);
+    D.ntot = ntot;
     launch_premasks(st, D);
     launch_main(st, D);
+    launch_paint(st, D);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);
     return HNT_OK;
 }
@@ -2243,6 +2314,7 @@ static hnt_status plan_build(hnt_plan* P, int32_t ntiles, const hnt_fields* fiel
     launch_repacks(P->cap, P->D.repacks);
     launch_premasks(P->cap, P->D);
     launch_main(P->cap, P->D);
+    launch_paint(P->cap, P->D);
     HNT_HIP(hipStreamEndCapture(P->cap, &P->graph));
     // the graph must hold kernels only (no memset or copy nodes)
     size_t nn = 0;
