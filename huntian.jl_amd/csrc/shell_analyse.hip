@@ -40,10 +40,13 @@ namespace {
 
 using namespace hnt;
 
-constexpr int PW = 24;  // doubles per shell in a profile slot
+// Doubles per shell in a profile slot: 22 used, padded to two whole 128-B lines (written
+// in full, so that no line of the slot is left half written for the L2 to evict: 24 doubles,
+// 1.5 lines per shell, took the 1024^3 shell kernel 6 ms longer).
+constexpr int PW = 32;
 // [0..2] G/rad  [3..5] Gf/rad  [6..11] SR+SR^T over rad: 11 12 13 22 23 33
 // [12] Fbar  [13] cumulative n  [14..16] cumulative S  [17..19] cumulative S2
-// [20] running lapd sum  [21] running lapd count
+// [20] running lapd sum  [21] running lapd count  [22..31] zero
 constexpr int K_F = 12, K_N = 13, K_S = 14, K_S2 = 17, K_LAP = 20, K_LAPN = 21;
 
 struct SAArgs {
@@ -915,6 +918,8 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
             X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
             X[K_LAP] = lap;
             X[K_LAPN] = (double)lapn;
+#pragma unroll
+            for (int q = K_LAPN + 1; q < PW; ++q) X[q] = 0.0;
             // early exit: with s_c the first shell past ir2min where Fbar < fcrit, the radial
             // logic reads Fbar up to s_c + 1 and kernel_strain inputs of shells with
             // rad < rad[s_c + 1] + 2 (and the filter-scale window up to mupp_rf)
@@ -2178,9 +2183,9 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     A.qctr = (unsigned long long*)W.queue.p;
     const bool sched = A.early && params->order != HNT_ORDER_PLAIN;
     if (sched && W.arena) HNT_TRY(reserve_in(ctx, W.stops, (size_t)ntot * 4, W.arena));
-    // profile scratch: 24 GiB at most, or what is left of a fixed region (fewer resident
+    // profile scratch: 32 GiB at most, or what is left of a fixed region (fewer resident
     // wavefronts then, same results)
-    size_t budget = (size_t)24 << 30;
+    size_t budget = (size_t)32 << 30;
     if (W.arena) {
         const size_t used = (W.arena->used + 255) & ~(size_t)255;
         budget = used < W.arena->cap ? W.arena->cap - used : 0;
