@@ -321,7 +321,8 @@ def main():
     t_pf = stage_ms["pf_mark"] + stage_ms["pf_compact"]
     # shell analysis = validation, repack, first sort, crossing pass and work-ordered sort,
     # gather + radial logic
-    t_sa = stage_ms["sa_prep"] + stage_ms.get("sa_cross", 0.0) + stage_ms["sa_kernel"]
+    t_sa = (stage_ms["sa_prep"] + stage_ms.get("sa_cross", 0.0) + stage_ms["sa_kernel"] +
+            stage_ms.get("sa_mask", 0.0))
     kern = {
         "shell_analyse_kernel": dict(ms=stage_ms["sa_kernel"], bytes=sa_bytes),
         "pf_mark": dict(ms=stage_ms["pf_mark"], bytes=float(len(radii)) * m ** 3 * 4 + m ** 3),

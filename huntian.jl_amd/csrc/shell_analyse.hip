@@ -2256,8 +2256,10 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
     D.ntot = ntot;
     launch_premasks(st, D);
     launch_main(st, D);
-    launch_paint(st, D);
     stage_end(ctx, HNT_STAGE_SA_KERNEL);
+    stage_begin(ctx, HNT_STAGE_SA_MASK);
+    launch_paint(st, D);
+    stage_end(ctx, HNT_STAGE_SA_MASK);
     return HNT_OK;
 }
 

@@ -19,7 +19,7 @@ HNT_ERR_CAPACITY = 3
 HNT_ERR_UNSUPPORTED = 4
 HNT_ERR_ALLOC = 5
 
-STAGES = ("pf_mark", "pf_compact", "sa_prep", "sa_kernel", "sa_cross", "pf_smooth")
+STAGES = ("pf_mark", "pf_compact", "sa_prep", "sa_kernel", "sa_cross", "pf_smooth", "sa_mask")
 
 HNT_VARIANT_A = 0
 HNT_VARIANT_B = 1

@@ -79,7 +79,8 @@ enum {
     HNT_STAGE_SA_KERNEL = 3,    /* shell analysis: gather and radial logic */
     HNT_STAGE_SA_CROSS = 4,     /* shell analysis: crossing pass and work-ordered peak sort */
     HNT_STAGE_PF_SMOOTH = 5,    /* tile peak finding: FFTs and window products */
-    HNT_NSTAGES = 6
+    HNT_STAGE_SA_MASK = 6,      /* shell analysis: marking the halos' cells in the mask */
+    HNT_NSTAGES = 7
 };
 hnt_status hnt_ctx_set_profiling(hnt_ctx* ctx, int on);
 hnt_status hnt_ctx_get_profile(hnt_ctx* ctx, double* ms, int64_t* calls, int32_t nstages);
