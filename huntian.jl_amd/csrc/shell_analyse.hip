@@ -1823,7 +1823,7 @@ void sort_pairs(SAWork& W, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipEr
 // taking group (b % 8) * G8 + b / 8): for each halo in turn, with r2 the r^2 of its last shell
 // within RTHL, every cell at d^2 <= r2 from the peak is set to 1 if it lies in the tile's
 // interior (nbuff < index <= n - nbuff on every axis), as the reference marks the cells of
-// shells 1 .. sR (RadialShell.jl:786-797, ShellAnalysisGPU.jl:514-520).  A lane takes one
+// shells 1 .. sR (RadialShell.jl:773-781, ShellAnalysisGPU.jl:512-527).  A lane takes one
 // row (dj, dk) of the ball and writes its run of |di| <= sqrt(r2 - dj^2 - dk^2) bytes, so a
 // store touches a row's one or two lines instead of one line per cell of 64 halos.
 __device__ __forceinline__ int isqrt_floor(int v) {
