@@ -44,10 +44,14 @@ using namespace hnt;
 // in full, so that no line of the slot is left half written for the L2 to evict: 24 doubles,
 // 1.5 lines per shell, took the 1024^3 shell kernel 6 ms longer).
 constexpr int PW = 32;
+// First line, what the radial logic reads at every shell:
 // [0..2] G/rad  [3..5] Gf/rad  [6..11] SR+SR^T over rad: 11 12 13 22 23 33
-// [12] Fbar  [13] cumulative n  [14..16] cumulative S  [17..19] cumulative S2
-// [20] running lapd sum  [21] running lapd count  [22..31] zero
-constexpr int K_F = 12, K_N = 13, K_S = 14, K_S2 = 17, K_LAP = 20, K_LAPN = 21;
+// [12] Fbar  [13] running lapd sum  [14] running lapd count  [15] zero
+// Second line, read only at the shell below the walk's start m0 (Sbar, Sbar2), which lies
+// below the crossing shell:
+// [16] cumulative n  [17..19] cumulative S  [20..22] cumulative S2  [23..31] zero
+constexpr int K_F = 12, K_LAP = 13, K_LAPN = 14, K_N = 16, K_S = 17, K_S2 = 20;
+constexpr int K_LINE2 = 16;
 
 struct SAArgs {
     const TileDev* tiles;
@@ -866,11 +870,13 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
         double* X = P + PW;
         for (int q = 0; q < 12; ++q) X[q] = 0.0;
         X[K_F] = Fprev;
+        X[K_LAP] = lap;
+        X[K_LAPN] = (double)lapn;
+        X[K_LINE2 - 1] = 0.0;
         X[K_N] = cumN;
         X[K_S] = cS0; X[K_S + 1] = cS1; X[K_S + 2] = cS2;
         X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
-        X[K_LAP] = lap;
-        X[K_LAPN] = (double)lapn;
+        for (int q = K_S2 + 3; q < PW; ++q) X[q] = 0.0;
     }
 
     bool active = valid && !none;
@@ -913,13 +919,18 @@ __device__ void analyse_group(const SAArgs& a, int64_t grp, double* __restrict__
             X[6] = r00 + r00; X[7] = r01 + r10; X[8] = r02 + r20;
             X[9] = r11 + r11; X[10] = r12 + r21; X[11] = r22 + r22;
             X[K_F] = Fb;
-            X[K_N] = cumN;
-            X[K_S] = cS0; X[K_S + 1] = cS1; X[K_S + 2] = cS2;
-            X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
             X[K_LAP] = lap;
             X[K_LAPN] = (double)lapn;
+            X[K_LINE2 - 1] = 0.0;
+            // the second line only up to the crossing shell: the radial logic reads it at
+            // m0 - 1 < crossing shell, and whole lines left out cost no partial writes
+            if (!crossed) {
+                X[K_N] = cumN;
+                X[K_S] = cS0; X[K_S + 1] = cS1; X[K_S + 2] = cS2;
+                X[K_S2] = cT0; X[K_S2 + 1] = cT1; X[K_S2 + 2] = cT2;
 #pragma unroll
-            for (int q = K_LAPN + 1; q < PW; ++q) X[q] = 0.0;
+                for (int q = K_S2 + 3; q < PW; ++q) X[q] = 0.0;
+            }
             // early exit: with s_c the first shell past ir2min where Fbar < fcrit, the radial
             // logic reads Fbar up to s_c + 1 and kernel_strain inputs of shells with
             // rad < rad[s_c + 1] + 2 (and the filter-scale window up to mupp_rf)
@@ -1248,6 +1259,14 @@ __global__ __launch_bounds__(64) void shell_gang_kernel(SAArgs a) {
 // other, then post-processed one lane each.
 constexpr int WCAP = 256, WMAXSH = 32;
 static_assert(WCAP % 8 == 0 && WMAXSH + 2 <= 64, "wave gather chunk shape");
+// Lane e < 22 of the wavefront produces profile entry e in this numbering (Fbar, n, S, S2,
+// lapd sum, lapd count after the twelve kernel_strain inputs); wave_pos gives its place in
+// the slot.
+constexpr int WQ_F = 12, WQ_N = 13, WQ_S = 14, WQ_S2 = 17, WQ_LAP = 20, WQ_LAPN = 21;
+__device__ __forceinline__ int wave_pos(int e) {
+    return e < WQ_F ? e : e == WQ_F ? K_F : e == WQ_N ? K_N : e < WQ_S2 ? K_S + e - WQ_S
+         : e < WQ_LAP ? K_S2 + e - WQ_S2 : e == WQ_LAP ? K_LAP : K_LAPN;
+}
 
 // lane l's double, read by every lane (l uniform)
 __device__ __forceinline__ double rl_double(double v, int l) {
@@ -1360,12 +1379,12 @@ __device__ void wave_gather(const SAArgs& a, const TileDev& T, int p, const Wave
     }
     if (lane < 22) {
         double x = 0.0;
-        if (lane == K_F) x = Fprev;
-        else if (lane == K_N) x = cumN;
-        else if (lane >= K_S && lane < K_LAP) x = run;
-        else if (lane == K_LAP) x = lap;
-        else if (lane == K_LAPN) x = (double)lapn;
-        P[PW + lane] = x;
+        if (lane == WQ_F) x = Fprev;
+        else if (lane == WQ_N) x = cumN;
+        else if (lane >= WQ_S && lane < WQ_LAP) x = run;
+        else if (lane == WQ_LAP) x = lap;
+        else if (lane == WQ_LAPN) x = (double)lapn;
+        P[PW + wave_pos(lane)] = x;
     }
 
     bool past = false, crossed = false, stop_set = false, done = false;
@@ -1484,17 +1503,17 @@ __device__ void wave_gather(const SAArgs& a, const TileDev& T, int p, const Wave
                     const int ib = lane == 6 ? 13 : lane == 7 ? 16 : lane == 8 ? 19
                                  : lane == 9 ? 17 : lane == 10 ? 20 : 21;
                     x = sh[ia] / radS + sh[ib] / radS;
-                } else if (lane == K_F) {
+                } else if (lane == WQ_F) {
                     x = Fb;
-                } else if (lane == K_N) {
+                } else if (lane == WQ_N) {
                     x = cumN;
-                } else if (lane < K_LAP) {
-                    if (nin > 0) run += sh[lane < K_S2 ? 1 + lane - K_S : 4 + lane - K_S2];
+                } else if (lane < WQ_LAP) {
+                    if (nin > 0) run += sh[lane < WQ_S2 ? 1 + lane - WQ_S : 4 + lane - WQ_S2];
                     x = run;
                 } else {
-                    x = sh[lane == K_LAP ? 22 : 23];
+                    x = sh[lane == WQ_LAP ? 22 : 23];
                 }
-                P[(int64_t)s * PW + lane] = x;
+                P[(int64_t)s * PW + wave_pos(lane)] = x;
             }
             if (!past && r2s > K.ir2min) past = true;
             if (past && !crossed && Fb < fcrit) {
