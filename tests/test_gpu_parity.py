@@ -280,6 +280,37 @@ def test_end_to_end_realistic_field(oracle, websky, cosmo64, gang):
         assert np.array_equal(md2.cpu().numpy(), mref2)
 
 
+def test_sbar_read_below_the_crossing(oracle, websky, cosmo64):
+    """Sbar and Sbar2 are the running sums at m0 - 1, the shell below where the walk
+    collapses (RadialShell.jl:738-754), and the walk can collapse anywhere below the crossing
+    shell, also past ir2min.  The shell kernel keeps those sums only for the shells up to the
+    crossing: the 64^3 field holds halos whose m0 - 1 lies past ir2min and two or more shells
+    below the crossing, and their Sbar / Sbar2 equal the oracle's, in both schedules."""
+    f, radii, sm = cosmo64
+    n, alatt, nbuff, nhunt = 64, 2.0, 11, 10
+    pk, _ = oracle.peak_find_multi(sm, [1.5] * len(radii), nbuff, _mask(n))
+    Rf = np.array([radii[q] for q in pk["filt"]])
+    ir2 = np.array([api.ir2min_of(r, alatt) for r in Rf], np.int32)
+    fcrit = api.fsc_of_z(websky, 0.0)
+    t = oracle.shell_tables(nhunt)
+    kw = dict(alatt=alatt, ZZon=1.0, fcrit=fcrit, variant="C", nbuff=nbuff)
+    ref = oracle_run(oracle, f, pk["i"], pk["j"], pk["k"], Rf, ir2, t, websky, **kw)
+    r2 = t["shell_r2"]
+    rad = np.concatenate([[0.0], np.sqrt(r2.astype(np.float64))])  # rad[s], s from 1
+    halo = ref["RTHL"] > 0
+    below = np.searchsorted(rad[1:], ref["RTHL"])  # m0 - 1, m0 the first s with rad >= RTHL
+    sel = halo & (r2[np.maximum(below - 1, 0)] > ir2) & (below <= ref["crossing_shell"] - 2)
+    assert sel.sum() >= 20, sel.sum()
+    stab = api.build_shell_tables(nhunt)
+    for order in ("crossing", "plain"):
+        g = to_host(api.shell_analyse(f, pk["i"], pk["j"], pk["k"], Rf, ir2, stab, websky,
+                                      order=order, **kw))
+        compare(g, ref, f"Sbar below the crossing, {order}")
+        for k in ("Sbar", "Sbar2"):
+            a, b = g[k][: len(sel)][sel], ref[k][sel]
+            assert np.all(np.abs(a - b) <= RTOL * np.maximum(np.abs(b), 1e-30)), (order, k)
+
+
 def test_multirf_mirror(oracle, bumps):
     """analyse_peaks_gpu_cuda_multirf returns the reference NamedTuple fields per batch."""
     ct_np = nontrivial_ctab(make_ctab)
