@@ -8,7 +8,7 @@ logit.  hnt_shell_analyse(order="learned") orders the peaks by hi so that a wave
 lanes finish their gathers together, then runs the exact search inside the bracket: the
 gather until the exact stop rule holds, the walk from the crossing down to lo - 2 first; a
 peak whose gather runs past hi + 2 or whose walk passes lo - 2 without collapsing fails
-verification and is finished by the full scan.  The plain method stays the judge: results
+verification and is finished by the full scan.  The exact rules decide every result: results
 never depend on the prediction.  hnt_results.bracket records each peak's verification.
 
 Training targets come from the plain method itself: the shell holding RTHL (1 without a

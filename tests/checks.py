@@ -46,7 +46,7 @@ def rel_errors(gpu, ref, keys=FLOAT_KEYS):
 
 
 def bar_errors(gpu, ref):
-    """Worst errors, over peaks that collapse on both sides, in the measure the north-star bars
+    """Worst errors, over peaks that collapse on both sides, in the measure the required bars
     are stated in: RTHL and Fbarx relative; a strain entry relative to itself or, if it is
     smaller, to 1e-3 of the peak's largest entry (off-diagonal entries can be ~0); e_v and
     p_v relative to themselves or, near 0, absolutely below 1e-3 (p ranges over [-e, e])."""

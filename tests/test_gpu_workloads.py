@@ -5,13 +5,13 @@ Threefry noise, BBKS P(k) with sigma8 0.81, 1LPT + 2LPT displacements, top-hat f
 threshold 1.5, collapse at z = 0 with HomelTab_websky.dat, alatt 2 Mpc/h.  The fields are
 then copied to the host and the oracle runs on the very same arrays.
 
-Bars (the north-star ones are the peak list and halo count bit-exact, RTHL and overdensity
+Bars (the required ones are the peak list and halo count bit-exact, RTHL and overdensity
 within 1e-5, strain / e_v / p_v within 1e-4 relative): the peak list (i, j, k, filter,
 value) and the mask equal the oracle's; the halo set equals the oracle's; every float field
 within 1e-8 relative for the FP64 sums (they differ from the oracle only by device vs host
 log10 / pow ulps, which stay below 1e-10 on all but a few of millions of peaks).  For the
 compensated-FP32 option the errors against FP64 are printed and the share of halos beyond
-the north-star bars is held below 1e-5.
+the required bars is held below 1e-5.
 
   tile128   128^3, box 256, seed 12345, 16 filters 1.5-20, nbuff 16, hunt radius 15:
             every candidate, variants A (full scan, the reference CPU path) and C (early exit)
@@ -150,7 +150,7 @@ def test_tile128_every_candidate_variants_A_B_and_C(oracle, websky_ct):
         if early:
             assert np.array_equal(g["nshells_used"], ref["nshells_used"])
             assert np.array_equal(g["crossing_shell"], ref["crossing_shell"])
-    # compensated FP32 shell sums against the FP64 oracle (variant C)
+    # compensated FP32 shell sums against the FP64 oracle (variant C): the required bars
     r = api.shell_analyse(w["fields"], pf["i"], pf["j"], pf["k"], Rf, ir2, stab, websky_ct,
                           variant="C", precision="fp32_kahan", **kw)
     g = to_host(r)
