@@ -1841,10 +1841,12 @@ hnt_status check_tile(const hnt_fields* f, const hnt_batch* b, const hnt_results
     return HNT_OK;
 }
 
-void sort_pairs(SAWork& W, size_t tmp_bytes, int64_t ntot, hipStream_t st, hipError_t* e) {
+// Sorts the keys' low end_bit bits (the bits tile_key_kernel can set for the call's tiles).
+void sort_pairs(SAWork& W, size_t tmp_bytes, int64_t ntot, int end_bit, hipStream_t st,
+                hipError_t* e) {
     *e = hipcub::DeviceRadixSort::SortPairs(W.cub_tmp.p, tmp_bytes, (uint64_t*)W.keys_a.p,
                                             (uint64_t*)W.keys_b.p, (int32_t*)W.vals_a.p,
-                                            (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st);
+                                            (int32_t*)W.vals_b.p, (int)ntot, 0, end_bit, st);
 }
 
 // Variant B with rmax2rs > 0 skips a peak whose cell the mask holds beyond
@@ -2109,7 +2111,19 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
                                                (uint64_t*)W.keys_b.p, (int32_t*)W.vals_a.p,
                                                (int32_t*)W.vals_b.p, (int)ntot, 0, 64, st));
     HNT_TRY(reserve_in(ctx, W.cub_tmp, tmp_bytes > 0 ? tmp_bytes : 16, W.arena));
+    // key bits in use: Morton codes of coordinates below 2^cb (3 cb bits), the 11-bit primary
+    // above the 3 L bits of a brick's cells, and the tile index from bit 50 when the call has
+    // several tiles
+    int cb = 0, tb = 0;
+    {
+        int maxn = 1;
+        for (int t = 0; t < ntiles; ++t) maxn = std::max(maxn, fields[t].n1);
+        while ((1 << cb) < maxn) ++cb;
+        while ((1LL << tb) < (int64_t)ntiles) ++tb;
+    }
     auto order_by = [&](const int32_t* const* prim, int L) -> hnt_status {
+        const int key_bits = ntiles > 1 ? std::min(64, 50 + tb)
+                                        : std::min(50, std::max(3 * cb, 3 * L) + 11);
         for (int t = 0; t < ntiles; ++t) {
             const hnt_batch& b = batches[t];
             if (b.npeaks == 0) continue;
@@ -2119,7 +2133,7 @@ hnt_status analyse_tiles(hnt_ctx* ctx, SAWork& W, int ntiles, const hnt_fields* 
                                (int32_t*)W.vals_a.p);
         }
         hipError_t e;
-        sort_pairs(W, tmp_bytes, ntot, st, &e);
+        sort_pairs(W, tmp_bytes, ntot, key_bits, st, &e);
         HNT_HIP(e);
         hipLaunchKernelGGL(fill_i32, dim3((unsigned)((ngroups * 64 + 255) / 256)), dim3(256), 0,
                            st, slot, ngroups * 64, -1);
