@@ -264,11 +264,22 @@ __global__ __launch_bounds__(256) void pf_count(const uint8_t* __restrict__ clai
     c[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * CHUNK;
-    for (int e = threadIdx.x; e < CHUNK; e += 256) {
-        int64_t idx = base + e;
-        if (idx < ncell) {
-            uint8_t v = claim[idx];
+    if (base + CHUNK <= ncell) {
+        // a whole chunk: 16 consecutive claim bytes per thread in one load
+        const uint4 w = *reinterpret_cast<const uint4*>(claim + base + threadIdx.x * 16);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t v = (ws[q >> 2] >> (8 * (q & 3))) & 0xffu;
             if (v < BLOCKED) atomicAdd(&c[v], 1u);
+        }
+    } else {
+        for (int e = threadIdx.x; e < CHUNK; e += 256) {
+            int64_t idx = base + e;
+            if (idx < ncell) {
+                uint8_t v = claim[idx];
+                if (v < BLOCKED) atomicAdd(&c[v], 1u);
+            }
         }
     }
     __syncthreads();
@@ -290,12 +301,20 @@ __global__ __launch_bounds__(256) void pf_scatter(
     // each thread owns 16 consecutive cells of the chunk
     int mine = 0;
     uint8_t v[16];
+    if (base + CHUNK <= ncell) {
+        const uint4 w = *reinterpret_cast<const uint4*>(claim + base + t * 16);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        int64_t idx = base + t * 16 + q;
-        v[q] = idx < ncell ? claim[idx] : NONE;
-        mine += v[q] < BLOCKED;
+        for (int q = 0; q < 16; ++q) v[q] = (uint8_t)(ws[q >> 2] >> (8 * (q & 3)));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int64_t idx = base + t * 16 + q;
+            v[q] = idx < ncell ? claim[idx] : NONE;
+        }
     }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mine += v[q] < BLOCKED;
     // block exclusive scan of the per-thread counts (Hillis-Steele in LDS)
     s_scan[t] = mine;
     __syncthreads();
