@@ -362,19 +362,18 @@ __global__ __launch_bounds__(256) void pf_scatter(
 }
 
 // Claim array of the tile path before its first filter: BLOCKED where the mask is 1.
+// Grid (ceil(m / 256), m, m): workgroup (x, jj, kq) sets claim cells ii = 256 x + t of row
+// (jj, kq) of the inner cube.
 __global__ __launch_bounds__(256) void pf_claim_init(const int8_t* __restrict__ mask, int n,
                                                      int nbuff, uint8_t* __restrict__ claim) {
     const int m = n - 2 * nbuff;
-    const int64_t mm = m, ncell = mm * mm * mm;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const int kq = (int)(c / (mm * mm));
-        const int64_t rem = c - kq * mm * mm;
-        const int jj = (int)(rem / mm), ii = (int)(rem - jj * mm);
-        const int64_t a = (int64_t)(ii + nbuff) + (int64_t)(jj + nbuff) * n +
-                          (int64_t)(kq + nbuff) * n * n;
-        claim[c] = mask[a] == 1 ? BLOCKED : NONE;
-    }
+    const int ii = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (ii >= m) return;
+    const int jj = (int)blockIdx.y, kq = (int)blockIdx.z;
+    const int64_t c = ii + (int64_t)m * (jj + (int64_t)m * kq);
+    const int64_t a = (int64_t)(ii + nbuff) + (int64_t)(jj + nbuff) * n +
+                      (int64_t)(kq + nbuff) * n * n;
+    claim[c] = mask[a] == 1 ? BLOCKED : NONE;
 }
 
 // Window product of _smooth_window_kernel! (ext/CUDAExt.jl:2831-2871) on the rfft spectrum
@@ -685,8 +684,8 @@ extern "C" hnt_status hnt_peak_find_tile(hnt_ctx* ctx, const float* delta, int32
         HNT_FFT(hipfftExecR2C(r2c, (hipfftReal*)field, (hipfftComplex*)spec));
     }
     stage_end(ctx, HNT_STAGE_PF_SMOOTH);
-    hipLaunchKernelGGL(pf_claim_init, dim3(2048), dim3(256), 0, st, (const int8_t*)mask, n,
-                       nbuff, (uint8_t*)ctx->claim.p);
+    hipLaunchKernelGGL(pf_claim_init, dim3((unsigned)((m + 255) / 256), (unsigned)m, (unsigned)m),
+                       dim3(256), 0, st, (const int8_t*)mask, n, nbuff, (uint8_t*)ctx->claim.p);
     hipLaunchKernelGGL(pf_taken_init, dim3(2048), dim3(256), 0, st, (const int8_t*)mask, n,
                        nbuff, NJ, NKG, (uint32_t*)ctx->fb_taken.p);
     const double dk = 2.0 * M_PI / ((double)n * bank->alatt);
